@@ -1,0 +1,168 @@
+/* cholmod_mi355x.h -- C ABI of the MI355X supernodal Cholesky numeric path.
+ *
+ * A drop-in for the numeric half of CHOLMOD's supernodal LL' path: the
+ * symbolic analysis stays on the CPU (bit-identical Super/Lpi/Lpx/Ls), while
+ * the numeric factorization, the triangular solves and FP64 iterative
+ * refinement run on one MI355X with the whole factor resident in HBM.
+ *
+ * Plain C types only: int64_t indices, double values, opaque handles.
+ * Every function returns one of the status codes below, which carry the same
+ * values as CHOLMOD's Common->status (CHOLMOD/Include/cholmod.h:833-840), so a
+ * caller can store them in cholmod_common.status unchanged.
+ *
+ * Reference interfaces replaced (paths relative to the SuiteSparse root):
+ *   mxc_analyze            cholmod_l_analyze_p   CHOLMOD/Include/cholmod.h:2734
+ *                          (CHOLMOD/Cholesky/cholmod_analyze.c:384-909, GIVEN/NATURAL/AMD)
+ *   mxc_symbolic_import    the supernodal fields of cholmod_factor, cholmod.h:1584-1594
+ *   mxc_device_init        cholmod_l_gpu_start / cholmod_l_gpu_allocate  cholmod.h:4148-4180
+ *                          (CHOLMOD/GPU/cholmod_gpu.c:234, :426)
+ *   mxc_factor_create      the per-analysis device set-up done by gpu_init
+ *                          (CHOLMOD/GPU/t_cholmod_gpu.c:84-160) and the relative maps of
+ *                          kernelCreateMap / kernelCreateRelativeMap
+ *                          (CHOLMOD/GPU/cholmod_gpu_kernels.cu:21-41)
+ *   mxc_factorize          cholmod_l_factorize_p cholmod.h:2794 (permutation,
+ *                          CHOLMOD/Cholesky/cholmod_factorize.c:220-250) +
+ *                          cholmod_l_super_numeric cholmod.h:4026
+ *                          (CHOLMOD/Supernodal/t_cholmod_super_numeric_worker.c:335-1279)
+ *   mxc_factor_get_Lx      L->x on return from cholmod_l_factorize (cholmod.h:1584-1594)
+ *   mxc_solve              cholmod_l_solve(CHOLMOD_A, ...) cholmod.h:2823, supernodal branch
+ *                          CHOLMOD/Cholesky/cholmod_solve.c:656-724
+ *   mxc_super_lsolve       cholmod_l_super_lsolve  cholmod.h:4047
+ *   mxc_super_ltsolve      cholmod_l_super_ltsolve cholmod.h:4068
+ *   mxc_residual           cholmod_l_sdmult(A, 0, -1, 1, X, B) used for r = b - A x
+ *                          (CHOLMOD/MatrixOps/cholmod_sdmult.c:62)
+ *   mxc_solve_refine       the iterative-refinement loop of CHOLMOD/Demo/cholmod_dl_demo.c:627-663,
+ *                          extended to a low-precision factor (no single routine in the reference)
+ */
+#ifndef CHOLMOD_MI355X_H
+#define CHOLMOD_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes: same values as CHOLMOD_OK, CHOLMOD_NOT_INSTALLED, ... */
+#define MXC_OK              0
+#define MXC_NOT_INSTALLED (-1)
+#define MXC_OUT_OF_MEMORY (-2)
+#define MXC_TOO_LARGE     (-3)
+#define MXC_INVALID       (-4)
+#define MXC_GPU_PROBLEM   (-5)
+#define MXC_NOT_POSDEF      1
+
+/* orderings (values of CHOLMOD_NATURAL, CHOLMOD_GIVEN, CHOLMOD_AMD) */
+#define MXC_ORDER_NATURAL 0
+#define MXC_ORDER_GIVEN   1
+#define MXC_ORDER_AMD     2
+
+/* working precision of the numeric factor */
+#define MXC_FP64 0
+#define MXC_FP32 1
+
+/* where a value/vector pointer lives */
+#define MXC_HOST   0
+#define MXC_DEVICE 1
+
+typedef struct mxc_symbolic mxc_symbolic;   /* CPU symbolic factor */
+typedef struct mxc_factor mxc_factor;       /* device plan + numeric factor */
+
+/* ---- symbolic analysis (CPU) ------------------------------------------- */
+
+/* Analyze tril(A) (n x n, CSC, Ap[n+1], Ai[Ap[n]], rows i >= j) under an
+ * ordering.  Produces Perm, ColCount, Super, Lpi, Lpx, Ls exactly as
+ * cholmod_l_analyze_p does with default parameters and supernodal = SUPERNODAL. */
+int mxc_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, int ordering,
+                const int64_t *UserPerm, mxc_symbolic **S);
+
+/* info[0..6] = n, nsuper, ssize, xsize, maxcsize, maxesize, ordering;
+ * dinfo[0..1] = fl (Common->fl), lnz (Common->lnz). */
+int mxc_symbolic_info(const mxc_symbolic *S, int64_t *info, double *dinfo);
+
+/* Copy out Perm[n], ColCount[n], Super[nsuper+1], Lpi[nsuper+1], Lpx[nsuper+1],
+ * Ls[ssize] (any pointer may be NULL). */
+int mxc_symbolic_export(const mxc_symbolic *S, int64_t *Perm, int64_t *ColCount,
+                        int64_t *Super, int64_t *Lpi, int64_t *Lpx, int64_t *Ls);
+
+/* Wrap an existing supernodal symbolic factor (e.g. from cholmod_l_analyze). */
+int mxc_symbolic_import(int64_t n, int64_t nsuper, const int64_t *Perm, const int64_t *Super,
+                        const int64_t *Lpi, const int64_t *Lpx, const int64_t *Ls,
+                        mxc_symbolic **S);
+
+void mxc_symbolic_free(mxc_symbolic *S);
+
+/* Approximate minimum degree ordering of tril(A) as used by cholmod_l_analyze
+ * with CHOLMOD_AMD: Perm[k] = original index of the k-th pivot. */
+int mxc_amd_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm);
+
+/* ---- device ------------------------------------------------------------- */
+
+/* Select the HIP device used by subsequent factor objects. */
+int mxc_device_init(int device);
+
+/* info[0..3] = compute units, HBM bytes, gcn arch number (950), clock kHz */
+int mxc_device_info(int64_t *info);
+
+/* ---- numeric factorization (device) ------------------------------------ */
+
+/* Build the device plan for S and the pattern of tril(A) (the pattern the
+ * values passed to mxc_factorize follow).  precision: MXC_FP64 or MXC_FP32. */
+int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const int64_t *Ai,
+                      int precision, mxc_factor **F);
+
+void mxc_factor_free(mxc_factor *F);
+
+/* info[0..7] = n, nsuper, xsize, levels, update pairs, update tiles,
+ *              device bytes held, precision;
+ * dinfo[0..1] = fl, lnz */
+int mxc_factor_info(const mxc_factor *F, int64_t *info, double *dinfo);
+
+/* Numeric factorization of tril(A) + beta*I with values Ax (pattern of
+ * create, unpermuted).  Ax is a host or device pointer (location).  On
+ * return *minor = n on success, else the first failing column, with
+ * MXC_NOT_POSDEF returned (L then holds the factor of the leading minor
+ * columns and zeros after, as CHOLMOD leaves it). */
+int mxc_factorize(mxc_factor *F, const double *Ax, int location, double beta, int64_t *minor);
+
+/* Copy the factor to host memory as double, in CHOLMOD's L->x layout. */
+int mxc_factor_get_Lx(const mxc_factor *F, double *Lx);
+
+/* X = A \ B (sys = CHOLMOD_A): B, X are n x nrhs column-major (ld n), host or
+ * device (both at location).  X may alias B. */
+int mxc_solve(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location);
+
+/* In-place L \ X and L' \ X on vectors already in the permuted order
+ * (cholmod_l_super_lsolve / ltsolve). */
+int mxc_super_lsolve(mxc_factor *F, double *X, int64_t nrhs, int location);
+int mxc_super_ltsolve(mxc_factor *F, double *X, int64_t nrhs, int location);
+
+/* R = B - A X in FP64 with the values of the last mxc_factorize. */
+int mxc_residual(mxc_factor *F, const double *B, const double *X, int64_t nrhs, double *R,
+                 int location);
+
+/* Solve with FP64 iterative refinement: x_0 = A\b through the factor, then
+ * r = b - A x (FP64), x += A\r until ||r||_2 / ||b||_2 <= tol for every
+ * column or maxiter corrections.  out[0] = corrections done, out[1] = final
+ * max relative residual, out[2] = initial max relative residual. */
+int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location,
+                     double tol, int maxiter, double *out);
+
+/* Wait for all work queued on the factor's stream. */
+int mxc_synchronize(mxc_factor *F);
+
+/* Per-phase device times (ms) of the last factorize / solve when timing is
+ * on: t[0] assembly, t[1] updates, t[2] diag potrf, t[3] trsm, t[4] trailing,
+ * t[5] forward solve, t[6] backward solve, t[7] residual, t[8] factor total,
+ * t[9] solve total; counts[0..4] = launches of update, potrf, trsm, trailing,
+ * solve. */
+int mxc_set_timing(mxc_factor *F, int on);
+int mxc_get_timing(const mxc_factor *F, double *t, int64_t *counts);
+
+/* Last HIP error text seen by the library (empty when none). */
+const char *mxc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,226 @@
+// ref_driver.c -- test-side glue around the reference CHOLMOD (CPU path).
+//
+// ORACLE / TEST INFRASTRUCTURE ONLY.  Linked into oracle/_ref/libssref.so
+// together with the reference sources (oracle/ref.mk).  Only tests/, the
+// smoke check and the CPU-baseline leg of bench.py load it; the product never
+// does.
+//
+// It exposes plain-pointer entry points over the reference's own API:
+//   cholmod_l_start / finish                 CHOLMOD/Include/cholmod.h:1028,1035
+//   cholmod_l_analyze / cholmod_l_analyze_p  cholmod.h:2714, :2734
+//   cholmod_l_factorize                      cholmod.h:2775
+//   cholmod_l_solve                          cholmod.h:2823
+//   cholmod_l_sdmult (residual)              cholmod.h (MatrixOps)
+// Input matrices are the lower triangle of a symmetric matrix in CSC form
+// (stype = -1), int64 indices, as CHOLMOD's int64 ("_l_") interface takes.
+
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <omp.h>
+#include "cholmod.h"
+
+extern int MKL_Set_Threading_Layer(int);   // exported by libmkl_rt
+extern void MKL_Set_Num_Threads(int);
+
+typedef struct {
+    cholmod_common cm;
+    cholmod_sparse *A;
+    cholmod_factor *L;
+    int dtype;             // CHOLMOD_DOUBLE or CHOLMOD_SINGLE for the factor
+} ref_handle;
+
+static double now_s(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+static int g_threads_set = 0;
+
+// Pin MKL to the GNU OpenMP layer (MKL_THREADING_GNU == 3) before any BLAS
+// call, and size both OpenMP and MKL thread pools.
+int ref_set_threads(int nthreads) {
+    setenv("MKL_THREADING_LAYER", "GNU", 1);
+    MKL_Set_Threading_Layer(3);
+    if (nthreads > 0) {
+        omp_set_num_threads(nthreads);
+        MKL_Set_Num_Threads(nthreads);
+    }
+    g_threads_set = 1;
+    return omp_get_max_threads();
+}
+
+// ordering: 0 = given (UserPerm), 1 = AMD, 2 = METIS, 3 = natural
+// dtype: 0 = double factor, 1 = single factor (values converted on input)
+void *ref_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax,
+                  int ordering, const int64_t *UserPerm, int single, double *t_analyze) {
+    if (!g_threads_set) ref_set_threads(0);
+    ref_handle *h = (ref_handle *)calloc(1, sizeof(ref_handle));
+    if (!h) return NULL;
+    cholmod_common *cm = &h->cm;
+    cholmod_l_start(cm);
+    cm->useGPU = 0;
+    cm->supernodal = CHOLMOD_SUPERNODAL;
+    cm->final_ll = 0;
+    cm->nmethods = 1;
+    cm->postorder = 1;
+    switch (ordering) {
+        case 0: cm->method[0].ordering = CHOLMOD_GIVEN; break;
+        case 1: cm->method[0].ordering = CHOLMOD_AMD; break;
+        case 2: cm->method[0].ordering = CHOLMOD_METIS; break;
+        default: cm->method[0].ordering = CHOLMOD_NATURAL; break;
+    }
+    int64_t nnz = Ap[n];
+    h->A = cholmod_l_allocate_sparse(n, n, nnz, 1, 1, -1, CHOLMOD_REAL + CHOLMOD_DOUBLE, cm);
+    if (!h->A) { cholmod_l_finish(cm); free(h); return NULL; }
+    memcpy(h->A->p, Ap, sizeof(int64_t) * (n + 1));
+    memcpy(h->A->i, Ai, sizeof(int64_t) * nnz);
+    if (Ax) memcpy(h->A->x, Ax, sizeof(double) * nnz);
+    else memset(h->A->x, 0, sizeof(double) * nnz);
+    if (single) {
+        cholmod_l_sparse_xtype(CHOLMOD_REAL + CHOLMOD_SINGLE, h->A, cm);
+        h->dtype = CHOLMOD_SINGLE;
+    }
+    double t0 = now_s();
+    if (ordering == 0)
+        h->L = cholmod_l_analyze_p(h->A, (int64_t *)UserPerm, NULL, 0, cm);
+    else
+        h->L = cholmod_l_analyze(h->A, cm);
+    if (t_analyze) *t_analyze = now_s() - t0;
+    if (!h->L) {
+        cholmod_l_free_sparse(&h->A, cm);
+        cholmod_l_finish(cm);
+        free(h);
+        return NULL;
+    }
+    return h;
+}
+
+// info[0..9] = n, nsuper, ssize, xsize, maxcsize, maxesize, is_super, ordering, status, minor
+// dinfo[0..1] = fl, lnz
+int ref_info(void *vh, int64_t *info, double *dinfo) {
+    ref_handle *h = (ref_handle *)vh;
+    cholmod_factor *L = h->L;
+    info[0] = (int64_t)L->n;
+    info[1] = (int64_t)L->nsuper;
+    info[2] = (int64_t)L->ssize;
+    info[3] = (int64_t)L->xsize;
+    info[4] = (int64_t)L->maxcsize;
+    info[5] = (int64_t)L->maxesize;
+    info[6] = (int64_t)L->is_super;
+    info[7] = (int64_t)L->ordering;
+    info[8] = (int64_t)h->cm.status;
+    info[9] = (int64_t)L->minor;
+    dinfo[0] = h->cm.fl;
+    dinfo[1] = h->cm.lnz;
+    return 0;
+}
+
+int ref_get_symbolic(void *vh, int64_t *Perm, int64_t *ColCount, int64_t *Super,
+                     int64_t *Lpi, int64_t *Lpx, int64_t *Ls) {
+    ref_handle *h = (ref_handle *)vh;
+    cholmod_factor *L = h->L;
+    size_t n = L->n, ns = L->nsuper;
+    if (Perm) memcpy(Perm, L->Perm, sizeof(int64_t) * n);
+    if (ColCount) memcpy(ColCount, L->ColCount, sizeof(int64_t) * n);
+    if (!L->is_super) return -1;
+    if (Super) memcpy(Super, L->super, sizeof(int64_t) * (ns + 1));
+    if (Lpi) memcpy(Lpi, L->pi, sizeof(int64_t) * (ns + 1));
+    if (Lpx) memcpy(Lpx, L->px, sizeof(int64_t) * (ns + 1));
+    if (Ls) memcpy(Ls, L->s, sizeof(int64_t) * L->ssize);
+    return 0;
+}
+
+// Numeric factorization.  Ax_new (optional) replaces the values of A (same
+// pattern, refactorization).  Returns cholmod status; L->minor in *minor.
+int ref_factorize(void *vh, const double *Ax_new, double *t_factor, int64_t *minor) {
+    ref_handle *h = (ref_handle *)vh;
+    cholmod_common *cm = &h->cm;
+    if (Ax_new) {
+        int64_t nnz = ((int64_t *)h->A->p)[h->A->ncol];
+        if (h->dtype == CHOLMOD_SINGLE) {
+            float *x = (float *)h->A->x;
+            for (int64_t k = 0; k < nnz; k++) x[k] = (float)Ax_new[k];
+        } else {
+            memcpy(h->A->x, Ax_new, sizeof(double) * nnz);
+        }
+    }
+    double t0 = now_s();
+    int ok = cholmod_l_factorize(h->A, h->L, cm);
+    if (t_factor) *t_factor = now_s() - t0;
+    if (minor) *minor = (int64_t)h->L->minor;
+    if (!ok) return cm->status != 0 ? cm->status : -99;
+    return cm->status;
+}
+
+// Copy L->x (supernodal values) out as double.
+int ref_get_Lx(void *vh, double *Lx) {
+    ref_handle *h = (ref_handle *)vh;
+    cholmod_factor *L = h->L;
+    if (!L->is_super || L->xtype == CHOLMOD_PATTERN) return -1;
+    if (h->dtype == CHOLMOD_SINGLE) {
+        float *x = (float *)L->x;
+        for (size_t k = 0; k < L->xsize; k++) Lx[k] = (double)x[k];
+    } else {
+        memcpy(Lx, L->x, sizeof(double) * L->xsize);
+    }
+    return 0;
+}
+
+// X = A \ B with the current factor (sys = CHOLMOD_A).  B, X column-major n x nrhs.
+int ref_solve(void *vh, const double *B, int64_t nrhs, double *X, double *t_solve) {
+    ref_handle *h = (ref_handle *)vh;
+    cholmod_common *cm = &h->cm;
+    int64_t n = (int64_t)h->L->n;
+    int xt = CHOLMOD_REAL + (h->dtype == CHOLMOD_SINGLE ? CHOLMOD_SINGLE : CHOLMOD_DOUBLE);
+    cholmod_dense *Bd = cholmod_l_allocate_dense(n, nrhs, n, xt, cm);
+    if (!Bd) return -2;
+    if (h->dtype == CHOLMOD_SINGLE) {
+        float *b = (float *)Bd->x;
+        for (int64_t k = 0; k < n * nrhs; k++) b[k] = (float)B[k];
+    } else {
+        memcpy(Bd->x, B, sizeof(double) * n * nrhs);
+    }
+    double t0 = now_s();
+    cholmod_dense *Xd = cholmod_l_solve(CHOLMOD_A, h->L, Bd, cm);
+    if (t_solve) *t_solve = now_s() - t0;
+    cholmod_l_free_dense(&Bd, cm);
+    if (!Xd) return cm->status != 0 ? cm->status : -99;
+    if (h->dtype == CHOLMOD_SINGLE) {
+        float *x = (float *)Xd->x;
+        for (int64_t k = 0; k < n * nrhs; k++) X[k] = (double)x[k];
+    } else {
+        memcpy(X, Xd->x, sizeof(double) * n * nrhs);
+    }
+    cholmod_l_free_dense(&Xd, cm);
+    return 0;
+}
+
+// R = B - A*X using the reference's cholmod_l_sdmult on the double-valued A.
+int ref_residual(void *vh, const double *B, const double *X, int64_t nrhs, double *R) {
+    ref_handle *h = (ref_handle *)vh;
+    cholmod_common *cm = &h->cm;
+    if (h->dtype == CHOLMOD_SINGLE) return -1;
+    int64_t n = (int64_t)h->L->n;
+    cholmod_dense *Xd = cholmod_l_allocate_dense(n, nrhs, n, CHOLMOD_REAL, cm);
+    cholmod_dense *Rd = cholmod_l_allocate_dense(n, nrhs, n, CHOLMOD_REAL, cm);
+    if (!Xd || !Rd) return -2;
+    memcpy(Xd->x, X, sizeof(double) * n * nrhs);
+    memcpy(Rd->x, B, sizeof(double) * n * nrhs);
+    double one[2] = {1, 0}, minusone[2] = {-1, 0};
+    cholmod_l_sdmult(h->A, 0, minusone, one, Xd, Rd, cm);
+    memcpy(R, Rd->x, sizeof(double) * n * nrhs);
+    cholmod_l_free_dense(&Xd, cm);
+    cholmod_l_free_dense(&Rd, cm);
+    return 0;
+}
+
+void ref_free(void *vh) {
+    ref_handle *h = (ref_handle *)vh;
+    if (!h) return;
+    cholmod_l_free_factor(&h->L, &h->cm);
+    cholmod_l_free_sparse(&h->A, &h->cm);
+    cholmod_l_finish(&h->cm);
+    free(h);
+}

@@ -1,0 +1,377 @@
+"""CHOLMOD-style host interface over the MI355X supernodal path.
+
+Mirrors the reference's int64 API for this path -- names, argument meaning and
+error behaviour -- on top of the C ABI (include/cholmod_mi355x.h):
+
+  ``analyze(A, cm)``                cholmod_l_analyze     CHOLMOD/Include/cholmod.h:2714
+  ``analyze_p(A, perm, cm)``        cholmod_l_analyze_p   cholmod.h:2734
+  ``factorize(A, L, cm)``           cholmod_l_factorize   cholmod.h:2775
+  ``factorize_p(A, beta, L, cm)``   cholmod_l_factorize_p cholmod.h:2794
+  ``solve(sys, L, B, cm)``          cholmod_l_solve       cholmod.h:2823
+  ``Common``                        cholmod_common: status, fl, lnz, useGPU, print
+  ``Factor``                        cholmod_factor: n, minor, Perm, ColCount, super, pi,
+                                    px, s, x, nsuper, ssize, xsize, maxcsize, maxesize
+
+As in CHOLMOD, a failing call returns False/None and leaves the reason in
+``cm.status`` (the same integer codes, cholmod.h:833-840); a matrix that is not
+positive definite still returns True with ``cm.status == CHOLMOD_NOT_POSDEF``
+and ``L.minor`` set to the failing column (cholmod_super_numeric.c:37-41).
+
+Matrices are :class:`suitesparse_amd.problems.SymMatrix` (lower triangle, CSC,
+int64), i.e. CHOLMOD's ``stype = -1`` storage.  Right-hand sides are numpy
+arrays (host) or ``torch`` CUDA tensors (device-resident, no copies).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+# status codes (cholmod.h:833-840)
+CHOLMOD_OK = 0
+CHOLMOD_NOT_INSTALLED = -1
+CHOLMOD_OUT_OF_MEMORY = -2
+CHOLMOD_TOO_LARGE = -3
+CHOLMOD_INVALID = -4
+CHOLMOD_GPU_PROBLEM = -5
+CHOLMOD_NOT_POSDEF = 1
+
+# orderings (cholmod.h:448-460)
+CHOLMOD_NATURAL = 0
+CHOLMOD_GIVEN = 1
+CHOLMOD_AMD = 2
+
+# solve systems (cholmod.h:2805-2816)
+CHOLMOD_A, CHOLMOD_LDLt, CHOLMOD_LD, CHOLMOD_DLt = 0, 1, 2, 3
+CHOLMOD_L, CHOLMOD_Lt, CHOLMOD_D, CHOLMOD_P, CHOLMOD_Pt = 4, 5, 6, 7, 8
+
+PRECISIONS = {"fp64": _lib.FP64, "fp32": _lib.FP32}
+
+
+@dataclasses.dataclass
+class Common:
+    """The parts of cholmod_common this path reads or writes."""
+    status: int = CHOLMOD_OK
+    fl: float = -1.0
+    lnz: float = -1.0
+    ordering: int = CHOLMOD_AMD        # method[0].ordering with nmethods = 1
+    useGPU: int = 1
+    device: int = 0
+    precision: str = "fp64"            # numeric factor: "fp64" or "fp32" (+ FP64 refinement)
+    print: int = 0
+
+    def __post_init__(self):
+        self._started = False
+
+    def start(self):
+        if not self._started:
+            st = lib().mxc_device_init(int(self.device))
+            if st != 0:
+                self.status = st
+                return False
+            self._started = True
+        return True
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _is_device(a) -> bool:
+    return hasattr(a, "is_cuda") and bool(a.is_cuda)
+
+
+class Factor:
+    """Symbolic + (device-resident) numeric supernodal factor."""
+
+    def __init__(self, sym_handle, A, cm: Common):
+        self._sym = sym_handle
+        self._num = None
+        self._num_precision = None
+        self.A_pattern = (np.ascontiguousarray(A.p, dtype=np.int64),
+                          np.ascontiguousarray(A.i, dtype=np.int64))
+        info = np.zeros(8, dtype=np.int64)
+        dinfo = np.zeros(2)
+        check("mxc_symbolic_info", lib().mxc_symbolic_info(sym_handle, _ptr(info), _ptr(dinfo)))
+        (self.n, self.nsuper, self.ssize, self.xsize, self.maxcsize,
+         self.maxesize, self.ordering) = (int(v) for v in info[:7])
+        self.fl, self.lnz = float(dinfo[0]), float(dinfo[1])
+        n, ns = self.n, self.nsuper
+        self.Perm = np.zeros(n, dtype=np.int64)
+        self.ColCount = np.zeros(n, dtype=np.int64)
+        self.super = np.zeros(ns + 1, dtype=np.int64)
+        self.pi = np.zeros(ns + 1, dtype=np.int64)
+        self.px = np.zeros(ns + 1, dtype=np.int64)
+        self.s = np.zeros(self.ssize, dtype=np.int64)
+        check("mxc_symbolic_export", lib().mxc_symbolic_export(
+            sym_handle, _ptr(self.Perm), _ptr(self.ColCount), _ptr(self.super), _ptr(self.pi),
+            _ptr(self.px), _ptr(self.s)))
+        self.is_super = True
+        self.is_ll = True
+        self.minor = n
+        self.is_numeric = False
+
+    # -- numeric object management
+    def _numeric(self, cm: Common):
+        prec = PRECISIONS[cm.precision]
+        if self._num is not None and self._num_precision == prec:
+            return self._num
+        if self._num is not None:
+            lib().mxc_factor_free(self._num)
+            self._num = None
+        h = C.c_void_p()
+        Ap, Ai = self.A_pattern
+        st = lib().mxc_factor_create(self._sym, _ptr(Ap), _ptr(Ai), prec, C.byref(h))
+        if st != 0:
+            cm.status = st
+            return None
+        self._num = h
+        self._num_precision = prec
+        return h
+
+    @property
+    def x(self) -> np.ndarray:
+        """L->x (copied from the device), double."""
+        if not self.is_numeric:
+            return None
+        out = np.zeros(self.xsize)
+        check("mxc_factor_get_Lx", lib().mxc_factor_get_Lx(self._num, _ptr(out)))
+        return out
+
+    def info(self) -> dict:
+        info = np.zeros(8, dtype=np.int64)
+        dinfo = np.zeros(2)
+        check("mxc_factor_info", lib().mxc_factor_info(self._num, _ptr(info), _ptr(dinfo)))
+        keys = ("n", "nsuper", "xsize", "levels", "pairs", "update_tiles", "device_bytes", "precision")
+        return dict(zip(keys, (int(v) for v in info)))
+
+    def set_timing(self, on: bool):
+        check("mxc_set_timing", lib().mxc_set_timing(self._num, int(on)))
+
+    def timing(self) -> dict:
+        t = np.zeros(10)
+        c = np.zeros(5, dtype=np.int64)
+        check("mxc_get_timing", lib().mxc_get_timing(self._num, _ptr(t), _ptr(c)))
+        keys = ("assembly", "updates", "potrf", "trsm", "trailing", "forward", "backward",
+                "residual", "factor_total", "solve_total")
+        out = {k: float(v) for k, v in zip(keys, t)}
+        out["launches"] = dict(zip(("update", "potrf", "trsm", "trailing", "solve"), (int(v) for v in c)))
+        return out
+
+    def free(self):
+        if self._num is not None:
+            lib().mxc_factor_free(self._num)
+            self._num = None
+        if self._sym is not None:
+            lib().mxc_symbolic_free(self._sym)
+            self._sym = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# analyze
+# ---------------------------------------------------------------------------
+
+def analyze_p(A, perm, cm: Common) -> Factor | None:
+    """Order (GIVEN when perm is not None, else cm.ordering) and analyze tril(A)."""
+    cm.status = CHOLMOD_OK
+    h = C.c_void_p()
+    p = np.ascontiguousarray(A.p, dtype=np.int64)
+    i = np.ascontiguousarray(A.i, dtype=np.int64)
+    if perm is not None:
+        perm = np.ascontiguousarray(perm, dtype=np.int64)
+        if perm.shape != (A.n,):
+            cm.status = CHOLMOD_INVALID
+            return None
+        ordering = CHOLMOD_GIVEN
+    else:
+        ordering = cm.ordering
+    st = lib().mxc_analyze(int(A.n), _ptr(p), _ptr(i), int(ordering), _ptr(perm), C.byref(h))
+    if st != 0:
+        cm.status = st
+        return None
+    L = Factor(h, A, cm)
+    cm.fl, cm.lnz = L.fl, L.lnz
+    return L
+
+
+def analyze(A, cm: Common) -> Factor | None:
+    return analyze_p(A, None, cm)
+
+
+def import_symbolic(A, sym: dict, cm: Common) -> Factor | None:
+    """Adopt a supernodal symbolic factor computed elsewhere (cholmod_factor fields)."""
+    cm.status = CHOLMOD_OK
+    h = C.c_void_p()
+    arrs = [np.ascontiguousarray(sym[k], dtype=np.int64) for k in ("Perm", "Super", "Lpi", "Lpx", "Ls")]
+    st = lib().mxc_symbolic_import(int(A.n), int(len(arrs[1]) - 1), *[_ptr(a) for a in arrs], C.byref(h))
+    if st != 0:
+        cm.status = st
+        return None
+    L = Factor(h, A, cm)
+    L.fl = float(sym.get("fl", L.fl))
+    return L
+
+
+# ---------------------------------------------------------------------------
+# factorize
+# ---------------------------------------------------------------------------
+
+def factorize_p(A, beta, L: Factor, cm: Common, values=None) -> bool:
+    """Numeric factorization of A + beta*I with L's pattern.  ``values`` (a numpy
+    array or CUDA tensor of tril(A)'s values) overrides A.x."""
+    cm.status = CHOLMOD_OK
+    if L is None or A.n != L.n:
+        cm.status = CHOLMOD_INVALID
+        return False
+    if not cm.start():
+        return False
+    h = L._numeric(cm)
+    if h is None:
+        return False
+    x = A.x if values is None else values
+    loc = _lib.DEVICE if _is_device(x) else _lib.HOST
+    if loc == _lib.HOST:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+    b = float(beta[0] if np.ndim(beta) else beta)
+    minor = C.c_int64(0)
+    st = lib().mxc_factorize(h, _ptr(x), loc, b, C.byref(minor))
+    L.minor = int(minor.value)
+    if st < 0:
+        cm.status = st
+        return False
+    cm.status = st          # CHOLMOD_OK or CHOLMOD_NOT_POSDEF (a warning: returns True)
+    L.is_numeric = True
+    return True
+
+
+def factorize(A, L: Factor, cm: Common, values=None) -> bool:
+    return factorize_p(A, 0.0, L, cm, values)
+
+
+# ---------------------------------------------------------------------------
+# solve
+# ---------------------------------------------------------------------------
+
+def solve(sys: int, L: Factor, B, cm: Common):
+    """X = A\\B (sys CHOLMOD_A / LDLt), L\\B (CHOLMOD_L), L'\\B (CHOLMOD_Lt),
+    P*B (CHOLMOD_P) or P'*B (CHOLMOD_Pt).  Returns a new array / tensor."""
+    cm.status = CHOLMOD_OK
+    if L is None or not L.is_numeric:
+        cm.status = CHOLMOD_INVALID
+        return None
+    dev = _is_device(B)
+    if dev:
+        import torch
+        Bc = B if B.dim() == 2 else B.reshape(-1, 1)
+        Bc = Bc.t().contiguous().t() if not Bc.t().is_contiguous() else Bc   # column-major
+        nrhs = Bc.shape[1]
+        X = torch.empty_like(Bc.t().contiguous()).t()
+        bptr, xptr, loc = Bc, X, _lib.DEVICE
+        if Bc.shape[0] != L.n:
+            cm.status = CHOLMOD_INVALID
+            return None
+    else:
+        B2 = np.asarray(B, dtype=np.float64)
+        vec = B2.ndim == 1
+        B2 = B2.reshape(L.n, -1) if vec else B2
+        if B2.shape[0] != L.n:
+            cm.status = CHOLMOD_INVALID
+            return None
+        nrhs = B2.shape[1]
+        Bc = np.asfortranarray(B2)
+        X = np.zeros_like(Bc, order="F")
+        bptr, xptr, loc = Bc, X, _lib.HOST
+    h = L._num
+    if sys in (CHOLMOD_A, CHOLMOD_LDLt):
+        st = lib().mxc_solve(h, _ptr(bptr), nrhs, _ptr(xptr), loc)
+    elif sys in (CHOLMOD_L, CHOLMOD_LD, CHOLMOD_Lt, CHOLMOD_DLt):
+        if dev:
+            X.copy_(Bc)
+        else:
+            X[...] = Bc
+        fn = lib().mxc_super_lsolve if sys in (CHOLMOD_L, CHOLMOD_LD) else lib().mxc_super_ltsolve
+        st = fn(h, _ptr(xptr), nrhs, loc)
+    elif sys == CHOLMOD_D:
+        st = 0
+        if dev:
+            X.copy_(Bc)
+        else:
+            X[...] = Bc
+    elif sys in (CHOLMOD_P, CHOLMOD_Pt):
+        st = 0
+        perm = L.Perm
+        if dev:
+            import torch
+            pt = torch.as_tensor(perm, device=Bc.device)
+            if sys == CHOLMOD_P:
+                X.copy_(Bc[pt])
+            else:
+                X[pt] = Bc
+        else:
+            if sys == CHOLMOD_P:
+                X[...] = Bc[perm]
+            else:
+                X[perm] = Bc
+    else:
+        cm.status = CHOLMOD_INVALID
+        return None
+    if st != 0:
+        cm.status = st
+        return None
+    if not dev and np.ndim(B) == 1:
+        return X[:, 0].copy()
+    return X
+
+
+def solve_refine(L: Factor, B, cm: Common, tol: float = 1e-10, maxiter: int = 10):
+    """x = A\\b with FP64 iterative refinement through the (FP32 or FP64) factor.
+    Returns (X, info) with info = {iterations, relres, relres0}."""
+    cm.status = CHOLMOD_OK
+    dev = _is_device(B)
+    if dev:
+        import torch
+        Bc = B if B.dim() == 2 else B.reshape(-1, 1)
+        Bc = Bc.t().contiguous().t()
+        X = torch.empty_like(Bc.t().contiguous()).t()
+        loc = _lib.DEVICE
+    else:
+        Bc = np.asfortranarray(np.asarray(B, dtype=np.float64).reshape(L.n, -1))
+        X = np.zeros_like(Bc, order="F")
+        loc = _lib.HOST
+    out = np.zeros(3)
+    st = lib().mxc_solve_refine(L._num, _ptr(Bc), Bc.shape[1], _ptr(X), loc, float(tol),
+                                int(maxiter), _ptr(out))
+    if st != 0:
+        cm.status = st
+        return None, None
+    return X, {"iterations": int(out[0]), "relres": float(out[1]), "relres0": float(out[2])}
+
+
+def residual(L: Factor, B, X):
+    """R = B - A X in FP64 on the device (values of the last factorization)."""
+    B2 = np.asfortranarray(np.asarray(B, dtype=np.float64).reshape(L.n, -1))
+    X2 = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(L.n, -1))
+    R = np.zeros_like(B2, order="F")
+    check("mxc_residual", lib().mxc_residual(L._num, _ptr(B2), _ptr(X2), B2.shape[1], _ptr(R), _lib.HOST))
+    return R
+
+
+def device_info() -> dict:
+    info = np.zeros(4, dtype=np.int64)
+    check("mxc_device_info", lib().mxc_device_info(_ptr(info)))
+    return {"cus": int(info[0]), "hbm_bytes": int(info[1]), "arch": f"gfx{int(info[2])}",
+            "clock_khz": int(info[3])}

@@ -1,0 +1,1076 @@
+// mxc_device.hip -- device plan, launches and the C ABI of the MI355X
+// supernodal Cholesky numeric path (see include/cholmod_mi355x.h).
+//
+// Schedule.  The supernodal elimination tree is cut into levels by height
+// (leaves = 0).  For level l, in order:
+//   1. every (descendant d, ancestor s) update with s in level l, as 64x64
+//      MFMA tiles fused with the relative-map scatter-subtract into s
+//      (one launch for the whole level);
+//   2. for t = 0, 1, ...: POTRF + inverse of diagonal block t of every
+//      supernode of the level that has one, TRSM of the rows below it through
+//      the inverse (MFMA), and the in-supernode trailing update (MFMA).
+// A descendant's update lands after all of its own updates are done because
+// d sits in a lower level than s.  The reference instead walks supernodes one
+// at a time (CHOLMOD/Supernodal/t_cholmod_super_numeric_worker.c:335) and, on
+// its GPU path, ships each descendant over PCIe (CHOLMOD/GPU/t_cholmod_gpu.c).
+//
+// Summation order differs from the reference (atomic scatter-adds of
+// independent descendants), so values agree to rounding, not bitwise.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cholmod_mi355x.h"
+#include "kernels_factor.h"
+#include "kernels_solve.h"
+#include "symbolic_internal.h"
+
+using namespace mxc;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+
+static int set_hip_err(hipError_t e, const char *what, int line) {
+    snprintf(g_err, sizeof(g_err), "%s failed at mxc_device.hip:%d: %s", what, line,
+             hipGetErrorString(e));
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? MXC_OUT_OF_MEMORY
+                                                                       : MXC_GPU_PROBLEM;
+}
+#define HIPTRY(x)                                              \
+    do {                                                       \
+        hipError_t e_ = (x);                                   \
+        if (e_ != hipSuccess) return set_hip_err(e_, #x, __LINE__); \
+    } while (0)
+
+extern "C" const char *mxc_last_error(void) { return g_err; }
+
+static int g_device = 0;
+
+extern "C" int mxc_device_init(int device) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        snprintf(g_err, sizeof(g_err), "no HIP device visible");
+        return MXC_GPU_PROBLEM;
+    }
+    if (device < 0 || device >= count) return MXC_INVALID;
+    HIPTRY(hipSetDevice(device));
+    g_device = device;
+    return MXC_OK;
+}
+
+extern "C" int mxc_device_info(int64_t *info) {
+    hipDeviceProp_t p;
+    HIPTRY(hipGetDeviceProperties(&p, g_device));
+    info[0] = p.multiProcessorCount;
+    info[1] = (int64_t)p.totalGlobalMem;
+    info[2] = 950;   // built for gfx950 only
+    info[3] = p.clockRate;
+    return MXC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+template <typename X>
+struct DBuf {
+    X *p = nullptr;
+    size_t n = 0;
+};
+
+struct Launch {
+    int kind;          // see enum below
+    long long begin;   // first task
+    long long count;   // number of tasks (grid size)
+};
+enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FDIAG = 4, L_FOFF = 5, L_BOFF = 6, L_BDIAG = 7 };
+
+struct mxc_factor {
+    int precision = MXC_FP64;
+    int64_t n = 0, nsuper = 0, xsize = 0, nnz_a = 0, nnz_full = 0;
+    int64_t wsize = 0;          // xsize + inverse blocks
+    int nlevels = 0;
+    int64_t npairs = 0, nupd_tiles = 0;
+    double fl = 0, lnz = 0;
+    hipStream_t stream = nullptr;
+    size_t bytes = 0;
+
+    // host copies
+    std::vector<int64_t> Super, Lpi, Lpx, Perm;
+    std::vector<int> level_of;  // per supernode
+
+    // device
+    void *W = nullptr;                  // T[wsize]
+    int *d_Ls = nullptr;
+    long long *d_pos = nullptr, *d_dpos = nullptr, *d_perm = nullptr;
+    UpdDesc *d_desc = nullptr;
+    Tile *d_tiles = nullptr;
+    DiagTask *d_diag = nullptr;
+    SolveTask *d_stask = nullptr;
+    int *d_fail = nullptr;
+    long long *d_rowptr = nullptr, *d_src = nullptr;
+    int *d_col = nullptr;
+    double *d_vals = nullptr, *d_Ax = nullptr;
+    double *d_norm = nullptr;
+    void *d_work = nullptr;             // T[n * work_nrhs] solve vector
+    double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr;   // FP64 n * work_nrhs
+    int64_t work_nrhs = 0;
+
+    std::vector<Launch> factor_launches;   // per level: UPD, then (DIAG, TRSM, TRAIL)*
+    std::vector<Launch> fwd_launches, bwd_launches;
+    bool values_loaded = false;
+
+    // timing
+    bool timing = false;
+    double t_ms[10] = {0};
+    int64_t counts[5] = {0};
+    std::vector<hipEvent_t> ev;
+};
+
+template <typename X>
+static int dalloc(mxc_factor *F, X **p, size_t n) {
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(X));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_hip_err(e, "hipMalloc", __LINE__);
+    }
+    F->bytes += n * sizeof(X);
+    return MXC_OK;
+}
+
+template <typename X>
+static int dupload(mxc_factor *F, X **p, const std::vector<X> &h, hipStream_t s) {
+    int rc = dalloc(F, p, h.size());
+    if (rc) return rc;
+    if (!h.empty()) HIPTRY(hipMemcpyAsync(*p, h.data(), h.size() * sizeof(X), hipMemcpyHostToDevice, s));
+    return MXC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plan construction (host)
+// ---------------------------------------------------------------------------
+static inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// local row of global row g inside supernode s (rows Ls[psi..psi+nsrow) sorted,
+// first nscol of them = k1..k2-1)
+static int64_t local_row(const mxc_symbolic *S, int64_t s, int64_t g) {
+    int64_t k1 = S->Super[s], k2 = S->Super[s + 1];
+    if (g >= k1 && g < k2) return g - k1;
+    int64_t lo = S->Lpi[s] + (k2 - k1), hi = S->Lpi[s + 1];
+    const int64_t *it = std::lower_bound(S->Ls + lo, S->Ls + hi, g);
+    if (it == S->Ls + hi || *it != g) return -1;
+    return (int64_t)(it - S->Ls) - S->Lpi[s];
+}
+
+template <typename T>
+static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, const int64_t *Ai) {
+    const int64_t n = S->n, ns = S->nsuper;
+    F->n = n;
+    F->nsuper = ns;
+    F->xsize = S->xsize;
+    F->fl = S->fl;
+    F->lnz = S->lnz;
+    F->Super.assign(S->Super, S->Super + ns + 1);
+    F->Lpi.assign(S->Lpi, S->Lpi + ns + 1);
+    F->Lpx.assign(S->Lpx, S->Lpx + ns + 1);
+    F->Perm.assign(S->Perm, S->Perm + n);
+    if (n >= (int64_t)INT32_MAX || S->ssize >= (int64_t)INT32_MAX) return MXC_TOO_LARGE;
+
+    std::vector<int64_t> SuperMap(n);
+    for (int64_t s = 0; s < ns; s++)
+        for (int64_t k = S->Super[s]; k < S->Super[s + 1]; k++) SuperMap[k] = s;
+    std::vector<int64_t> Pinv(n);
+    for (int64_t k = 0; k < n; k++) Pinv[S->Perm[k]] = k;
+
+    // ---- value positions of tril(A) inside L (permutation folded in)
+    const int64_t nnz = Ap[n];
+    F->nnz_a = nnz;
+    std::vector<long long> pos(nnz > 0 ? nnz : 1, 0);
+    for (int64_t j = 0; j < n; j++) {
+        for (int64_t p = Ap[j]; p < Ap[j + 1]; p++) {
+            int64_t i = Ai[p];
+            if (i < j) {   // strictly upper entries are ignored, as CHOLMOD does for stype < 0
+                pos[p] = F->xsize;   // dump slot (the first inverse-block entry, rewritten later)
+                continue;
+            }
+            int64_t a = Pinv[i], b = Pinv[j];
+            int64_t row = std::max(a, b), col = std::min(a, b);
+            int64_t s = SuperMap[col];
+            int64_t lr = local_row(S, s, row);
+            if (lr < 0) return MXC_INVALID;   // pattern of A is not inside the analyzed L
+            int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
+            pos[p] = S->Lpx[s] + lr + (col - S->Super[s]) * nsrow;
+        }
+    }
+    std::vector<long long> dpos(n > 0 ? n : 1, 0);
+    for (int64_t k = 0; k < n; k++) {
+        int64_t s = SuperMap[k];
+        int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
+        int64_t c = k - S->Super[s];
+        dpos[k] = S->Lpx[s] + c + c * nsrow;
+    }
+
+    // ---- full CSR of A for the FP64 residual
+    std::vector<long long> rowptr(n + 1, 0);
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t p = Ap[j]; p < Ap[j + 1]; p++) {
+            int64_t i = Ai[p];
+            if (i < j) continue;
+            rowptr[i + 1]++;
+            if (i != j) rowptr[j + 1]++;
+        }
+    for (int64_t i = 0; i < n; i++) rowptr[i + 1] += rowptr[i];
+    F->nnz_full = rowptr[n];
+    std::vector<int> colv(F->nnz_full > 0 ? F->nnz_full : 1);
+    std::vector<long long> src(F->nnz_full > 0 ? F->nnz_full : 1);
+    {
+        std::vector<long long> w(rowptr.begin(), rowptr.end() - 1);
+        for (int64_t j = 0; j < n; j++)
+            for (int64_t p = Ap[j]; p < Ap[j + 1]; p++) {
+                int64_t i = Ai[p];
+                if (i < j) continue;
+                long long q = w[i]++;
+                colv[q] = (int)j;
+                src[q] = p;
+                if (i != j) {
+                    q = w[j]++;
+                    colv[q] = (int)i;
+                    src[q] = p;
+                }
+            }
+    }
+
+    // ---- supernodal etree and levels (height from the leaves)
+    std::vector<int64_t> sparent(ns, -1);
+    std::vector<int> level(ns, 0);
+    for (int64_t s = 0; s < ns; s++) {
+        int64_t nscol = S->Super[s + 1] - S->Super[s], nsrow = S->Lpi[s + 1] - S->Lpi[s];
+        if (nsrow > nscol) sparent[s] = SuperMap[S->Ls[S->Lpi[s] + nscol]];
+    }
+    int nlev = ns > 0 ? 1 : 0;
+    for (int64_t s = 0; s < ns; s++) {
+        if (sparent[s] >= 0) level[sparent[s]] = std::max(level[sparent[s]], level[s] + 1);
+        nlev = std::max(nlev, level[s] + 1);
+    }
+    F->nlevels = nlev;
+    F->level_of = level;
+    std::vector<std::vector<int64_t>> by_level(nlev);
+    for (int64_t s = 0; s < ns; s++) by_level[level[s]].push_back(s);
+
+    // ---- inverse-block slots: block (s,t) of width nb -> nb*nb entries
+    std::vector<int64_t> winv_base(ns + 1, 0);
+    int64_t woff = F->xsize;
+    for (int64_t s = 0; s < ns; s++) {
+        int64_t nscol = S->Super[s + 1] - S->Super[s];
+        winv_base[s] = woff;
+        for (int64_t c0 = 0; c0 < nscol; c0 += NB) {
+            int64_t nb = std::min<int64_t>(NB, nscol - c0);
+            woff += nb * nb;
+        }
+    }
+    F->wsize = woff;
+
+    // ---- (d, s) update pairs, grouped by the level of s
+    std::vector<std::vector<UpdDesc>> pairs_by_level(nlev);
+    int64_t npairs = 0;
+    for (int64_t d = 0; d < ns; d++) {
+        int64_t ndcol = S->Super[d + 1] - S->Super[d];
+        int64_t pdi = S->Lpi[d], pdend = S->Lpi[d + 1], ndrow = pdend - pdi;
+        int64_t q = pdi + ndcol;
+        while (q < pdend) {
+            int64_t s = SuperMap[S->Ls[q]];
+            int64_t k2 = S->Super[s + 1];
+            int64_t q2 = q;
+            while (q2 < pdend && S->Ls[q2] < k2) q2++;
+            UpdDesc u;
+            memset(&u, 0, sizeof(u));
+            int64_t p = q - pdi;   // Lpos: first row of d inside s
+            u.x_off = S->Lpx[d] + p;
+            u.y_off = u.x_off;
+            u.ldx = u.ldy = (int)ndrow;
+            u.K = (int)ndcol;
+            u.M = (int)(pdend - q);       // ndrow2
+            u.N = (int)(q2 - q);          // ndrow1
+            u.c_off = S->Lpx[s];
+            u.ldc = (int)(S->Lpi[s + 1] - S->Lpi[s]);
+            u.ls_off = (int)q;
+            u.base = (int)S->Super[s];
+            u.t_psi = (int)S->Lpi[s];
+            u.t_nscol = (int)(k2 - S->Super[s]);
+            u.t_nsrow = u.ldc;
+            pairs_by_level[level[s]].push_back(u);
+            npairs++;
+            q = q2;
+        }
+    }
+    F->npairs = npairs;
+
+    // ---- flatten: descriptors, tiles, diag tasks, launches
+    std::vector<UpdDesc> descs;
+    std::vector<Tile> tiles;
+    std::vector<DiagTask> diags;
+    F->factor_launches.clear();
+    auto add_tiles = [&](const UpdDesc &u, bool lower_only) {
+        int di = (int)descs.size();
+        descs.push_back(u);
+        int64_t mt = ceil_div(u.M, TILE), nt = ceil_div(u.N, TILE);
+        for (int64_t ti = 0; ti < mt; ti++)
+            for (int64_t tj = 0; tj < nt; tj++) {
+                if (lower_only && ti < tj) continue;   // tile wholly above the diagonal
+                tiles.push_back(Tile{di, (int)ti, (int)tj});
+            }
+    };
+    int64_t nupd = 0;
+    for (int l = 0; l < nlev; l++) {
+        long long b = (long long)tiles.size();
+        for (const UpdDesc &u : pairs_by_level[l]) add_tiles(u, true);
+        long long c = (long long)tiles.size() - b;
+        nupd += c;
+        if (c > 0) F->factor_launches.push_back(Launch{L_UPD, b, c});
+        int64_t maxsteps = 0;
+        for (int64_t s : by_level[l])
+            maxsteps = std::max(maxsteps, ceil_div(S->Super[s + 1] - S->Super[s], NB));
+        for (int64_t t = 0; t < maxsteps; t++) {
+            long long db = (long long)diags.size();
+            long long tb = (long long)tiles.size();
+            std::vector<UpdDesc> trail;
+            for (int64_t s : by_level[l]) {
+                int64_t nscol = S->Super[s + 1] - S->Super[s];
+                int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
+                int64_t c0 = t * NB;
+                if (c0 >= nscol) continue;
+                int64_t nb = std::min<int64_t>(NB, nscol - c0);
+                int64_t psx = S->Lpx[s];
+                int64_t wv = winv_base[s];
+                for (int64_t cc = 0; cc < c0; cc += NB) {
+                    int64_t b2 = std::min<int64_t>(NB, nscol - cc);
+                    wv += b2 * b2;
+                }
+                DiagTask dt;
+                dt.off = psx + c0 + c0 * nsrow;
+                dt.winv = wv;
+                dt.ld = (int)nsrow;
+                dt.nb = (int)nb;
+                dt.col0 = (int)(S->Super[s] + c0);
+                dt.pad = 0;
+                diags.push_back(dt);
+                int64_t r0 = c0 + nb;
+                if (nsrow > r0) {
+                    UpdDesc u;
+                    memset(&u, 0, sizeof(u));
+                    u.x_off = wv;
+                    u.ldx = (int)nb;
+                    u.y_off = psx + r0 + c0 * nsrow;
+                    u.ldy = (int)nsrow;
+                    u.K = (int)nb;
+                    u.M = (int)(nsrow - r0);
+                    u.N = (int)nb;
+                    u.c_off = u.y_off;
+                    u.ldc = (int)nsrow;
+                    add_tiles(u, false);
+                }
+                if (nscol > r0) {
+                    UpdDesc u;
+                    memset(&u, 0, sizeof(u));
+                    u.x_off = psx + r0 + c0 * nsrow;
+                    u.y_off = u.x_off;
+                    u.ldx = u.ldy = (int)nsrow;
+                    u.K = (int)nb;
+                    u.M = (int)(nsrow - r0);
+                    u.N = (int)(nscol - r0);
+                    u.c_off = psx + r0 + r0 * nsrow;
+                    u.ldc = (int)nsrow;
+                    trail.push_back(u);
+                }
+            }
+            long long dc = (long long)diags.size() - db;
+            long long tc = (long long)tiles.size() - tb;
+            if (dc > 0) F->factor_launches.push_back(Launch{L_DIAG, db, dc});
+            if (tc > 0) F->factor_launches.push_back(Launch{L_TRSM, tb, tc});
+            long long rb = (long long)tiles.size();
+            for (const UpdDesc &u : trail) add_tiles(u, true);
+            long long rc = (long long)tiles.size() - rb;
+            if (rc > 0) F->factor_launches.push_back(Launch{L_TRAIL, rb, rc});
+        }
+    }
+    F->nupd_tiles = nupd;
+
+    // ---- solve tasks: forward (levels up, steps up), backward (levels down, steps down)
+    std::vector<SolveTask> st;
+    F->fwd_launches.clear();
+    F->bwd_launches.clear();
+    const int ROWS = 256;
+    auto make_task = [&](int64_t s, int64_t t, int64_t r0, int64_t r1) {
+        SolveTask x;
+        int64_t nscol = S->Super[s + 1] - S->Super[s];
+        x.lx = S->Lpx[s];
+        int64_t wv = winv_base[s];
+        for (int64_t cc = 0; cc < t * NB; cc += NB) {
+            int64_t b2 = std::min<int64_t>(NB, nscol - cc);
+            wv += b2 * b2;
+        }
+        x.winv = wv;
+        x.nsrow = (int)(S->Lpi[s + 1] - S->Lpi[s]);
+        x.nscol = (int)nscol;
+        x.k1 = (int)S->Super[s];
+        x.psi = (int)S->Lpi[s];
+        x.c0 = (int)(t * NB);
+        x.nb = (int)std::min<int64_t>(NB, nscol - t * NB);
+        x.r0 = (int)r0;
+        x.r1 = (int)r1;
+        return x;
+    };
+    for (int l = 0; l < nlev; l++) {
+        int64_t maxsteps = 0;
+        for (int64_t s : by_level[l])
+            maxsteps = std::max(maxsteps, ceil_div(S->Super[s + 1] - S->Super[s], NB));
+        for (int64_t t = 0; t < maxsteps; t++) {
+            long long b = (long long)st.size();
+            for (int64_t s : by_level[l])
+                if (t * NB < S->Super[s + 1] - S->Super[s]) st.push_back(make_task(s, t, 0, 0));
+            if ((long long)st.size() > b) F->fwd_launches.push_back(Launch{L_FDIAG, b, (long long)st.size() - b});
+            b = (long long)st.size();
+            for (int64_t s : by_level[l]) {
+                int64_t nscol = S->Super[s + 1] - S->Super[s];
+                int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
+                if (t * NB >= nscol) continue;
+                int64_t r0 = std::min<int64_t>(nscol, t * NB + NB);
+                for (int64_t r = r0; r < nsrow; r += ROWS)
+                    st.push_back(make_task(s, t, r, std::min<int64_t>(nsrow, r + ROWS)));
+            }
+            if ((long long)st.size() > b) F->fwd_launches.push_back(Launch{L_FOFF, b, (long long)st.size() - b});
+        }
+    }
+    const int BROWS = 1024;
+    for (int l = nlev - 1; l >= 0; l--) {
+        int64_t maxsteps = 0;
+        for (int64_t s : by_level[l])
+            maxsteps = std::max(maxsteps, ceil_div(S->Super[s + 1] - S->Super[s], NB));
+        for (int64_t t = maxsteps - 1; t >= 0; t--) {
+            long long b = (long long)st.size();
+            for (int64_t s : by_level[l]) {
+                int64_t nscol = S->Super[s + 1] - S->Super[s];
+                int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
+                if (t * NB >= nscol) continue;
+                int64_t r0 = std::min<int64_t>(nscol, t * NB + NB);
+                for (int64_t r = r0; r < nsrow; r += BROWS)
+                    st.push_back(make_task(s, t, r, std::min<int64_t>(nsrow, r + BROWS)));
+            }
+            if ((long long)st.size() > b) F->bwd_launches.push_back(Launch{L_BOFF, b, (long long)st.size() - b});
+            b = (long long)st.size();
+            for (int64_t s : by_level[l])
+                if (t * NB < S->Super[s + 1] - S->Super[s]) st.push_back(make_task(s, t, 0, 0));
+            if ((long long)st.size() > b) F->bwd_launches.push_back(Launch{L_BDIAG, b, (long long)st.size() - b});
+        }
+    }
+
+    // ---- device upload
+    std::vector<int> Ls32(S->ssize);
+    for (int64_t k = 0; k < S->ssize; k++) Ls32[k] = (int)S->Ls[k];
+    std::vector<long long> perm64(S->Perm, S->Perm + n);
+    hipStream_t sm = F->stream;
+    int rc;
+    if ((rc = dalloc(F, (T **)&F->W, (size_t)F->wsize))) return rc;
+    if ((rc = dupload(F, &F->d_Ls, Ls32, sm))) return rc;
+    if ((rc = dupload(F, &F->d_pos, pos, sm))) return rc;
+    if ((rc = dupload(F, &F->d_dpos, dpos, sm))) return rc;
+    if ((rc = dupload(F, &F->d_perm, perm64, sm))) return rc;
+    if ((rc = dupload(F, &F->d_desc, descs, sm))) return rc;
+    if ((rc = dupload(F, &F->d_tiles, tiles, sm))) return rc;
+    if ((rc = dupload(F, &F->d_diag, diags, sm))) return rc;
+    if ((rc = dupload(F, &F->d_stask, st, sm))) return rc;
+    if ((rc = dupload(F, &F->d_rowptr, rowptr, sm))) return rc;
+    if ((rc = dupload(F, &F->d_col, colv, sm))) return rc;
+    if ((rc = dupload(F, &F->d_src, src, sm))) return rc;
+    if ((rc = dalloc(F, &F->d_vals, (size_t)std::max<int64_t>(F->nnz_full, 1)))) return rc;
+    if ((rc = dalloc(F, &F->d_Ax, (size_t)std::max<int64_t>(nnz, 1)))) return rc;
+    if ((rc = dalloc(F, &F->d_fail, 1))) return rc;
+    if ((rc = dalloc(F, &F->d_norm, 64))) return rc;
+    HIPTRY(hipStreamSynchronize(sm));
+    return MXC_OK;
+}
+
+static int ensure_work(mxc_factor *F, int64_t nrhs) {
+    if (F->work_nrhs >= nrhs) return MXC_OK;
+    size_t esz = F->precision == MXC_FP64 ? sizeof(double) : sizeof(float);
+    if (F->d_work) { (void)hipFree(F->d_work); F->d_work = nullptr; }
+    if (F->d_b) { (void)hipFree(F->d_b); F->d_b = nullptr; }
+    if (F->d_x) { (void)hipFree(F->d_x); F->d_x = nullptr; }
+    if (F->d_r) { (void)hipFree(F->d_r); F->d_r = nullptr; }
+    size_t len = (size_t)std::max<int64_t>(F->n, 1) * (size_t)nrhs;
+    HIPTRY(hipMalloc(&F->d_work, len * esz));
+    HIPTRY(hipMalloc((void **)&F->d_b, len * sizeof(double)));
+    HIPTRY(hipMalloc((void **)&F->d_x, len * sizeof(double)));
+    HIPTRY(hipMalloc((void **)&F->d_r, len * sizeof(double)));
+    F->work_nrhs = nrhs;
+    return MXC_OK;
+}
+
+extern "C" void mxc_factor_free(mxc_factor *F) {
+    if (!F) return;
+    void *ptrs[] = {F->W, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
+                    F->d_diag, F->d_stask, F->d_fail, F->d_rowptr, F->d_src, F->d_col,
+                    F->d_vals, F->d_Ax, F->d_norm, F->d_work, F->d_b, F->d_x, F->d_r};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
+    if (F->stream) (void)hipStreamDestroy(F->stream);
+    delete F;
+}
+
+extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const int64_t *Ai,
+                                 int precision, mxc_factor **out) {
+    if (!S || !out || (S->n > 0 && (!Ap || !Ai)) || !S->Super || !S->Lpi || !S->Lpx || !S->Ls ||
+        !S->Perm)
+        return MXC_INVALID;
+    if (precision != MXC_FP64 && precision != MXC_FP32) return MXC_INVALID;
+    *out = nullptr;
+    mxc_factor *F = new (std::nothrow) mxc_factor();
+    if (!F) return MXC_OUT_OF_MEMORY;
+    F->precision = precision;
+    hipError_t e = hipSetDevice(g_device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        int rc = set_hip_err(e, "hipStreamCreate", __LINE__);
+        delete F;
+        return rc;
+    }
+    int rc = precision == MXC_FP64 ? build_plan<double>(F, S, Ap, Ai) : build_plan<float>(F, S, Ap, Ai);
+    if (rc != MXC_OK) {
+        mxc_factor_free(F);
+        return rc;
+    }
+    *out = F;
+    return MXC_OK;
+}
+
+extern "C" int mxc_factor_info(const mxc_factor *F, int64_t *info, double *dinfo) {
+    if (!F) return MXC_INVALID;
+    if (info) {
+        info[0] = F->n;
+        info[1] = F->nsuper;
+        info[2] = F->xsize;
+        info[3] = F->nlevels;
+        info[4] = F->npairs;
+        info[5] = F->nupd_tiles;
+        info[6] = (int64_t)F->bytes;
+        info[7] = F->precision;
+    }
+    if (dinfo) {
+        dinfo[0] = F->fl;
+        dinfo[1] = F->lnz;
+    }
+    return MXC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// timing helpers: an event pair around each launch when timing is on
+// ---------------------------------------------------------------------------
+struct TimedRegion {
+    mxc_factor *F;
+    int cls;
+    size_t slot;
+};
+static std::vector<std::pair<int, size_t>> g_pending;   // (class, first event index)
+
+static void tick(mxc_factor *F, size_t *slot) {
+    if (!F->timing) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, F->stream);
+    F->ev.push_back(e);
+    *slot = F->ev.size() - 1;
+}
+
+static void tock(mxc_factor *F, int cls, size_t start) {
+    if (!F->timing) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, F->stream);
+    F->ev.push_back(e);
+    g_pending.push_back({cls, start});
+}
+
+static void collect_timing(mxc_factor *F) {
+    if (!F->timing) return;
+    (void)hipStreamSynchronize(F->stream);
+    for (auto &pr : g_pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, F->ev[pr.second], F->ev[pr.second + 1]) == hipSuccess)
+            F->t_ms[pr.first] += ms;
+    }
+    g_pending.clear();
+    for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
+    F->ev.clear();
+}
+
+extern "C" int mxc_set_timing(mxc_factor *F, int on) {
+    if (!F) return MXC_INVALID;
+    collect_timing(F);
+    F->timing = on != 0;
+    for (double &t : F->t_ms) t = 0;
+    for (int64_t &c : F->counts) c = 0;
+    return MXC_OK;
+}
+
+extern "C" int mxc_get_timing(const mxc_factor *F, double *t, int64_t *counts) {
+    if (!F) return MXC_INVALID;
+    collect_timing(const_cast<mxc_factor *>(F));
+    if (t) for (int i = 0; i < 10; i++) t[i] = F->t_ms[i];
+    if (counts) for (int i = 0; i < 5; i++) counts[i] = F->counts[i];
+    return MXC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// factorization
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(long long n, int bs, long long cap = 65536) {
+    long long g = (n + bs - 1) / bs;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+template <typename T>
+static int run_factor(mxc_factor *F, double beta) {
+    hipStream_t sm = F->stream;
+    T *W = (T *)F->W;
+    size_t t0 = 0, tt = 0;
+    tick(F, &t0);
+    int big = 0x7fffffff;
+    HIPTRY(hipMemcpyAsync(F->d_fail, &big, sizeof(int), hipMemcpyHostToDevice, sm));
+    // assembly: L = 0, scatter A, + beta on the diagonal; also refresh the CSR values
+    tick(F, &tt);
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(F->xsize, 256)), dim3(256), 0, sm, W, (long long)F->xsize, (T)0);
+    hipLaunchKernelGGL(k_scatter_a<T>, dim3(grid_for(F->nnz_a, 256)), dim3(256), 0, sm, F->d_Ax,
+                       F->d_pos, (long long)F->nnz_a, W);
+    if (beta != 0)
+        hipLaunchKernelGGL(k_add_diag<T>, dim3(grid_for(F->n, 256, 1LL << 30)), dim3(256), 0, sm,
+                           F->d_dpos, (long long)F->n, beta, W);
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(F->nnz_full, 256)), dim3(256), 0, sm,
+                       F->d_Ax, F->d_src, (long long)F->nnz_full, F->d_vals);
+    tock(F, 0, tt);
+    for (const Launch &L : F->factor_launches) {
+        tick(F, &tt);
+        switch (L.kind) {
+            case L_UPD:
+                hipLaunchKernelGGL((k_tile<T, MODE_SCATTER>), dim3((unsigned)L.count), dim3(256), 0, sm,
+                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+                F->counts[0]++;
+                break;
+            case L_DIAG:
+                hipLaunchKernelGGL(k_potrf_diag<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
+                                   F->d_diag + L.begin, W, F->d_fail);
+                F->counts[1]++;
+                break;
+            case L_TRSM:
+                hipLaunchKernelGGL((k_tile<T, MODE_STORE>), dim3((unsigned)L.count), dim3(256), 0, sm,
+                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+                F->counts[2]++;
+                break;
+            case L_TRAIL:
+                hipLaunchKernelGGL((k_tile<T, MODE_TRAIL>), dim3((unsigned)L.count), dim3(256), 0, sm,
+                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+                F->counts[3]++;
+                break;
+        }
+        tock(F, L.kind == L_UPD ? 1 : L.kind == L_DIAG ? 2 : L.kind == L_TRSM ? 3 : 4, tt);
+    }
+    HIPTRY(hipGetLastError());
+    tock(F, 8, t0);
+    return MXC_OK;
+}
+
+// Not positive definite: CHOLMOD's result is the factor of the leading
+// (minor) columns, with the failing supernode refactored for its first
+// info-1 columns and everything after it zero
+// (t_cholmod_super_numeric_worker.c:1084-1146).  Rare path: redo the
+// factorization with supernodes past the failing one excluded, on the host
+// copy of the assembled values.
+template <typename T>
+static int finish_not_posdef(mxc_factor *F, int fail_col, double beta);
+
+extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, double beta,
+                             int64_t *minor) {
+    if (!F || (!Ax && F->nnz_a > 0)) return MXC_INVALID;
+    HIPTRY(hipSetDevice(g_device));
+    hipStream_t sm = F->stream;
+    if (F->nnz_a > 0)
+        HIPTRY(hipMemcpyAsync(F->d_Ax, Ax, F->nnz_a * sizeof(double),
+                              location == MXC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, sm));
+    int rc = F->precision == MXC_FP64 ? run_factor<double>(F, beta) : run_factor<float>(F, beta);
+    if (rc) return rc;
+    int fail = 0;
+    HIPTRY(hipMemcpyAsync(&fail, F->d_fail, sizeof(int), hipMemcpyDeviceToHost, sm));
+    HIPTRY(hipStreamSynchronize(sm));
+    F->values_loaded = true;
+    if (fail == 0x7fffffff) {
+        if (minor) *minor = F->n;
+        return MXC_OK;
+    }
+    rc = F->precision == MXC_FP64 ? finish_not_posdef<double>(F, fail, beta)
+                                  : finish_not_posdef<float>(F, fail, beta);
+    if (minor) *minor = fail;
+    return rc == MXC_OK ? MXC_NOT_POSDEF : rc;
+}
+
+// Host completion of the not-positive-definite case.  Supernodes before the
+// failing supernode s* are final on the device (none depends on s*).  s* is
+// re-assembled from A and its descendants' updates on the host (a copy of
+// the factored descendants is pulled back), its first info-1 columns are
+// factored and solved for, and everything from column k1+info-1 on is zeroed.
+template <typename T>
+static int finish_not_posdef(mxc_factor *F, int fail_col, double beta) {
+    hipStream_t sm = F->stream;
+    const int64_t ns = F->nsuper;
+    int64_t s = std::upper_bound(F->Super.begin(), F->Super.end(), (int64_t)fail_col) - F->Super.begin() - 1;
+    const int64_t k1 = F->Super[s], nscol = F->Super[s + 1] - k1;
+    const int64_t psi = F->Lpi[s], nsrow = F->Lpi[s + 1] - psi, psx = F->Lpx[s];
+    const int64_t info = fail_col - k1 + 1;
+    // host copy of everything up to and including s
+    std::vector<T> Lx((size_t)F->Lpx[s + 1]);
+    HIPTRY(hipMemcpyAsync(Lx.data(), F->W, Lx.size() * sizeof(T), hipMemcpyDeviceToHost, sm));
+    std::vector<int> Ls((size_t)F->Lpi[ns]);
+    HIPTRY(hipMemcpyAsync(Ls.data(), F->d_Ls, Ls.size() * sizeof(int), hipMemcpyDeviceToHost, sm));
+    std::vector<double> Ax((size_t)std::max<int64_t>(F->nnz_a, 1));
+    std::vector<long long> pos((size_t)std::max<int64_t>(F->nnz_a, 1));
+    HIPTRY(hipMemcpyAsync(Ax.data(), F->d_Ax, F->nnz_a * sizeof(double), hipMemcpyDeviceToHost, sm));
+    HIPTRY(hipMemcpyAsync(pos.data(), F->d_pos, F->nnz_a * sizeof(long long), hipMemcpyDeviceToHost, sm));
+    HIPTRY(hipStreamSynchronize(sm));
+    // re-assemble s: zero, scatter A entries that land in s, beta
+    std::vector<double> Sx((size_t)(nsrow * nscol), 0.0);
+    for (int64_t p = 0; p < F->nnz_a; p++) {
+        long long q = pos[p];
+        if (q >= psx && q < psx + nsrow * nscol) Sx[q - psx] = Ax[p];
+    }
+    for (int64_t c = 0; c < nscol; c++) Sx[c + c * nsrow] += beta;
+    // local row map of s
+    std::vector<int64_t> map_g;   // global row -> local, via binary search on Ls
+    auto loc = [&](int64_t g) -> int64_t {
+        if (g >= k1 && g < k1 + nscol) return g - k1;
+        auto b = Ls.begin() + psi + nscol, e = Ls.begin() + psi + nsrow;
+        auto it = std::lower_bound(b, e, (int)g);
+        return (it != e && *it == (int)g) ? (int64_t)(it - Ls.begin()) - psi : -1;
+    };
+    // descendant updates into s
+    for (int64_t d = 0; d < s; d++) {
+        int64_t ndcol = F->Super[d + 1] - F->Super[d];
+        int64_t pdi = F->Lpi[d], pdend = F->Lpi[d + 1], ndrow = pdend - pdi, pdx = F->Lpx[d];
+        int64_t q = pdi + ndcol;
+        while (q < pdend && Ls[q] < k1) q++;
+        if (q >= pdend || Ls[q] >= k1 + nscol) continue;
+        int64_t q2 = q;
+        while (q2 < pdend && Ls[q2] < k1 + nscol) q2++;
+        int64_t ndrow1 = q2 - q, ndrow2 = pdend - q;
+        std::vector<int64_t> rel(ndrow2);
+        for (int64_t i = 0; i < ndrow2; i++) rel[i] = loc(Ls[q + i]);
+        for (int64_t j = 0; j < ndrow1; j++)
+            for (int64_t i = j; i < ndrow2; i++) {
+                double c = 0;
+                for (int64_t k = 0; k < ndcol; k++)
+                    c += (double)Lx[pdx + (q - pdi) + i + k * ndrow] * (double)Lx[pdx + (q - pdi) + j + k * ndrow];
+                Sx[rel[i] + rel[j] * nsrow] -= c;
+            }
+    }
+    // factor the first info-1 columns, solve the rows below them, zero the rest
+    const int64_t nc = info - 1;
+    for (int64_t j = 0; j < nc; j++) {
+        double dj = Sx[j + j * nsrow];
+        for (int64_t k = 0; k < j; k++) dj -= Sx[j + k * nsrow] * Sx[j + k * nsrow];
+        dj = std::sqrt(dj);
+        Sx[j + j * nsrow] = dj;
+        for (int64_t i = j + 1; i < nsrow; i++) {
+            double v = Sx[i + j * nsrow];
+            for (int64_t k = 0; k < j; k++) v -= Sx[i + k * nsrow] * Sx[j + k * nsrow];
+            Sx[i + j * nsrow] = v / dj;
+        }
+    }
+    for (int64_t j = nc; j < nscol; j++)
+        for (int64_t i = 0; i < nsrow; i++) Sx[i + j * nsrow] = 0;
+    for (int64_t j = 0; j < nc; j++)       // strict upper part of the diagonal block stays 0
+        for (int64_t i = 0; i < j; i++) Sx[i + j * nsrow] = 0;
+    std::vector<T> St(Sx.size());
+    for (size_t i = 0; i < Sx.size(); i++) St[i] = (T)Sx[i];
+    T *W = (T *)F->W;
+    HIPTRY(hipMemcpyAsync(W + psx, St.data(), St.size() * sizeof(T), hipMemcpyHostToDevice, sm));
+    int64_t tail = F->xsize - F->Lpx[s + 1];
+    if (tail > 0)
+        hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(tail, 256)), dim3(256), 0, sm, W + F->Lpx[s + 1],
+                           (long long)tail, (T)0);
+    HIPTRY(hipStreamSynchronize(sm));
+    return MXC_OK;
+}
+
+extern "C" int mxc_factor_get_Lx(const mxc_factor *F, double *Lx) {
+    if (!F || !Lx) return MXC_INVALID;
+    HIPTRY(hipSetDevice(g_device));
+    if (F->precision == MXC_FP64) {
+        HIPTRY(hipMemcpyAsync(Lx, F->W, F->xsize * sizeof(double), hipMemcpyDeviceToHost, F->stream));
+        HIPTRY(hipStreamSynchronize(F->stream));
+    } else {
+        std::vector<float> tmp(F->xsize);
+        HIPTRY(hipMemcpyAsync(tmp.data(), F->W, F->xsize * sizeof(float), hipMemcpyDeviceToHost, F->stream));
+        HIPTRY(hipStreamSynchronize(F->stream));
+        for (int64_t i = 0; i < F->xsize; i++) Lx[i] = tmp[i];
+    }
+    return MXC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// solves
+// ---------------------------------------------------------------------------
+template <typename T>
+static int run_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
+    hipStream_t sm = F->stream;
+    const T *W = (const T *)F->W;
+    size_t tt = 0;
+    int n = (int)F->n;
+    int nr = (int)nrhs;
+    if (fwd) {
+        tick(F, &tt);
+        for (const Launch &L : F->fwd_launches) {
+            if (L.kind == L_FDIAG)
+                hipLaunchKernelGGL(k_fwd_diag<T>, dim3((unsigned)L.count), dim3(64), 0, sm,
+                                   F->d_stask + L.begin, W, Xw, (long long)n, nr);
+            else
+                hipLaunchKernelGGL(k_fwd_off<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
+                                   F->d_stask + L.begin, W, F->d_Ls, Xw, (long long)n, nr);
+            F->counts[4]++;
+        }
+        tock(F, 5, tt);
+    }
+    if (bwd) {
+        tick(F, &tt);
+        for (const Launch &L : F->bwd_launches) {
+            if (L.kind == L_BDIAG)
+                hipLaunchKernelGGL(k_bwd_diag<T>, dim3((unsigned)L.count), dim3(64), 0, sm,
+                                   F->d_stask + L.begin, W, Xw, (long long)n, nr);
+            else
+                hipLaunchKernelGGL(k_bwd_off<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
+                                   F->d_stask + L.begin, W, F->d_Ls, Xw, (long long)n, nr);
+            F->counts[4]++;
+        }
+        tock(F, 6, tt);
+    }
+    HIPTRY(hipGetLastError());
+    return MXC_OK;
+}
+
+// device-resident full solve: dX = A \ dB (both FP64, device, n x nrhs)
+template <typename T>
+static int solve_device(mxc_factor *F, const double *dB, int64_t nrhs, double *dX) {
+    hipStream_t sm = F->stream;
+    T *Xw = (T *)F->d_work;
+    unsigned g = grid_for(F->n, 256, 1LL << 30);
+    size_t tt = 0;
+    tick(F, &tt);
+    hipLaunchKernelGGL(k_perm_gather<T>, dim3(g), dim3(256), 0, sm, dB, F->d_perm, (long long)F->n,
+                       (int)nrhs, Xw);
+    int rc = run_sweep<T>(F, Xw, nrhs, true, true);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_perm_scatter<T>, dim3(g), dim3(256), 0, sm, Xw, F->d_perm, (long long)F->n,
+                       (int)nrhs, dX);
+    tock(F, 9, tt);
+    HIPTRY(hipGetLastError());
+    return MXC_OK;
+}
+
+static int stage_in(mxc_factor *F, const double *B, int64_t nrhs, int location, const double **dB) {
+    if (location == MXC_DEVICE) {
+        *dB = B;
+        return MXC_OK;
+    }
+    HIPTRY(hipMemcpyAsync(F->d_b, B, F->n * nrhs * sizeof(double), hipMemcpyHostToDevice, F->stream));
+    *dB = F->d_b;
+    return MXC_OK;
+}
+
+extern "C" int mxc_solve(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location) {
+    if (!F || !B || !X || nrhs < 0) return MXC_INVALID;
+    if (!F->values_loaded) return MXC_INVALID;
+    if (nrhs == 0 || F->n == 0) return MXC_OK;
+    HIPTRY(hipSetDevice(g_device));
+    int rc = ensure_work(F, nrhs);
+    if (rc) return rc;
+    const double *dB;
+    if ((rc = stage_in(F, B, nrhs, location, &dB))) return rc;
+    double *dX = location == MXC_DEVICE ? X : F->d_x;
+    rc = F->precision == MXC_FP64 ? solve_device<double>(F, dB, nrhs, dX)
+                                  : solve_device<float>(F, dB, nrhs, dX);
+    if (rc) return rc;
+    if (location != MXC_DEVICE)
+        HIPTRY(hipMemcpyAsync(X, dX, F->n * nrhs * sizeof(double), hipMemcpyDeviceToHost, F->stream));
+    HIPTRY(hipStreamSynchronize(F->stream));
+    return MXC_OK;
+}
+
+static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, bool fwd) {
+    if (!F || !X || nrhs < 0) return MXC_INVALID;
+    if (!F->values_loaded) return MXC_INVALID;
+    if (nrhs == 0 || F->n == 0) return MXC_OK;
+    HIPTRY(hipSetDevice(g_device));
+    int rc = ensure_work(F, nrhs);
+    if (rc) return rc;
+    hipStream_t sm = F->stream;
+    size_t len = (size_t)F->n * nrhs;
+    if (location != MXC_DEVICE)
+        HIPTRY(hipMemcpyAsync(F->d_x, X, len * sizeof(double), hipMemcpyHostToDevice, sm));
+    double *dX = location == MXC_DEVICE ? X : F->d_x;
+    if (F->precision == MXC_FP64) {
+        rc = run_sweep<double>(F, dX, nrhs, fwd, !fwd);
+    } else {
+        float *Xw = (float *)F->d_work;
+        std::vector<float> tmp;
+        std::vector<double> h(len);   // convert through the FP32 work vector
+        HIPTRY(hipMemcpyAsync(h.data(), dX, len * sizeof(double), hipMemcpyDeviceToHost, sm));
+        HIPTRY(hipStreamSynchronize(sm));
+        tmp.assign(h.begin(), h.end());
+        HIPTRY(hipMemcpyAsync(Xw, tmp.data(), len * sizeof(float), hipMemcpyHostToDevice, sm));
+        rc = run_sweep<float>(F, Xw, nrhs, fwd, !fwd);
+        if (rc) return rc;
+        HIPTRY(hipMemcpyAsync(tmp.data(), Xw, len * sizeof(float), hipMemcpyDeviceToHost, sm));
+        HIPTRY(hipStreamSynchronize(sm));
+        for (size_t i = 0; i < len; i++) h[i] = tmp[i];
+        HIPTRY(hipMemcpyAsync(dX, h.data(), len * sizeof(double), hipMemcpyHostToDevice, sm));
+    }
+    if (rc) return rc;
+    if (location != MXC_DEVICE)
+        HIPTRY(hipMemcpyAsync(X, dX, len * sizeof(double), hipMemcpyDeviceToHost, sm));
+    HIPTRY(hipStreamSynchronize(sm));
+    return MXC_OK;
+}
+
+extern "C" int mxc_super_lsolve(mxc_factor *F, double *X, int64_t nrhs, int location) {
+    return super_sweep(F, X, nrhs, location, true);
+}
+extern "C" int mxc_super_ltsolve(mxc_factor *F, double *X, int64_t nrhs, int location) {
+    return super_sweep(F, X, nrhs, location, false);
+}
+
+// ---------------------------------------------------------------------------
+// residual and iterative refinement
+// ---------------------------------------------------------------------------
+static void launch_residual(mxc_factor *F, const double *dB, const double *dX, double *dR, int64_t nrhs) {
+    hipLaunchKernelGGL(k_residual, dim3(grid_for(F->n, 256, 1LL << 30)), dim3(256), 0, F->stream,
+                       F->d_rowptr, F->d_col, F->d_vals, dB, dX, dR, (long long)F->n, (int)nrhs);
+}
+
+extern "C" int mxc_residual(mxc_factor *F, const double *B, const double *X, int64_t nrhs, double *R,
+                            int location) {
+    if (!F || !B || !X || !R || nrhs < 0) return MXC_INVALID;
+    if (!F->values_loaded) return MXC_INVALID;
+    if (nrhs == 0 || F->n == 0) return MXC_OK;
+    HIPTRY(hipSetDevice(g_device));
+    int rc = ensure_work(F, nrhs);
+    if (rc) return rc;
+    hipStream_t sm = F->stream;
+    size_t len = (size_t)F->n * nrhs;
+    const double *dB = B, *dX = X;
+    double *dR = R;
+    if (location != MXC_DEVICE) {
+        HIPTRY(hipMemcpyAsync(F->d_b, B, len * sizeof(double), hipMemcpyHostToDevice, sm));
+        HIPTRY(hipMemcpyAsync(F->d_x, X, len * sizeof(double), hipMemcpyHostToDevice, sm));
+        dB = F->d_b;
+        dX = F->d_x;
+        dR = F->d_r;
+    }
+    size_t tt = 0;
+    tick(F, &tt);
+    launch_residual(F, dB, dX, dR, nrhs);
+    tock(F, 7, tt);
+    HIPTRY(hipGetLastError());
+    if (location != MXC_DEVICE) HIPTRY(hipMemcpyAsync(R, dR, len * sizeof(double), hipMemcpyDeviceToHost, sm));
+    HIPTRY(hipStreamSynchronize(sm));
+    return MXC_OK;
+}
+
+// max over columns of ||V(:,r)|| / ||B(:,r)|| (norms of B cached in bn)
+static int colnorms(mxc_factor *F, const double *dV, int64_t nrhs, std::vector<double> &out) {
+    hipStream_t sm = F->stream;
+    HIPTRY(hipMemsetAsync(F->d_norm, 0, sizeof(double) * nrhs, sm));
+    hipLaunchKernelGGL(k_sumsq, dim3(grid_for(F->n, 256, 1024)), dim3(256), 0, sm, dV, (long long)F->n,
+                       (int)nrhs, F->d_norm);
+    out.assign(nrhs, 0);
+    HIPTRY(hipMemcpyAsync(out.data(), F->d_norm, sizeof(double) * nrhs, hipMemcpyDeviceToHost, sm));
+    HIPTRY(hipStreamSynchronize(sm));
+    for (auto &v : out) v = std::sqrt(v);
+    return MXC_OK;
+}
+
+template <typename T>
+static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *dX, double tol,
+                         int maxiter, double *out) {
+    hipStream_t sm = F->stream;
+    size_t len = (size_t)F->n * nrhs;
+    std::vector<double> bn, rn;
+    int rc = colnorms(F, dB, nrhs, bn);
+    if (rc) return rc;
+    // x0 = A \ b
+    T *Xw = (T *)F->d_work;
+    unsigned g = grid_for(F->n, 256, 1LL << 30);
+    hipLaunchKernelGGL(k_perm_gather<T>, dim3(g), dim3(256), 0, sm, dB, F->d_perm, (long long)F->n, (int)nrhs, Xw);
+    if ((rc = run_sweep<T>(F, Xw, nrhs, true, true))) return rc;
+    hipLaunchKernelGGL(k_perm_scatter<T>, dim3(g), dim3(256), 0, sm, Xw, F->d_perm, (long long)F->n, (int)nrhs, dX);
+    int it = 0;
+    double rel = 0, rel0 = -1;
+    for (;;) {
+        launch_residual(F, dB, dX, F->d_r, nrhs);
+        if ((rc = colnorms(F, F->d_r, nrhs, rn))) return rc;
+        rel = 0;
+        for (int64_t r = 0; r < nrhs; r++) {
+            double v = bn[r] > 0 ? rn[r] / bn[r] : rn[r];
+            rel = std::max(rel, v);
+        }
+        if (rel0 < 0) rel0 = rel;
+        if (rel <= tol || it >= maxiter || !std::isfinite(rel)) break;
+        // d = A \ r ; x += d
+        hipLaunchKernelGGL(k_perm_gather<T>, dim3(g), dim3(256), 0, sm, F->d_r, F->d_perm, (long long)F->n, (int)nrhs, Xw);
+        if ((rc = run_sweep<T>(F, Xw, nrhs, true, true))) return rc;
+        hipLaunchKernelGGL(k_perm_scatter<T>, dim3(g), dim3(256), 0, sm, Xw, F->d_perm, (long long)F->n, (int)nrhs, F->d_r);
+        hipLaunchKernelGGL(k_axpy_update<double>, dim3(grid_for((long long)len, 256)), dim3(256), 0, sm, dX,
+                           (const double *)F->d_r, (long long)len);
+        it++;
+    }
+    HIPTRY(hipGetLastError());
+    if (out) {
+        out[0] = it;
+        out[1] = rel;
+        out[2] = rel0;
+    }
+    return MXC_OK;
+}
+
+extern "C" int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location,
+                                double tol, int maxiter, double *out) {
+    if (!F || !B || !X || nrhs < 0 || nrhs > 64) return MXC_INVALID;
+    if (!F->values_loaded) return MXC_INVALID;
+    if (nrhs == 0 || F->n == 0) return MXC_OK;
+    HIPTRY(hipSetDevice(g_device));
+    int rc = ensure_work(F, nrhs);
+    if (rc) return rc;
+    const double *dB;
+    if ((rc = stage_in(F, B, nrhs, location, &dB))) return rc;
+    double *dX = location == MXC_DEVICE ? X : F->d_x;
+    rc = F->precision == MXC_FP64 ? refine_device<double>(F, dB, nrhs, dX, tol, maxiter, out)
+                                  : refine_device<float>(F, dB, nrhs, dX, tol, maxiter, out);
+    if (rc) return rc;
+    if (location != MXC_DEVICE)
+        HIPTRY(hipMemcpyAsync(X, dX, F->n * nrhs * sizeof(double), hipMemcpyDeviceToHost, F->stream));
+    HIPTRY(hipStreamSynchronize(F->stream));
+    return MXC_OK;
+}
+
+extern "C" int mxc_synchronize(mxc_factor *F) {
+    if (!F) return MXC_INVALID;
+    HIPTRY(hipStreamSynchronize(F->stream));
+    return MXC_OK;
+}

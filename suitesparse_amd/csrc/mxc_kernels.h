@@ -1,0 +1,66 @@
+// mxc_kernels.h -- gfx950 kernels of the supernodal numeric factorization and
+// solve.  Templated on the working type T (double: FP64 MFMA, float: FP32 MFMA).
+//
+// Storage: one device buffer W holds L->x (xsize entries, CHOLMOD's layout:
+// supernode s is a column-major nsrow x nscol block at Lpx[s], ld = nsrow)
+// followed by the inverses of the 64-column diagonal blocks of every
+// supernode (block (s,t) of width nb stored column-major with ld = nb).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mxc {
+
+constexpr int TILE = 64;     // C tile edge (rows and columns)
+constexpr int BK = 16;       // K chunk staged through LDS
+constexpr int NB = 64;       // diagonal block width of the blocked POTRF
+constexpr int LDS_PAD = 4;
+
+// One dense update C = Yp * Xp' (C is M x N) applied to a target.
+//   X side (C columns): row j of Xp is W[x_off + j + k*ldx], k < K
+//   Y side (C rows):    row i of Yp is W[y_off + i + k*ldy], k < K
+// MODE_SCATTER: W[c_off + relc(j)*ldc + relr(i)] -= C(i,j) for i >= j, with the
+//   relative map of a descendant d into ancestor s (t_* describe s's rows):
+//   C row i is global row Ls[ls_off+i]; it lands on local row g-base if
+//   g < base+t_nscol (one of s's own columns) else at its position in
+//   Ls[t_psi+t_nscol .. t_psi+t_nsrow); C column j lands on local column g-base.
+// MODE_TRAIL:   W[c_off + j*ldc + i] -= C(i,j) for i >= j (in-place trailing update)
+// MODE_STORE:   W[c_off + j*ldc + i]  = C(i,j)           (TRSM by an inverse block)
+struct UpdDesc {
+    long long x_off, y_off, c_off;
+    int ldx, ldy, ldc;
+    int K, M, N;
+    int ls_off, base;
+    int t_psi, t_nscol, t_nsrow;
+    int pad;
+};
+
+struct Tile {
+    int desc;
+    int ti;
+    int tj;
+};
+
+// Diagonal block (s, t): rows/cols [c0, c0+nb) of supernode s.
+struct DiagTask {
+    long long off;     // W offset of element (c0, c0)
+    long long winv;    // W offset of the inverse block (ld nb)
+    int ld;            // nsrow of s
+    int nb;
+    int col0;          // global column index k1 + c0 (for the failing-column report)
+    int pad;
+};
+
+// Solve work item: column block (s, t) and a row range [r0, r1) of s.
+struct SolveTask {
+    long long lx;      // W offset of supernode s (Lpx[s])
+    long long winv;    // inverse of block t (ld nb)
+    int nsrow, nscol;
+    int k1, psi;
+    int c0, nb;
+    int r0, r1;
+};
+
+enum { MODE_SCATTER = 0, MODE_TRAIL = 1, MODE_STORE = 2 };
+
+}  // namespace mxc

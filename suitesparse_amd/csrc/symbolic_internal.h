@@ -1,0 +1,34 @@
+// symbolic_internal.h -- layout of the CPU symbolic factor (shared by the C
+// symbolic code and the C++/HIP plan builder).
+#ifndef MXC_SYMBOLIC_INTERNAL_H
+#define MXC_SYMBOLIC_INTERNAL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct mxc_symbolic {
+    int64_t n;
+    int64_t nsuper;
+    int64_t ssize;
+    int64_t xsize;
+    int64_t maxcsize;
+    int64_t maxesize;
+    int ordering;
+    double fl;
+    double lnz;
+    int64_t *Perm;      // n: Perm[k] = original index of pivot k
+    int64_t *ColCount;  // n (NULL after import)
+    int64_t *Parent;    // n, etree of the permuted matrix (NULL after import)
+    int64_t *Super;     // nsuper+1
+    int64_t *Lpi;       // nsuper+1
+    int64_t *Lpx;       // nsuper+1
+    int64_t *Ls;        // ssize
+};
+
+#ifdef __cplusplus
+}
+#endif
+#endif
