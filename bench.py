@@ -1,0 +1,262 @@
+"""Benchmark: supernodal Cholesky factor + solve on one MI355X.
+
+Default workload (one "step" = one numeric factorization + one solve of the
+same system, inputs resident in HBM):
+  * headline: 3D 7-point variable-coefficient diffusion on 64^3 (n = 262,144),
+    METIS ordering (the reference's own permutation), FP64, 1 RHS,
+    metric = factor+solve GFLOP/s = (fl + 4*lnz*nrhs) / (t_factor + t_solve)
+  * mixed precision: 3D 27-point stencil on 96^3 (n = 884,736), 4 RHS: plain
+    FP64 factor+solve vs FP32 factor + FP64 iterative refinement to a 1e-10
+    relative residual (time-to-solution and speedup)
+  * cpu_baseline: the reference CHOLMOD (compiled from the reference sources,
+    MKL BLAS) on the host cores, same matrix and ordering.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-mixed] [--no-cpu]
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+MFMA_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_perm(name: str) -> np.ndarray:
+    return np.load(ROOT / "tests" / "golden" / f"perm_{name}.npz")["perm"].astype(np.int64)
+
+
+def relres(A, B, X) -> float:
+    R = B - A.matvec(X)
+    return float((np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)).max())
+
+
+def gpu_factor_solve(A, perm, B, precision, steps, warmup, refine=False, tol=1e-10,
+                     time_updates=False):
+    """Run `steps` timed factor+solve steps; returns timings and the last X."""
+    import torch
+    from suitesparse_amd import cholmod
+    cm = cholmod.Common(precision=precision)
+    t0 = time.perf_counter()
+    L = cholmod.analyze_p(A, perm, cm)
+    assert L is not None, cm.status
+    t_an = time.perf_counter() - t0
+    assert cm.start()
+    dev = torch.device("cuda:0")
+    Ax = torch.as_tensor(A.x, device=dev)
+    Bd = torch.as_tensor(np.asfortranarray(B), device=dev)
+    Bd = Bd.t().contiguous().t()
+    t0 = time.perf_counter()
+    L._numeric(cm)
+    t_plan = time.perf_counter() - t0
+    info = None
+
+    def step():
+        nonlocal info
+        ok = cholmod.factorize(A, L, cm, values=Ax)
+        assert ok and cm.status == 0, cm.status
+        if refine:
+            X, info = cholmod.solve_refine(L, Bd, cm, tol=tol, maxiter=20)
+        else:
+            X = cholmod.solve(cholmod.CHOLMOD_A, L, Bd, cm)
+        return X
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    if time_updates:
+        L.set_timing(True, classes=[1])      # HIP events around the update launches only
+    lat = []
+    torch.cuda.synchronize()
+    tstart = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        X = step()
+        lat.append(time.perf_counter() - t1)
+    torch.cuda.synchronize()
+    ttot = time.perf_counter() - tstart
+    timing = L.timing() if time_updates else None
+    breakdown = None
+    if time_updates:
+        L.set_timing(True)                   # one extra step with every class timed
+        step()
+        breakdown = L.timing()
+        L.set_timing(False)
+    # split factor / solve times on one extra untimed step
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    cholmod.factorize(A, L, cm, values=Ax)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    if refine:
+        X2, info2 = cholmod.solve_refine(L, Bd, cm, tol=tol, maxiter=20)
+    else:
+        X2 = cholmod.solve(cholmod.CHOLMOD_A, L, Bd, cm)
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t1
+    Xh = X.t().contiguous().t().cpu().numpy() if hasattr(X, "cpu") else X
+    res = {
+        "L": L, "lat": lat, "t_total": ttot, "t_factor": tf, "t_solve": ts,
+        "t_analyze": t_an, "t_plan": t_plan, "X": np.asarray(Xh), "timing": timing,
+        "refine": info, "info": L.info(), "breakdown": breakdown,
+    }
+    return res
+
+
+def cpu_baseline(A, perm, B, fl, lnz, budget_s=20.0):
+    """Reference CHOLMOD (FP64, MKL, OpenMP) on the host: factor + solve."""
+    from oracle import ref
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads = min(threads, os.cpu_count() or threads)
+    ref.set_threads(threads)
+    F = ref.RefFactor(A, ref.GIVEN, perm=perm)
+    F.factorize()          # warm-up
+    X = F.solve(B)
+    tf, ts, reps = [], [], 0
+    t0 = time.perf_counter()
+    while reps < 5 and (time.perf_counter() - t0 < budget_s or reps < 1):
+        F.factorize()
+        tf.append(F.t_factor)
+        X = F.solve(B)
+        ts.append(F.t_solve)
+        reps += 1
+    t = float(np.median(tf)) + float(np.median(ts))
+    nrhs = B.shape[1]
+    return {
+        "value": (fl + 4.0 * lnz * nrhs) / t / 1e9, "unit": "GFLOP/s", "cores": threads,
+        "kind": "reference",
+        "sample": f"{reps} factor+solve reps of the same matrix/ordering, median; "
+                  f"t_factor={np.median(tf):.3f}s t_solve={np.median(ts):.4f}s",
+        "t_factor_s": float(np.median(tf)), "t_solve_s": float(np.median(ts)),
+        "factor_gflops": fl / float(np.median(tf)) / 1e9,
+        "relres": relres(A, B, X), "X": X,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-side", type=int, default=96)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the factorization would shard along the elimination tree, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from suitesparse_amd import problems
+
+    # ---------------- headline: 64^3 7-point, FP64
+    log("[bench] generating 64^3 7-point diffusion")
+    A, B, xt = problems.workload_diffusion3d_64()
+    perm = load_perm("diffusion3d_64_metis")
+    log("[bench] FP64 factor+solve on GPU")
+    g = gpu_factor_solve(A, perm, B, "fp64", args.steps, args.warmup, time_updates=True)
+    L = g["L"]
+    fl, lnz, nrhs = L.fl, L.lnz, B.shape[1]
+    ms = g["t_total"] / args.steps * 1e3
+    flops_step = fl + 4.0 * lnz * nrhs
+    value = flops_step / (g["t_total"] / args.steps) / 1e9
+    rr = relres(A, B, g["X"])
+    fe = float(np.linalg.norm(g["X"] - xt) / np.linalg.norm(xt))
+    lat = np.array(g["lat"]) * 1e3
+    log(f"[bench] 64^3 fp64: {ms:.1f} ms/step, {value:.1f} GFLOP/s, relres {rr:.2e}, fwd {fe:.2e}")
+
+    # roofline of the dominant kernel (MFMA update tiles), from HIP events
+    tim = g["timing"]
+    roof = None
+    if tim is not None:
+        upd_ms = tim["updates"] / args.steps
+        from suitesparse_amd import cholmod
+        uf = L.kernel_flops()["update"]
+        nl = tim["launches"]["update"] / args.steps
+        achieved = uf / (upd_ms / 1e3) / 1e12 if upd_ms > 0 else 0.0
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_PEAK_TFLOPS["f64"],
+                "unit": "TFLOP/s", "frac": round(achieved / MFMA_PEAK_TFLOPS["f64"], 4),
+                "traffic": None, "kernel": "k_tile<double,SCATTER> (descendant update + scatter)",
+                "launches_per_step": nl, "avg_launch_ms": round(upd_ms / max(nl, 1), 4),
+                "flops_per_step": uf}
+    out = {
+        "metric": "factor+solve GFLOP/s (FP64, 3D 7-pt 64^3, METIS)",
+        "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "diffusion3d_7pt_64^3_metis_fp64_1rhs", "n": A.n,
+                   "nnz": A.nnz_full, "fl": fl, "lnz": lnz, "nrhs": nrhs},
+        "solves_per_s": round(1e3 / ms, 3),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3)},
+        "t_factor_ms": round(g["t_factor"] * 1e3, 3), "t_solve_ms": round(g["t_solve"] * 1e3, 3),
+        "factor_gflops": round(fl / g["t_factor"] / 1e9, 2),
+        "relres": rr, "fwd_err": fe, "roofline": roof,
+        "breakdown_ms": {k: (round(v, 3) if k != "launches" else v)
+                         for k, v in (g["breakdown"] or {}).items()},
+    }
+
+    # ---------------- CPU baseline (reference CHOLMOD) on the same problem
+    if not args.no_cpu:
+        log("[bench] reference CPU baseline")
+        try:
+            cb = cpu_baseline(A, perm, B, fl, lnz)
+            Xr = cb.pop("X")
+            out["fwd_err_vs_ref"] = float(np.linalg.norm(g["X"] - Xr) / np.linalg.norm(Xr))
+            out["cpu_baseline"] = cb
+            out["speedup_vs_cpu"] = round(value / cb["value"], 2)
+            log(f"[bench] cpu: {cb['value']:.1f} GFLOP/s on {cb['cores']} threads")
+        except Exception as e:   # reference library missing on this host
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+
+    # ---------------- mixed precision on the 27-point 96^3 stencil
+    if not args.no_mixed:
+        side = args.mixed_side
+        log(f"[bench] generating {side}^3 27-point stencil")
+        A3, B3, _ = problems.workload_stencil27_96(side)
+        perm3 = load_perm("stencil27_96_metis") if side == 96 else None
+        if perm3 is None:
+            from suitesparse_amd import cholmod as _c
+            perm3 = None
+        mix = {"workload": f"stencil3d_27pt_{side}^3_metis_4rhs", "n": A3.n}
+        for prec in ("fp64", "fp32"):
+            log(f"[bench] {side}^3 {prec}")
+            r = gpu_factor_solve(A3, perm3, B3, prec, max(1, args.steps // 2), 1,
+                                 refine=(prec == "fp32"))
+            tts = r["t_total"] / max(1, args.steps // 2)
+            rr3 = relres(A3, B3, r["X"])
+            mix[prec] = {"time_to_solution_ms": round(tts * 1e3, 2),
+                         "t_factor_ms": round(r["t_factor"] * 1e3, 2),
+                         "t_solve_ms": round(r["t_solve"] * 1e3, 2),
+                         "factor_gflops": round(r["L"].fl / r["t_factor"] / 1e9, 1),
+                         "factor_solve_gflops": round((r["L"].fl + 4 * r["L"].lnz * B3.shape[1]) / tts / 1e9, 1),
+                         "relres": rr3, "refine": r["refine"]}
+            log(f"[bench]   {mix[prec]}")
+            del r
+            import gc
+            gc.collect()
+        mix["speedup_fp32ir_vs_fp64"] = round(mix["fp64"]["time_to_solution_ms"] /
+                                              mix["fp32"]["time_to_solution_ms"], 3)
+        out["mixed_precision"] = mix
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

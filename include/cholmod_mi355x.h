@@ -115,7 +115,9 @@ void mxc_factor_free(mxc_factor *F);
 
 /* info[0..7] = n, nsuper, xsize, levels, update pairs, update tiles,
  *              device bytes held, precision;
- * dinfo[0..1] = fl, lnz */
+ * dinfo[0..5] = fl, lnz, and the algorithmic flops of one factorization by
+ *               kernel class: descendant updates (dsyrk + dgemm counts), diagonal
+ *               POTRF, panel TRSM, in-supernode trailing updates */
 int mxc_factor_info(const mxc_factor *F, int64_t *info, double *dinfo);
 
 /* Numeric factorization of tril(A) + beta*I with values Ax (pattern of
@@ -151,12 +153,13 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
 /* Wait for all work queued on the factor's stream. */
 int mxc_synchronize(mxc_factor *F);
 
-/* Per-phase device times (ms) of the last factorize / solve when timing is
- * on: t[0] assembly, t[1] updates, t[2] diag potrf, t[3] trsm, t[4] trailing,
- * t[5] forward solve, t[6] backward solve, t[7] residual, t[8] factor total,
- * t[9] solve total; counts[0..4] = launches of update, potrf, trsm, trailing,
- * solve. */
-int mxc_set_timing(mxc_factor *F, int on);
+/* Device times (ms, from HIP events on the factor's stream) accumulated since
+ * mxc_set_timing, per class: t[0] assembly, t[1] updates, t[2] diag potrf,
+ * t[3] trsm, t[4] trailing, t[5] forward solve, t[6] backward solve,
+ * t[7] residual, t[8] factor total, t[9] solve total.  mask: bit c times
+ * class c (1 = all classes, 0 = off).  counts[0..4] = launches of update,
+ * potrf, trsm, trailing, solve. */
+int mxc_set_timing(mxc_factor *F, int mask);
 int mxc_get_timing(const mxc_factor *F, double *t, int64_t *counts);
 
 /* Last HIP error text seen by the library (empty when none). */

@@ -148,13 +148,26 @@ class Factor:
 
     def info(self) -> dict:
         info = np.zeros(8, dtype=np.int64)
-        dinfo = np.zeros(2)
+        dinfo = np.zeros(6)
         check("mxc_factor_info", lib().mxc_factor_info(self._num, _ptr(info), _ptr(dinfo)))
         keys = ("n", "nsuper", "xsize", "levels", "pairs", "update_tiles", "device_bytes", "precision")
         return dict(zip(keys, (int(v) for v in info)))
 
-    def set_timing(self, on: bool):
-        check("mxc_set_timing", lib().mxc_set_timing(self._num, int(on)))
+    def set_timing(self, on, classes=None):
+        """Time kernel classes with HIP events (classes: iterable of class indices
+        0..9 as listed in timing(); default all)."""
+        mask = 0
+        if on:
+            mask = 1 if classes is None else sum(1 << c for c in classes) | (1 << 30)
+        check("mxc_set_timing", lib().mxc_set_timing(self._num, int(mask)))
+
+    def kernel_flops(self) -> dict:
+        """Algorithmic flops of one factorization by kernel class."""
+        info = np.zeros(8, dtype=np.int64)
+        dinfo = np.zeros(6)
+        check("mxc_factor_info", lib().mxc_factor_info(self._num, _ptr(info), _ptr(dinfo)))
+        return {"update": float(dinfo[2]), "potrf": float(dinfo[3]), "trsm": float(dinfo[4]),
+                "trailing": float(dinfo[5])}
 
     def timing(self) -> dict:
         t = np.zeros(10)

@@ -128,11 +128,16 @@ struct mxc_factor {
     std::vector<Launch> fwd_launches, bwd_launches;
     bool values_loaded = false;
 
-    // timing
-    bool timing = false;
+    // algorithmic flops per factorization, by kernel class
+    double flops_update = 0, flops_potrf = 0, flops_trsm = 0, flops_trail = 0;
+
+    // timing: an event pair around each launch of a class whose bit is set
+    unsigned timing_mask = 0;
     double t_ms[10] = {0};
     int64_t counts[5] = {0};
-    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> ev;            // pool, grown on demand
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> pending;   // (class, index of the start event)
 };
 
 template <typename X>
@@ -309,6 +314,10 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
             u.t_nsrow = u.ldc;
             pairs_by_level[level[s]].push_back(u);
             npairs++;
+            {
+                double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
+                F->flops_update += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;   // dsyrk + dgemm
+            }
             q = q2;
         }
     }
@@ -363,6 +372,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                 dt.col0 = (int)(S->Super[s] + c0);
                 dt.pad = 0;
                 diags.push_back(dt);
+                F->flops_potrf += (double)nb * nb * nb / 3.0;
                 int64_t r0 = c0 + nb;
                 if (nsrow > r0) {
                     UpdDesc u;
@@ -377,6 +387,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.c_off = u.y_off;
                     u.ldc = (int)nsrow;
                     add_tiles(u, false);
+                    F->flops_trsm += (double)u.M * nb * nb;
                 }
                 if (nscol > r0) {
                     UpdDesc u;
@@ -390,6 +401,10 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.c_off = psx + r0 + r0 * nsrow;
                     u.ldc = (int)nsrow;
                     trail.push_back(u);
+                    {
+                        double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
+                        F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
+                    }
                 }
             }
             long long dc = (long long)diags.size() - db;
@@ -523,6 +538,7 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
+    F->ev.clear();
     if (F->stream) (void)hipStreamDestroy(F->stream);
     delete F;
 }
@@ -568,55 +584,57 @@ extern "C" int mxc_factor_info(const mxc_factor *F, int64_t *info, double *dinfo
     if (dinfo) {
         dinfo[0] = F->fl;
         dinfo[1] = F->lnz;
+        dinfo[2] = F->flops_update;
+        dinfo[3] = F->flops_potrf;
+        dinfo[4] = F->flops_trsm;
+        dinfo[5] = F->flops_trail;
     }
     return MXC_OK;
 }
 
 // ---------------------------------------------------------------------------
-// timing helpers: an event pair around each launch when timing is on
+// timing helpers: an event pair around each timed launch (classes selected by
+// the mask given to mxc_set_timing); events come from a per-factor pool
 // ---------------------------------------------------------------------------
-struct TimedRegion {
-    mxc_factor *F;
-    int cls;
-    size_t slot;
-};
-static std::vector<std::pair<int, size_t>> g_pending;   // (class, first event index)
+static bool next_event(mxc_factor *F, size_t *idx) {
+    if (F->ev_used == F->ev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return false;
+        F->ev.push_back(e);
+    }
+    *idx = F->ev_used++;
+    return hipEventRecord(F->ev[*idx], F->stream) == hipSuccess;
+}
 
-static void tick(mxc_factor *F, size_t *slot) {
-    if (!F->timing) return;
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
-    (void)hipEventRecord(e, F->stream);
-    F->ev.push_back(e);
-    *slot = F->ev.size() - 1;
+static void tick(mxc_factor *F, int cls, size_t *slot) {
+    *slot = (size_t)-1;
+    if (!(F->timing_mask & (1u << cls))) return;
+    size_t i;
+    if (next_event(F, &i)) *slot = i;
 }
 
 static void tock(mxc_factor *F, int cls, size_t start) {
-    if (!F->timing) return;
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
-    (void)hipEventRecord(e, F->stream);
-    F->ev.push_back(e);
-    g_pending.push_back({cls, start});
+    if (start == (size_t)-1) return;
+    size_t i;
+    if (next_event(F, &i) && i == start + 1) F->pending.push_back({cls, start});
 }
 
 static void collect_timing(mxc_factor *F) {
-    if (!F->timing) return;
+    if (F->pending.empty() && F->ev_used == 0) return;
     (void)hipStreamSynchronize(F->stream);
-    for (auto &pr : g_pending) {
+    for (auto &pr : F->pending) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, F->ev[pr.second], F->ev[pr.second + 1]) == hipSuccess)
             F->t_ms[pr.first] += ms;
     }
-    g_pending.clear();
-    for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
-    F->ev.clear();
+    F->pending.clear();
+    F->ev_used = 0;
 }
 
-extern "C" int mxc_set_timing(mxc_factor *F, int on) {
+extern "C" int mxc_set_timing(mxc_factor *F, int mask) {
     if (!F) return MXC_INVALID;
     collect_timing(F);
-    F->timing = on != 0;
+    F->timing_mask = (mask == 1) ? 0x3ffu : (unsigned)mask;
     for (double &t : F->t_ms) t = 0;
     for (int64_t &c : F->counts) c = 0;
     return MXC_OK;
@@ -645,11 +663,11 @@ static int run_factor(mxc_factor *F, double beta) {
     hipStream_t sm = F->stream;
     T *W = (T *)F->W;
     size_t t0 = 0, tt = 0;
-    tick(F, &t0);
+    tick(F, 8, &t0);
     int big = 0x7fffffff;
     HIPTRY(hipMemcpyAsync(F->d_fail, &big, sizeof(int), hipMemcpyHostToDevice, sm));
     // assembly: L = 0, scatter A, + beta on the diagonal; also refresh the CSR values
-    tick(F, &tt);
+    tick(F, 0, &tt);
     hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(F->xsize, 256)), dim3(256), 0, sm, W, (long long)F->xsize, (T)0);
     hipLaunchKernelGGL(k_scatter_a<T>, dim3(grid_for(F->nnz_a, 256)), dim3(256), 0, sm, F->d_Ax,
                        F->d_pos, (long long)F->nnz_a, W);
@@ -660,7 +678,8 @@ static int run_factor(mxc_factor *F, double beta) {
                        F->d_Ax, F->d_src, (long long)F->nnz_full, F->d_vals);
     tock(F, 0, tt);
     for (const Launch &L : F->factor_launches) {
-        tick(F, &tt);
+        const int cls = L.kind == L_UPD ? 1 : L.kind == L_DIAG ? 2 : L.kind == L_TRSM ? 3 : 4;
+        tick(F, cls, &tt);
         switch (L.kind) {
             case L_UPD:
                 hipLaunchKernelGGL((k_tile<T, MODE_SCATTER>), dim3((unsigned)L.count), dim3(256), 0, sm,
@@ -683,7 +702,7 @@ static int run_factor(mxc_factor *F, double beta) {
                 F->counts[3]++;
                 break;
         }
-        tock(F, L.kind == L_UPD ? 1 : L.kind == L_DIAG ? 2 : L.kind == L_TRSM ? 3 : 4, tt);
+        tock(F, cls, tt);
     }
     HIPTRY(hipGetLastError());
     tock(F, 8, t0);
@@ -836,7 +855,7 @@ static int run_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
     int n = (int)F->n;
     int nr = (int)nrhs;
     if (fwd) {
-        tick(F, &tt);
+        tick(F, 5, &tt);
         for (const Launch &L : F->fwd_launches) {
             if (L.kind == L_FDIAG)
                 hipLaunchKernelGGL(k_fwd_diag<T>, dim3((unsigned)L.count), dim3(64), 0, sm,
@@ -849,7 +868,7 @@ static int run_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
         tock(F, 5, tt);
     }
     if (bwd) {
-        tick(F, &tt);
+        tick(F, 6, &tt);
         for (const Launch &L : F->bwd_launches) {
             if (L.kind == L_BDIAG)
                 hipLaunchKernelGGL(k_bwd_diag<T>, dim3((unsigned)L.count), dim3(64), 0, sm,
@@ -872,7 +891,7 @@ static int solve_device(mxc_factor *F, const double *dB, int64_t nrhs, double *d
     T *Xw = (T *)F->d_work;
     unsigned g = grid_for(F->n, 256, 1LL << 30);
     size_t tt = 0;
-    tick(F, &tt);
+    tick(F, 9, &tt);
     hipLaunchKernelGGL(k_perm_gather<T>, dim3(g), dim3(256), 0, sm, dB, F->d_perm, (long long)F->n,
                        (int)nrhs, Xw);
     int rc = run_sweep<T>(F, Xw, nrhs, true, true);
@@ -984,7 +1003,7 @@ extern "C" int mxc_residual(mxc_factor *F, const double *B, const double *X, int
         dR = F->d_r;
     }
     size_t tt = 0;
-    tick(F, &tt);
+    tick(F, 7, &tt);
     launch_residual(F, dB, dX, dR, nrhs);
     tock(F, 7, tt);
     HIPTRY(hipGetLastError());
