@@ -127,3 +127,45 @@ def test_invalid_inputs():
     bad = {"Perm": np.arange(A.n), "Super": np.array([0, A.n + 1]), "Lpi": np.array([0, 1]),
            "Lpx": np.array([0, 1]), "Ls": np.array([0])}
     assert cholmod.import_symbolic(A, bad, cm) is None and cm.status == cholmod.CHOLMOD_INVALID
+
+
+def _random_pattern(n, density, seed, dense_rows=0):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    M = sp.random(n, n, density=density, random_state=seed, format="csc")
+    M = M + M.T
+    if dense_rows:
+        rows = rng.choice(n, dense_rows, replace=False)
+        D = sp.lil_matrix((n, n))
+        for r in rows:
+            D[r, :] = 1.0
+            D[:, r] = 1.0
+        M = M + D.tocsc()
+    M = (abs(M) + sp.eye(n) * n).tocsc()
+    T = sp.tril(M).tocsc()
+    T.sort_indices()
+    return problems.SymMatrix(n=n, p=T.indptr.astype(np.int64), i=T.indices.astype(np.int64),
+                              x=T.data.astype(np.float64), name=f"rand{n}_{seed}")
+
+
+def test_amd_bitexact_vs_reference(ref_lib):
+    """The product's AMD + analysis reproduces cholmod_l_analyze with CHOLMOD_AMD."""
+    mats = _matrices() + [problems.laplace2d_5pt(200),
+                          _random_pattern(300, 0.02, 1), _random_pattern(500, 0.01, 2, dense_rows=3),
+                          _random_pattern(64, 0.3, 3), _random_pattern(1000, 0.004, 4, dense_rows=1)]
+    for A in mats:
+        R = ref_lib.RefFactor(A, ref_lib.AMD)
+        sym = R.symbolic()
+        cm = cholmod.Common(ordering=cholmod.CHOLMOD_AMD)
+        L = cholmod.analyze(A, cm)
+        assert L is not None and cm.status == 0
+        _check_equal(L, R, sym)
+
+
+def test_amd_golden_laplace100():
+    """Without the reference: AMD on the 100x100 Laplacian reproduces the committed digests."""
+    gold = json.loads((GOLD / "symbolic_golden.json").read_text())["laplace2d_100_amd"]
+    cm = cholmod.Common(ordering=cholmod.CHOLMOD_AMD)
+    L = cholmod.analyze(problems.laplace2d_5pt(100), cm)
+    for k, a in FIELDS:
+        assert _digest(getattr(L, a)) == gold["sha256"][k], k
