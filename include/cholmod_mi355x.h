@@ -150,6 +150,12 @@ int mxc_residual(mxc_factor *F, const double *B, const double *X, int64_t nrhs, 
 int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location,
                      double tol, int maxiter, double *out);
 
+/* Options.  MXC_OPT_GRAPHS (default 1): capture the factorization and the
+ * solve sweeps into hipGraphs on first use and replay them afterwards (the
+ * launch sequence depends only on the pattern); 0 enqueues every launch. */
+#define MXC_OPT_GRAPHS 1
+int mxc_set_option(mxc_factor *F, int option, int64_t value);
+
 /* Wait for all work queued on the factor's stream. */
 int mxc_synchronize(mxc_factor *F);
 

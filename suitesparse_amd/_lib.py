@@ -47,6 +47,7 @@ _PROTOS = {
     "mxc_residual": (i32, [vp, vp, vp, i64, vp, i32]),
     "mxc_solve_refine": (i32, [vp, vp, i64, vp, i32, f64, i32, vp]),
     "mxc_synchronize": (i32, [vp]),
+    "mxc_set_option": (i32, [vp, i32, i64]),
     "mxc_set_timing": (i32, [vp, i32]),
     "mxc_get_timing": (i32, [vp, vp, vp]),
     "mxc_last_error": (C.c_char_p, []),

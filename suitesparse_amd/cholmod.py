@@ -161,6 +161,10 @@ class Factor:
             mask = 1 if classes is None else sum(1 << c for c in classes) | (1 << 30)
         check("mxc_set_timing", lib().mxc_set_timing(self._num, int(mask)))
 
+    def set_graphs(self, on: bool):
+        """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""
+        check("mxc_set_option", lib().mxc_set_option(self._num, 1, int(bool(on))))
+
     def kernel_flops(self) -> dict:
         """Algorithmic flops of one factorization by kernel class."""
         info = np.zeros(8, dtype=np.int64)

@@ -59,12 +59,18 @@ __global__ void k_scatter_a(const double *__restrict__ Ax, const long long *__re
     for (; p < nnz; p += stride) W[pos[p]] = (T)Ax[p];
 }
 
+// beta is read from device memory so that a captured factorization graph
+// serves every beta
 template <typename T>
-__global__ void k_add_diag(const long long *__restrict__ dpos, long long n, double beta,
-                           T *__restrict__ W) {
+__global__ void k_add_diag(const long long *__restrict__ dpos, long long n,
+                           const double *__restrict__ beta, T *__restrict__ W) {
+    const double b = *beta;
+    if (b == 0.0) return;
     long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) W[dpos[k]] += (T)beta;
+    if (k < n) W[dpos[k]] += (T)b;
 }
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
 
 // ---------------------------------------------------------------------------
 // Dense 64x64 update tile on MFMA (see UpdDesc).  256 threads = 4 waves; wave
@@ -79,8 +85,9 @@ __global__ __launch_bounds__(256) void k_tile(const UpdDesc *__restrict__ descs,
                                               const Tile *__restrict__ tiles, T *__restrict__ W,
                                               const int *__restrict__ Ls) {
     typedef typename MfmaOp<T>::acc_t acc_t;
-    __shared__ T Xs[BK][TILE + LDS_PAD];
-    __shared__ T Ys[BK][TILE + LDS_PAD];
+    constexpr int PER = (BK * TILE) / 256;      // elements of X (and of Y) each thread stages
+    __shared__ T Xs[2][BK][TILE + LDS_PAD];
+    __shared__ T Ys[2][BK][TILE + LDS_PAD];
     __shared__ int relr[TILE];
     __shared__ int relc[TILE];
 
@@ -122,36 +129,46 @@ __global__ __launch_bounds__(256) void k_tile(const UpdDesc *__restrict__ descs,
 
     const T *__restrict__ Xp = W + d.x_off;
     const T *__restrict__ Yp = W + d.y_off;
+    // thread t stages rows r = t % 64 of chunk columns k = t/64 + 4q, q < PER
+    const int r = tid & (TILE - 1), kq = tid >> 6;
+    const bool rowy = (i0 + r) < d.M, rowx = (j0 + r) < d.N;
+    const T *yrow = Yp + (i0 + r);
+    const T *xrow = Xp + (j0 + r);
+    T ry[PER], rx[PER];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int gk = k0 + kq + 4 * q;
+            const bool kin = gk < d.K;
+            ry[q] = (kin && rowy) ? yrow[(long long)gk * d.ldy] : (T)0;
+            rx[q] = (kin && rowx) ? xrow[(long long)gk * d.ldx] : (T)0;
+        }
+    };
+    fetch(0);
+    int buf = 0;
     for (int k0 = 0; k0 < d.K; k0 += BK) {
 #pragma unroll
-        for (int q = 0; q < (BK * TILE) / 256; q++) {
-            int e = tid + 256 * q;
-            int r = e & (TILE - 1), k = e >> 6;
-            int gk = k0 + k;
-            int gi = i0 + r, gj = j0 + r;
-            T vy = 0, vx = 0;
-            if (gk < d.K) {
-                if (gi < d.M) vy = Yp[gi + (long long)gk * d.ldy];
-                if (gj < d.N) vx = Xp[gj + (long long)gk * d.ldx];
-            }
-            Ys[k][r] = vy;
-            Xs[k][r] = vx;
+        for (int q = 0; q < PER; q++) {
+            Ys[buf][kq + 4 * q][r] = ry[q];
+            Xs[buf][kq + 4 * q][r] = rx[q];
         }
         __syncthreads();
+        if (k0 + BK < d.K) fetch(k0 + BK);      // next chunk in flight during the MFMAs
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             T xa[2], yb[2];
 #pragma unroll
-            for (int m = 0; m < 2; m++) xa[m] = Xs[kk + (lane >> 4)][wc * 32 + m * 16 + (lane & 15)];
+            for (int m = 0; m < 2; m++) xa[m] = Xs[buf][kk + (lane >> 4)][wc * 32 + m * 16 + (lane & 15)];
 #pragma unroll
-            for (int n = 0; n < 2; n++) yb[n] = Ys[kk + (lane >> 4)][wr * 32 + n * 16 + (lane & 15)];
+            for (int n = 0; n < 2; n++) yb[n] = Ys[buf][kk + (lane >> 4)][wr * 32 + n * 16 + (lane & 15)];
 #pragma unroll
             for (int m = 0; m < 2; m++)
 #pragma unroll
                 for (int n = 0; n < 2; n++) acc[m][n] = MfmaOp<T>::mma(xa[m], yb[n], acc[m][n]);
         }
-        __syncthreads();
+        buf ^= 1;
     }
+    if (MODE == MODE_STORE) __syncthreads();   // in-place TRSM: every read of the panel is done
 
     // epilogue: acc[m][n][v] = C(i, j), i = i0 + 32wr + 16n + lane%16,
     //                                   j = j0 + 32wc + 16m + row(lane, v)

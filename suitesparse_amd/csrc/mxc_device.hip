@@ -20,7 +20,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <map>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -120,6 +122,7 @@ struct mxc_factor {
     int *d_col = nullptr;
     double *d_vals = nullptr, *d_Ax = nullptr;
     double *d_norm = nullptr;
+    double *d_beta = nullptr;
     void *d_work = nullptr;             // T[n * work_nrhs] solve vector
     double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr;   // FP64 n * work_nrhs
     int64_t work_nrhs = 0;
@@ -127,6 +130,13 @@ struct mxc_factor {
     std::vector<Launch> factor_launches;   // per level: UPD, then (DIAG, TRSM, TRAIL)*
     std::vector<Launch> fwd_launches, bwd_launches;
     bool values_loaded = false;
+
+    // hipGraph replay: the launch sequence depends only on the pattern, so it is
+    // captured once and replayed for every refactorization / solve
+    bool use_graphs = true;
+    hipGraph_t g_factor = nullptr;
+    hipGraphExec_t x_factor = nullptr;
+    std::map<int64_t, std::pair<hipGraph_t, hipGraphExec_t>> g_sweep;   // key: nrhs*4 + fwd*2 + bwd
 
     // algorithmic flops per factorization, by kernel class
     double flops_update = 0, flops_potrf = 0, flops_trsm = 0, flops_trail = 0;
@@ -510,12 +520,22 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     if ((rc = dalloc(F, &F->d_Ax, (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if ((rc = dalloc(F, &F->d_fail, 1))) return rc;
     if ((rc = dalloc(F, &F->d_norm, 64))) return rc;
+    if ((rc = dalloc(F, &F->d_beta, 1))) return rc;
     HIPTRY(hipStreamSynchronize(sm));
     return MXC_OK;
 }
 
+static void drop_sweep_graphs(mxc_factor *F) {
+    for (auto &kv : F->g_sweep) {
+        if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+        if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+    }
+    F->g_sweep.clear();
+}
+
 static int ensure_work(mxc_factor *F, int64_t nrhs) {
     if (F->work_nrhs >= nrhs) return MXC_OK;
+    drop_sweep_graphs(F);
     size_t esz = F->precision == MXC_FP64 ? sizeof(double) : sizeof(float);
     if (F->d_work) { (void)hipFree(F->d_work); F->d_work = nullptr; }
     if (F->d_b) { (void)hipFree(F->d_b); F->d_b = nullptr; }
@@ -532,9 +552,12 @@ static int ensure_work(mxc_factor *F, int64_t nrhs) {
 
 extern "C" void mxc_factor_free(mxc_factor *F) {
     if (!F) return;
+    drop_sweep_graphs(F);
+    if (F->x_factor) (void)hipGraphExecDestroy(F->x_factor);
+    if (F->g_factor) (void)hipGraphDestroy(F->g_factor);
     void *ptrs[] = {F->W, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
                     F->d_diag, F->d_stask, F->d_fail, F->d_rowptr, F->d_src, F->d_col,
-                    F->d_vals, F->d_Ax, F->d_norm, F->d_work, F->d_b, F->d_x, F->d_r};
+                    F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_b, F->d_x, F->d_r};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
@@ -658,22 +681,22 @@ static inline unsigned grid_for(long long n, int bs, long long cap = 65536) {
     return (unsigned)g;
 }
 
+// Enqueue one whole factorization (values already in d_Ax, beta in d_beta).
+// Kernel launches only, so the sequence can be captured into a graph.
 template <typename T>
-static int run_factor(mxc_factor *F, double beta) {
+static int enqueue_factor(mxc_factor *F) {
     hipStream_t sm = F->stream;
     T *W = (T *)F->W;
     size_t t0 = 0, tt = 0;
     tick(F, 8, &t0);
-    int big = 0x7fffffff;
-    HIPTRY(hipMemcpyAsync(F->d_fail, &big, sizeof(int), hipMemcpyHostToDevice, sm));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_fail, INT_MAX);
     // assembly: L = 0, scatter A, + beta on the diagonal; also refresh the CSR values
     tick(F, 0, &tt);
     hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(F->xsize, 256)), dim3(256), 0, sm, W, (long long)F->xsize, (T)0);
     hipLaunchKernelGGL(k_scatter_a<T>, dim3(grid_for(F->nnz_a, 256)), dim3(256), 0, sm, F->d_Ax,
                        F->d_pos, (long long)F->nnz_a, W);
-    if (beta != 0)
-        hipLaunchKernelGGL(k_add_diag<T>, dim3(grid_for(F->n, 256, 1LL << 30)), dim3(256), 0, sm,
-                           F->d_dpos, (long long)F->n, beta, W);
+    hipLaunchKernelGGL(k_add_diag<T>, dim3(grid_for(F->n, 256, 1LL << 30)), dim3(256), 0, sm,
+                       F->d_dpos, (long long)F->n, (const double *)F->d_beta, W);
     hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(F->nnz_full, 256)), dim3(256), 0, sm,
                        F->d_Ax, F->d_src, (long long)F->nnz_full, F->d_vals);
     tock(F, 0, tt);
@@ -709,6 +732,34 @@ static int run_factor(mxc_factor *F, double beta) {
     return MXC_OK;
 }
 
+// Run the factorization: replay the captured graph (capturing it on first
+// use), or enqueue directly when per-launch timing is on or graphs are off.
+template <typename T>
+static int run_factor(mxc_factor *F) {
+    hipStream_t sm = F->stream;
+    if (!F->use_graphs || F->timing_mask != 0) return enqueue_factor<T>(F);
+    if (!F->x_factor) {
+        HIPTRY(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_factor<T>(F);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(sm, &g);
+        if (rc != MXC_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return set_hip_err(e, "hipStreamEndCapture", __LINE__);
+        e = hipGraphInstantiate(&F->x_factor, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            F->x_factor = nullptr;
+            return set_hip_err(e, "hipGraphInstantiate", __LINE__);
+        }
+        F->g_factor = g;
+    }
+    HIPTRY(hipGraphLaunch(F->x_factor, sm));
+    return MXC_OK;
+}
+
 // Not positive definite: CHOLMOD's result is the factor of the leading
 // (minor) columns, with the failing supernode refactored for its first
 // info-1 columns and everything after it zero
@@ -726,7 +777,8 @@ extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, doub
     if (F->nnz_a > 0)
         HIPTRY(hipMemcpyAsync(F->d_Ax, Ax, F->nnz_a * sizeof(double),
                               location == MXC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, sm));
-    int rc = F->precision == MXC_FP64 ? run_factor<double>(F, beta) : run_factor<float>(F, beta);
+    HIPTRY(hipMemcpyAsync(F->d_beta, &beta, sizeof(double), hipMemcpyHostToDevice, sm));
+    int rc = F->precision == MXC_FP64 ? run_factor<double>(F) : run_factor<float>(F);
     if (rc) return rc;
     int fail = 0;
     HIPTRY(hipMemcpyAsync(&fail, F->d_fail, sizeof(int), hipMemcpyDeviceToHost, sm));
@@ -848,7 +900,7 @@ extern "C" int mxc_factor_get_Lx(const mxc_factor *F, double *Lx) {
 // solves
 // ---------------------------------------------------------------------------
 template <typename T>
-static int run_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
+static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
     hipStream_t sm = F->stream;
     const T *W = (const T *)F->W;
     size_t tt = 0;
@@ -881,6 +933,37 @@ static int run_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
         tock(F, 6, tt);
     }
     HIPTRY(hipGetLastError());
+    return MXC_OK;
+}
+
+// Sweeps over the factor's own work vector are replayed from a graph per
+// (nrhs, direction); sweeps over caller memory are enqueued directly.
+template <typename T>
+static int run_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
+    if (!F->use_graphs || F->timing_mask != 0 || (void *)Xw != F->d_work)
+        return enqueue_sweep<T>(F, Xw, nrhs, fwd, bwd);
+    hipStream_t sm = F->stream;
+    int64_t key = nrhs * 4 + (fwd ? 2 : 0) + (bwd ? 1 : 0);
+    auto it = F->g_sweep.find(key);
+    if (it == F->g_sweep.end()) {
+        HIPTRY(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_sweep<T>(F, Xw, nrhs, fwd, bwd);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(sm, &g);
+        if (rc != MXC_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return set_hip_err(e, "hipStreamEndCapture", __LINE__);
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return set_hip_err(e, "hipGraphInstantiate", __LINE__);
+        }
+        it = F->g_sweep.emplace(key, std::make_pair(g, x)).first;
+    }
+    HIPTRY(hipGraphLaunch(it->second.second, sm));
     return MXC_OK;
 }
 
@@ -1086,6 +1169,17 @@ extern "C" int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, do
         HIPTRY(hipMemcpyAsync(X, dX, F->n * nrhs * sizeof(double), hipMemcpyDeviceToHost, F->stream));
     HIPTRY(hipStreamSynchronize(F->stream));
     return MXC_OK;
+}
+
+extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
+    if (!F) return MXC_INVALID;
+    switch (option) {
+        case MXC_OPT_GRAPHS:
+            F->use_graphs = value != 0;
+            return MXC_OK;
+        default:
+            return MXC_INVALID;
+    }
 }
 
 extern "C" int mxc_synchronize(mxc_factor *F) {
