@@ -156,6 +156,14 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
 #define MXC_OPT_GRAPHS 1
 int mxc_set_option(mxc_factor *F, int option, int64_t value);
 
+/* As mxc_solve_refine for a block-diagonal batch of independent systems: rows
+ * segoff[g] .. segoff[g+1]-1 are system g (segoff[0] = 0, segoff[nseg] = n), and
+ * refinement continues until every system's relative residual is <= tol.
+ * out[1], out[2] are the max over systems and columns. */
+int mxc_solve_refine_segments(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location,
+                              double tol, int maxiter, int64_t nseg, const int64_t *segoff,
+                              double *out);
+
 /* Wait for all work queued on the factor's stream. */
 int mxc_synchronize(mxc_factor *F);
 

@@ -354,9 +354,10 @@ def solve(sys: int, L: Factor, B, cm: Common):
     return X
 
 
-def solve_refine(L: Factor, B, cm: Common, tol: float = 1e-10, maxiter: int = 10):
+def solve_refine(L: Factor, B, cm: Common, tol: float = 1e-10, maxiter: int = 10, segments=None):
     """x = A\\b with FP64 iterative refinement through the (FP32 or FP64) factor.
-    Returns (X, info) with info = {iterations, relres, relres0}."""
+    ``segments`` (row offsets of independent systems) makes the stopping test
+    per system.  Returns (X, info) with info = {iterations, relres, relres0}."""
     cm.status = CHOLMOD_OK
     dev = _is_device(B)
     if dev:
@@ -370,8 +371,13 @@ def solve_refine(L: Factor, B, cm: Common, tol: float = 1e-10, maxiter: int = 10
         X = np.zeros_like(Bc, order="F")
         loc = _lib.HOST
     out = np.zeros(3)
-    st = lib().mxc_solve_refine(L._num, _ptr(Bc), Bc.shape[1], _ptr(X), loc, float(tol),
-                                int(maxiter), _ptr(out))
+    if segments is None:
+        st = lib().mxc_solve_refine(L._num, _ptr(Bc), Bc.shape[1], _ptr(X), loc, float(tol),
+                                    int(maxiter), _ptr(out))
+    else:
+        seg = np.ascontiguousarray(segments, dtype=np.int64)
+        st = lib().mxc_solve_refine_segments(L._num, _ptr(Bc), Bc.shape[1], _ptr(X), loc, float(tol),
+                                             int(maxiter), len(seg) - 1, _ptr(seg), _ptr(out))
     if st != 0:
         cm.status = st
         return None, None

@@ -73,32 +73,36 @@ __global__ void k_add_diag(const long long *__restrict__ dpos, long long n,
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
 // ---------------------------------------------------------------------------
-// Dense 64x64 update tile on MFMA (see UpdDesc).  256 threads = 4 waves; wave
-// w owns C rows [32*(w&1), +32) x C cols [32*(w>>1), +32) as 2x2 16x16 MFMA
-// blocks.  The MFMA is fed X (C-column side) as its A operand and Y (C-row
-// side) as its B operand, so each lane's 4 accumulators hold 4 C columns of
-// one C row and 16 consecutive lanes hold 16 consecutive C rows: stores into
-// the column-major target are 16-element contiguous runs.
+// Dense TS x TS update tile on MFMA (see UpdDesc), TS = 64 (4 waves, 256
+// threads) or 32 (1 wave, 64 threads, for the many small (d,s) pairs).  Each
+// wave owns a 32x32 block of C as 2x2 16x16 MFMA blocks.  The MFMA is fed X
+// (C-column side) as its A operand and Y (C-row side) as its B operand, so
+// each lane's 4 accumulators hold 4 C columns of one C row and 16
+// consecutive lanes hold 16 consecutive C rows: stores into the
+// column-major target are 16-element contiguous runs.
 // ---------------------------------------------------------------------------
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_tile(const UpdDesc *__restrict__ descs,
-                                              const Tile *__restrict__ tiles, T *__restrict__ W,
-                                              const int *__restrict__ Ls) {
+template <typename T, int MODE, int TS>
+__global__ __launch_bounds__(TS * TS / 16) void k_tile(const UpdDesc *__restrict__ descs,
+                                                      const Tile *__restrict__ tiles, T *__restrict__ W,
+                                                      const int *__restrict__ Ls) {
     typedef typename MfmaOp<T>::acc_t acc_t;
-    constexpr int PER = (BK * TILE) / 256;      // elements of X (and of Y) each thread stages
-    __shared__ T Xs[2][BK][TILE + LDS_PAD];
-    __shared__ T Ys[2][BK][TILE + LDS_PAD];
-    __shared__ int relr[TILE];
-    __shared__ int relc[TILE];
+    constexpr int NT = TS * TS / 16;            // threads: 64 per 32x32 wave block
+    constexpr int KG = NT / TS;                 // k groups staged per pass
+    constexpr int PER = (BK * TS) / NT;         // elements of X (and of Y) each thread stages
+    constexpr int WPR = TS / 32;                // waves per tile row
+    __shared__ T Xs[2][BK][TS + LDS_PAD];
+    __shared__ T Ys[2][BK][TS + LDS_PAD];
+    __shared__ int relr[TS];
+    __shared__ int relc[TS];
 
     const Tile tl = tiles[blockIdx.x];
     const UpdDesc d = descs[tl.desc];
-    const int i0 = tl.ti * TILE, j0 = tl.tj * TILE;
+    const int i0 = tl.ti * TS, j0 = tl.tj * TS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave & 1, wc = wave >> 1;
+    const int wr = wave % WPR, wc = wave / WPR;
 
     if (MODE == MODE_SCATTER) {
-        if (tid < TILE) {
+        if (tid < TS) {
             int i = i0 + tid;
             int r = -1;
             if (i < d.M) {
@@ -115,9 +119,9 @@ __global__ __launch_bounds__(256) void k_tile(const UpdDesc *__restrict__ descs,
                 }
             }
             relr[tid] = r;
-        } else if (tid < 2 * TILE) {
-            int j = j0 + tid - TILE;
-            relc[tid - TILE] = (j < d.N) ? (Ls[d.ls_off + j] - d.base) : -1;
+        } else if (tid < 2 * TS) {
+            int j = j0 + tid - TS;
+            relc[tid - TS] = (j < d.N) ? (Ls[d.ls_off + j] - d.base) : -1;
         }
     }
 
@@ -129,8 +133,8 @@ __global__ __launch_bounds__(256) void k_tile(const UpdDesc *__restrict__ descs,
 
     const T *__restrict__ Xp = W + d.x_off;
     const T *__restrict__ Yp = W + d.y_off;
-    // thread t stages rows r = t % 64 of chunk columns k = t/64 + 4q, q < PER
-    const int r = tid & (TILE - 1), kq = tid >> 6;
+    // thread t stages row r = t % TS of chunk columns k = t/TS + KG*q, q < PER
+    const int r = tid % TS, kq = tid / TS;
     const bool rowy = (i0 + r) < d.M, rowx = (j0 + r) < d.N;
     const T *yrow = Yp + (i0 + r);
     const T *xrow = Xp + (j0 + r);
@@ -138,7 +142,7 @@ __global__ __launch_bounds__(256) void k_tile(const UpdDesc *__restrict__ descs,
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int gk = k0 + kq + 4 * q;
+            const int gk = k0 + kq + KG * q;
             const bool kin = gk < d.K;
             ry[q] = (kin && rowy) ? yrow[(long long)gk * d.ldy] : (T)0;
             rx[q] = (kin && rowx) ? xrow[(long long)gk * d.ldx] : (T)0;
@@ -149,8 +153,8 @@ __global__ __launch_bounds__(256) void k_tile(const UpdDesc *__restrict__ descs,
     for (int k0 = 0; k0 < d.K; k0 += BK) {
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            Ys[buf][kq + 4 * q][r] = ry[q];
-            Xs[buf][kq + 4 * q][r] = rx[q];
+            Ys[buf][kq + KG * q][r] = ry[q];
+            Xs[buf][kq + KG * q][r] = rx[q];
         }
         __syncthreads();
         if (k0 + BK < d.K) fetch(k0 + BK);      // next chunk in flight during the MFMAs

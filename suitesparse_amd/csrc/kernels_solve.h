@@ -231,6 +231,29 @@ __global__ void k_sumsq(const double *__restrict__ V, long long n, int nrhs,
     }
 }
 
+// out[g * nrhs + r] = sum over rows of segment g of v[i, r]^2; one block per segment
+__global__ void k_seg_sumsq(const double *__restrict__ V, long long n, int nrhs,
+                            const long long *__restrict__ segoff, double *__restrict__ out) {
+    __shared__ double red[256];
+    const int g = blockIdx.x;
+    const long long a = segoff[g], b = segoff[g + 1];
+    for (int r = 0; r < nrhs; r++) {
+        double s = 0;
+        for (long long i = a + threadIdx.x; i < b; i += blockDim.x) {
+            double v = V[i + r * n];
+            s += v * v;
+        }
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(long long)g * nrhs + r] = red[0];
+        __syncthreads();
+    }
+}
+
 // X += D (converted), over n*nrhs entries
 template <typename T>
 __global__ void k_axpy_update(double *__restrict__ X, const T *__restrict__ D, long long len) {

@@ -92,7 +92,16 @@ struct Launch {
     int kind;          // see enum below
     long long begin;   // first task
     long long count;   // number of tasks (grid size)
+    int tile;          // tile edge of k_tile launches (32 or 64)
 };
+
+// Tile-size buckets over (rows M, columns N) of an update block: 5 x 5 classes.
+static inline int tile_bucket(int M, int N) {
+    int mb = M <= 16 ? 0 : M <= 32 ? 1 : M <= 64 ? 2 : M <= 128 ? 3 : 4;
+    int nb = N <= 8 ? 0 : N <= 16 ? 1 : N <= 32 ? 2 : N <= 64 ? 3 : 4;
+    return mb * 5 + nb;
+}
+static constexpr int NBUCKET = 25;
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FDIAG = 4, L_FOFF = 5, L_BOFF = 6, L_BDIAG = 7 };
 
 struct mxc_factor {
@@ -130,6 +139,9 @@ struct mxc_factor {
     std::vector<Launch> factor_launches;   // per level: UPD, then (DIAG, TRSM, TRAIL)*
     std::vector<Launch> fwd_launches, bwd_launches;
     bool values_loaded = false;
+
+    // tile edge per (M, N) bucket; default 32 for blocks up to 64 x 32
+    int tune[NBUCKET];
 
     // hipGraph replay: the launch sequence depends only on the pattern, so it is
     // captured once and replayed for every refactorization / solve
@@ -333,35 +345,43 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     }
     F->npairs = npairs;
 
-    // ---- flatten: descriptors, tiles, diag tasks, launches
+    // ---- flatten: descriptors, tiles, diag tasks, launches.  Each phase's
+    // blocks are split by tile class (32 or 64, from the tuning table) into
+    // one launch per class.
     std::vector<UpdDesc> descs;
     std::vector<Tile> tiles;
     std::vector<DiagTask> diags;
     F->factor_launches.clear();
-    auto add_tiles = [&](const UpdDesc &u, bool lower_only) {
-        int di = (int)descs.size();
-        descs.push_back(u);
-        int64_t mt = ceil_div(u.M, TILE), nt = ceil_div(u.N, TILE);
-        for (int64_t ti = 0; ti < mt; ti++)
-            for (int64_t tj = 0; tj < nt; tj++) {
-                if (lower_only && ti < tj) continue;   // tile wholly above the diagonal
-                tiles.push_back(Tile{di, (int)ti, (int)tj});
-            }
-    };
     int64_t nupd = 0;
+    auto emit = [&](int kind, const std::vector<UpdDesc> &blocks, bool lower_only) {
+        long long total = 0;
+        for (int ts : {32, 64}) {
+            long long b = (long long)tiles.size();
+            for (const UpdDesc &u : blocks) {
+                if (F->tune[tile_bucket(u.M, u.N)] != ts) continue;
+                int di = (int)descs.size();
+                descs.push_back(u);
+                int64_t mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
+                for (int64_t ti = 0; ti < mt; ti++)
+                    for (int64_t tj = 0; tj < nt; tj++) {
+                        if (lower_only && ti < tj) continue;   // tile wholly above the diagonal
+                        tiles.push_back(Tile{di, (int)ti, (int)tj});
+                    }
+            }
+            long long c = (long long)tiles.size() - b;
+            if (c > 0) F->factor_launches.push_back(Launch{kind, b, c, ts});
+            total += c;
+        }
+        return total;
+    };
     for (int l = 0; l < nlev; l++) {
-        long long b = (long long)tiles.size();
-        for (const UpdDesc &u : pairs_by_level[l]) add_tiles(u, true);
-        long long c = (long long)tiles.size() - b;
-        nupd += c;
-        if (c > 0) F->factor_launches.push_back(Launch{L_UPD, b, c});
+        nupd += emit(L_UPD, pairs_by_level[l], true);
         int64_t maxsteps = 0;
         for (int64_t s : by_level[l])
             maxsteps = std::max(maxsteps, ceil_div(S->Super[s + 1] - S->Super[s], NB));
         for (int64_t t = 0; t < maxsteps; t++) {
             long long db = (long long)diags.size();
-            long long tb = (long long)tiles.size();
-            std::vector<UpdDesc> trail;
+            std::vector<UpdDesc> trsm, trail;
             for (int64_t s : by_level[l]) {
                 int64_t nscol = S->Super[s + 1] - S->Super[s];
                 int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
@@ -396,7 +416,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.N = (int)nb;
                     u.c_off = u.y_off;
                     u.ldc = (int)nsrow;
-                    add_tiles(u, false);
+                    trsm.push_back(u);
                     F->flops_trsm += (double)u.M * nb * nb;
                 }
                 if (nscol > r0) {
@@ -411,20 +431,14 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.c_off = psx + r0 + r0 * nsrow;
                     u.ldc = (int)nsrow;
                     trail.push_back(u);
-                    {
-                        double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
-                        F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
-                    }
+                    double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
+                    F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
                 }
             }
             long long dc = (long long)diags.size() - db;
-            long long tc = (long long)tiles.size() - tb;
-            if (dc > 0) F->factor_launches.push_back(Launch{L_DIAG, db, dc});
-            if (tc > 0) F->factor_launches.push_back(Launch{L_TRSM, tb, tc});
-            long long rb = (long long)tiles.size();
-            for (const UpdDesc &u : trail) add_tiles(u, true);
-            long long rc = (long long)tiles.size() - rb;
-            if (rc > 0) F->factor_launches.push_back(Launch{L_TRAIL, rb, rc});
+            if (dc > 0) F->factor_launches.push_back(Launch{L_DIAG, db, dc, 0});
+            emit(L_TRSM, trsm, false);
+            emit(L_TRAIL, trail, true);
         }
     }
     F->nupd_tiles = nupd;
@@ -462,7 +476,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
             long long b = (long long)st.size();
             for (int64_t s : by_level[l])
                 if (t * NB < S->Super[s + 1] - S->Super[s]) st.push_back(make_task(s, t, 0, 0));
-            if ((long long)st.size() > b) F->fwd_launches.push_back(Launch{L_FDIAG, b, (long long)st.size() - b});
+            if ((long long)st.size() > b) F->fwd_launches.push_back(Launch{L_FDIAG, b, (long long)st.size() - b, 0});
             b = (long long)st.size();
             for (int64_t s : by_level[l]) {
                 int64_t nscol = S->Super[s + 1] - S->Super[s];
@@ -472,7 +486,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                 for (int64_t r = r0; r < nsrow; r += ROWS)
                     st.push_back(make_task(s, t, r, std::min<int64_t>(nsrow, r + ROWS)));
             }
-            if ((long long)st.size() > b) F->fwd_launches.push_back(Launch{L_FOFF, b, (long long)st.size() - b});
+            if ((long long)st.size() > b) F->fwd_launches.push_back(Launch{L_FOFF, b, (long long)st.size() - b, 0});
         }
     }
     const int BROWS = 1024;
@@ -490,11 +504,11 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                 for (int64_t r = r0; r < nsrow; r += BROWS)
                     st.push_back(make_task(s, t, r, std::min<int64_t>(nsrow, r + BROWS)));
             }
-            if ((long long)st.size() > b) F->bwd_launches.push_back(Launch{L_BOFF, b, (long long)st.size() - b});
+            if ((long long)st.size() > b) F->bwd_launches.push_back(Launch{L_BOFF, b, (long long)st.size() - b, 0});
             b = (long long)st.size();
             for (int64_t s : by_level[l])
                 if (t * NB < S->Super[s + 1] - S->Super[s]) st.push_back(make_task(s, t, 0, 0));
-            if ((long long)st.size() > b) F->bwd_launches.push_back(Launch{L_BDIAG, b, (long long)st.size() - b});
+            if ((long long)st.size() > b) F->bwd_launches.push_back(Launch{L_BDIAG, b, (long long)st.size() - b, 0});
         }
     }
 
@@ -576,6 +590,10 @@ extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const
     mxc_factor *F = new (std::nothrow) mxc_factor();
     if (!F) return MXC_OUT_OF_MEMORY;
     F->precision = precision;
+    for (int b = 0; b < NBUCKET; b++) {
+        int mb = b / 5, nbk = b % 5;
+        F->tune[b] = (mb <= 2 && nbk <= 2) ? 32 : 64;
+    }
     hipError_t e = hipSetDevice(g_device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -681,6 +699,16 @@ static inline unsigned grid_for(long long n, int bs, long long cap = 65536) {
     return (unsigned)g;
 }
 
+template <typename T, int MODE>
+static void launch_tile(mxc_factor *F, const Launch &L, T *W) {
+    if (L.tile == 32)
+        hipLaunchKernelGGL((k_tile<T, MODE, 32>), dim3((unsigned)L.count), dim3(64), 0, F->stream,
+                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+    else
+        hipLaunchKernelGGL((k_tile<T, MODE, 64>), dim3((unsigned)L.count), dim3(256), 0, F->stream,
+                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+}
+
 // Enqueue one whole factorization (values already in d_Ax, beta in d_beta).
 // Kernel launches only, so the sequence can be captured into a graph.
 template <typename T>
@@ -705,8 +733,7 @@ static int enqueue_factor(mxc_factor *F) {
         tick(F, cls, &tt);
         switch (L.kind) {
             case L_UPD:
-                hipLaunchKernelGGL((k_tile<T, MODE_SCATTER>), dim3((unsigned)L.count), dim3(256), 0, sm,
-                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+                launch_tile<T, MODE_SCATTER>(F, L, W);
                 F->counts[0]++;
                 break;
             case L_DIAG:
@@ -715,13 +742,11 @@ static int enqueue_factor(mxc_factor *F) {
                 F->counts[1]++;
                 break;
             case L_TRSM:
-                hipLaunchKernelGGL((k_tile<T, MODE_STORE>), dim3((unsigned)L.count), dim3(256), 0, sm,
-                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+                launch_tile<T, MODE_STORE>(F, L, W);
                 F->counts[2]++;
                 break;
             case L_TRAIL:
-                hipLaunchKernelGGL((k_tile<T, MODE_TRAIL>), dim3((unsigned)L.count), dim3(256), 0, sm,
-                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+                launch_tile<T, MODE_TRAIL>(F, L, W);
                 F->counts[3]++;
                 break;
         }
@@ -1095,14 +1120,22 @@ extern "C" int mxc_residual(mxc_factor *F, const double *B, const double *X, int
     return MXC_OK;
 }
 
-// max over columns of ||V(:,r)|| / ||B(:,r)|| (norms of B cached in bn)
-static int colnorms(mxc_factor *F, const double *dV, int64_t nrhs, std::vector<double> &out) {
+// 2-norms of the columns of V, per segment of rows when segments are given
+// (out[g*nrhs + r]), else of whole columns (out[r]).
+static int colnorms(mxc_factor *F, const double *dV, int64_t nrhs, int64_t nseg, const long long *d_seg,
+                    double *d_out, std::vector<double> &out) {
     hipStream_t sm = F->stream;
-    HIPTRY(hipMemsetAsync(F->d_norm, 0, sizeof(double) * nrhs, sm));
-    hipLaunchKernelGGL(k_sumsq, dim3(grid_for(F->n, 256, 1024)), dim3(256), 0, sm, dV, (long long)F->n,
-                       (int)nrhs, F->d_norm);
-    out.assign(nrhs, 0);
-    HIPTRY(hipMemcpyAsync(out.data(), F->d_norm, sizeof(double) * nrhs, hipMemcpyDeviceToHost, sm));
+    if (nseg > 0) {
+        hipLaunchKernelGGL(k_seg_sumsq, dim3((unsigned)nseg), dim3(256), 0, sm, dV, (long long)F->n,
+                           (int)nrhs, d_seg, d_out);
+        out.assign((size_t)(nseg * nrhs), 0);
+    } else {
+        HIPTRY(hipMemsetAsync(d_out, 0, sizeof(double) * nrhs, sm));
+        hipLaunchKernelGGL(k_sumsq, dim3(grid_for(F->n, 256, 1024)), dim3(256), 0, sm, dV, (long long)F->n,
+                           (int)nrhs, d_out);
+        out.assign((size_t)nrhs, 0);
+    }
+    HIPTRY(hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, sm));
     HIPTRY(hipStreamSynchronize(sm));
     for (auto &v : out) v = std::sqrt(v);
     return MXC_OK;
@@ -1110,11 +1143,24 @@ static int colnorms(mxc_factor *F, const double *dV, int64_t nrhs, std::vector<d
 
 template <typename T>
 static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *dX, double tol,
-                         int maxiter, double *out) {
+                         int maxiter, int64_t nseg, const int64_t *segoff, double *out) {
     hipStream_t sm = F->stream;
     size_t len = (size_t)F->n * nrhs;
     std::vector<double> bn, rn;
-    int rc = colnorms(F, dB, nrhs, bn);
+    long long *d_seg = nullptr;
+    double *d_out = F->d_norm;
+    struct Free {
+        void *a, *b;
+        ~Free() { if (a) (void)hipFree(a); if (b) (void)hipFree(b); }
+    } fr{nullptr, nullptr};
+    if (nseg > 0) {
+        HIPTRY(hipMalloc((void **)&d_seg, sizeof(long long) * (nseg + 1)));
+        fr.a = d_seg;
+        HIPTRY(hipMalloc((void **)&d_out, sizeof(double) * nseg * nrhs));
+        fr.b = d_out;
+        HIPTRY(hipMemcpyAsync(d_seg, segoff, sizeof(long long) * (nseg + 1), hipMemcpyHostToDevice, sm));
+    }
+    int rc = colnorms(F, dB, nrhs, nseg, d_seg, d_out, bn);
     if (rc) return rc;
     // x0 = A \ b
     T *Xw = (T *)F->d_work;
@@ -1126,10 +1172,10 @@ static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *
     double rel = 0, rel0 = -1;
     for (;;) {
         launch_residual(F, dB, dX, F->d_r, nrhs);
-        if ((rc = colnorms(F, F->d_r, nrhs, rn))) return rc;
+        if ((rc = colnorms(F, F->d_r, nrhs, nseg, d_seg, d_out, rn))) return rc;
         rel = 0;
-        for (int64_t r = 0; r < nrhs; r++) {
-            double v = bn[r] > 0 ? rn[r] / bn[r] : rn[r];
+        for (size_t q = 0; q < rn.size(); q++) {
+            double v = bn[q] > 0 ? rn[q] / bn[q] : rn[q];
             rel = std::max(rel, v);
         }
         if (rel0 < 0) rel0 = rel;
@@ -1151,9 +1197,15 @@ static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *
     return MXC_OK;
 }
 
-extern "C" int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location,
-                                double tol, int maxiter, double *out) {
+static int solve_refine_impl(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location,
+                             double tol, int maxiter, int64_t nseg, const int64_t *segoff, double *out) {
     if (!F || !B || !X || nrhs < 0 || nrhs > 64) return MXC_INVALID;
+    if (nseg < 0 || (nseg > 0 && !segoff)) return MXC_INVALID;
+    if (nseg > 0) {
+        if (segoff[0] != 0 || segoff[nseg] != F->n) return MXC_INVALID;
+        for (int64_t g = 0; g < nseg; g++)
+            if (segoff[g + 1] < segoff[g]) return MXC_INVALID;
+    }
     if (!F->values_loaded) return MXC_INVALID;
     if (nrhs == 0 || F->n == 0) return MXC_OK;
     HIPTRY(hipSetDevice(g_device));
@@ -1162,8 +1214,8 @@ extern "C" int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, do
     const double *dB;
     if ((rc = stage_in(F, B, nrhs, location, &dB))) return rc;
     double *dX = location == MXC_DEVICE ? X : F->d_x;
-    rc = F->precision == MXC_FP64 ? refine_device<double>(F, dB, nrhs, dX, tol, maxiter, out)
-                                  : refine_device<float>(F, dB, nrhs, dX, tol, maxiter, out);
+    rc = F->precision == MXC_FP64 ? refine_device<double>(F, dB, nrhs, dX, tol, maxiter, nseg, segoff, out)
+                                  : refine_device<float>(F, dB, nrhs, dX, tol, maxiter, nseg, segoff, out);
     if (rc) return rc;
     if (location != MXC_DEVICE)
         HIPTRY(hipMemcpyAsync(X, dX, F->n * nrhs * sizeof(double), hipMemcpyDeviceToHost, F->stream));
@@ -1180,6 +1232,17 @@ extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
         default:
             return MXC_INVALID;
     }
+}
+
+extern "C" int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location,
+                                double tol, int maxiter, double *out) {
+    return solve_refine_impl(F, B, nrhs, X, location, tol, maxiter, 0, nullptr, out);
+}
+
+extern "C" int mxc_solve_refine_segments(mxc_factor *F, const double *B, int64_t nrhs, double *X,
+                                         int location, double tol, int maxiter, int64_t nseg,
+                                         const int64_t *segoff, double *out) {
+    return solve_refine_impl(F, B, nrhs, X, location, tol, maxiter, nseg, segoff, out);
 }
 
 extern "C" int mxc_synchronize(mxc_factor *F) {
