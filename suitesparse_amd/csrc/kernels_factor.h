@@ -203,77 +203,199 @@ __global__ __launch_bounds__(TS * TS / 16) void k_tile(const UpdDesc *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block Cholesky + triangular inverse, one 256-thread workgroup per
-// 64x64 block held in LDS (a block narrower than 64 is extended by the
-// identity).  Right-looking, two barriers per column.  The lower factor is
-// written back in place (the strict upper triangle is left alone, as dpotrf
-// leaves it) and inv(L) goes to the block's inverse slot (ld nb).  A
-// non-positive pivot at column j records col0 + j in *fail (atomic min).
+// Diagonal block Cholesky + triangular inverse of a 64x64 block (a block
+// narrower than 64 is extended by the identity), one 256-thread workgroup,
+// block-recursive over 16x16 sub-blocks with the 16x16x4 MFMA:
+//   for p = 0..3:  POTRF + inverse of D_pp (one wave, registers)
+//                  panel TRSM  L_tp = A_tp inv(D_pp)'      (MFMA, t > p)
+//                  trailing    A_ij -= L_ip L_jp'          (MFMA, p < j <= i)
+//   inverse:       X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj  (MFMA, block rows i)
+// 18 barriers in all.  The lower factor is written back in place (the strict
+// upper triangle is left alone, as dpotrf leaves it) and inv(L) (zero above
+// the diagonal) goes to the block's inverse slot (ld nb).  A non-positive
+// pivot at column j records col0 + j in *fail (atomic min).
 // ---------------------------------------------------------------------------
+constexpr int DS = NB + 1;     // LDS row stride of the 64x64 block
+
+// value of v in lane k (k wave-uniform), through a scalar register
+__device__ __forceinline__ double lane_bcast(double v, int k) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_bcast(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// acc += X * Y' for 16x16 blocks X at S[rx..][cx..], Y at S[ry..][cy..] (row-major, stride DS)
+template <typename T>
+__device__ __forceinline__ void mma_nt16(const T *S, int rx, int cx, const T *S2, int ry, int cy,
+                                         typename MfmaOp<T>::acc_t &acc, int lane) {
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+        const T a = S[(rx + (lane & 15)) * DS + cx + kk + (lane >> 4)];
+        const T b = S2[(ry + (lane & 15)) * DS + cy + kk + (lane >> 4)];
+        acc = MfmaOp<T>::mma(a, b, acc);
+    }
+}
+
+// acc += X * Y for 16x16 blocks X at S[rx..][cx..], Y at S2[ry..][cy..]
+template <typename T>
+__device__ __forceinline__ void mma_nn16(const T *S, int rx, int cx, const T *S2, int ry, int cy,
+                                         typename MfmaOp<T>::acc_t &acc, int lane) {
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+        const T a = S[(rx + (lane & 15)) * DS + cx + kk + (lane >> 4)];
+        const T b = S2[(ry + kk + (lane >> 4)) * DS + cy + (lane & 15)];
+        acc = MfmaOp<T>::mma(a, b, acc);
+    }
+}
+
+// C at S[r0..][c0..] (op) acc, where acc holds D[row(lane,v)][lane%16] = C(m, n) with
+// m indexing the MFMA A side (rows) and n the B side (columns)
+template <typename T>
+__device__ __forceinline__ void store16(T *S, int r0, int c0, const typename MfmaOp<T>::acc_t &acc,
+                                        int lane, T alpha, bool accumulate) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        T *q = S + (r0 + MfmaOp<T>::row(lane, v)) * DS + c0 + (lane & 15);
+        *q = accumulate ? *q + alpha * acc[v] : alpha * acc[v];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__ tasks,
                                                     T *__restrict__ W, int *__restrict__ fail) {
-    __shared__ T A[NB][NB + 1];
-    __shared__ T X[NB][NB + 1];
-    __shared__ T red[4][NB];
-    __shared__ T dsh[NB];
+    typedef typename MfmaOp<T>::acc_t acc_t;
+    __shared__ T S[NB * DS];       // the block, row-major
+    __shared__ T X[NB * DS];       // its inverse
     __shared__ int sfail;
     const DiagTask t = tasks[blockIdx.x];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nb = t.nb;
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int i = e & (NB - 1), c = e >> 6;
-        T v = 0;
-        if (i < nb && c < nb) {
-            if (c <= i) v = W[t.off + i + (long long)c * t.ld];
-        } else if (i == c) {
-            v = 1;
+    const int np = (nb + 15) >> 4;          // 16-column panels that hold real columns
+    {
+        // all 16 loads of a thread in flight at once, then the LDS stores
+        T v[16];
+        const int i = tid & (NB - 1), c0 = tid >> 6;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int c = c0 + 4 * q;
+            v[q] = (i < nb && c < nb && c <= i) ? W[t.off + i + (long long)c * t.ld] : (T)0;
         }
-        A[i][c] = v;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int c = c0 + 4 * q;
+            S[i * DS + c] = (i >= nb && i == c) ? (T)1 : v[q];
+            X[i * DS + c] = 0;
+        }
     }
     if (tid == 0) sfail = -1;
-    const int ri = tid >> 2, cg = tid & 3;
-    for (int j = 0; j < NB; j++) {
-        __syncthreads();
-        T djj = A[j][j];
-        if (!(djj > (T)0)) {
-            if (tid == 0 && j < nb && sfail < 0) sfail = j;
-            djj = 1;
-        }
-        const T dj = sqrt(djj);
-        if (tid > j && tid < NB) A[tid][j] /= dj;
-        if (tid == 0) dsh[j] = dj;
-        __syncthreads();
-        if (ri > j) {
-            const T lij = A[ri][j];
-            for (int c = j + 1 + cg; c <= ri; c += 4) A[ri][c] -= lij * A[c][j];
-        }
-    }
     __syncthreads();
-    if (tid < NB) A[tid][tid] = dsh[tid];
+
+    for (int p = 0; p < np; p++) {
+        const int R = 16 * p;
+        // (1) POTRF + inverse of the 16x16 diagonal block, one wave, lane i < 16 owns row i
+        if (wave == 0) {
+            T a[16];
+#pragma unroll
+            for (int c = 0; c < 16; c++) a[c] = (lane < 16) ? S[(R + lane) * DS + R + c] : (T)0;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                T djj = lane_bcast(a[j], j);
+                if (!(djj > (T)0)) {
+                    if (lane == 0 && R + j < nb && sfail < 0) sfail = R + j;
+                    djj = 1;
+                }
+                const T dj = sqrt(djj);
+                const T l = (lane > j) ? a[j] / dj : ((lane == j) ? dj : (T)0);
+                a[j] = l;
+#pragma unroll
+                for (int c = j + 1; c < 16; c++) a[c] -= l * lane_bcast(l, c);
+            }
+            if (lane < 16) {
+#pragma unroll
+                for (int c = 0; c < 16; c++)
+                    if (c <= lane) S[(R + lane) * DS + R + c] = a[c];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            // inverse: lane c < 16 forms column c by forward substitution
+            T x[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                T sacc = (i == lane) ? (T)1 : (T)0;
+#pragma unroll
+                for (int k = 0; k < i; k++) sacc -= S[(R + i) * DS + R + k] * x[k];
+                x[i] = sacc / S[(R + i) * DS + R + i];
+            }
+            if (lane < 16) {
+#pragma unroll
+                for (int i = 0; i < 16; i++) X[(R + i) * DS + R + lane] = (i >= lane) ? x[i] : (T)0;
+            }
+        }
+        __syncthreads();
+        // (2) panel TRSM: rows below, 16-row tiles t = p+1..np-1, one wave each
+        if (wave < np - 1 - p) {
+            const int rt = 16 * (p + 1 + wave);
+            acc_t acc = acc_t{0, 0, 0, 0};
+            // C(m = row of inv, n = panel row) = sum_k X(m, k) A(n, k)  ->  L(n, m)
+            mma_nt16<T>(X, R, R, S, rt, R, acc, lane);
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                S[(rt + (lane & 15)) * DS + R + MfmaOp<T>::row(lane, v)] = acc[v];
+        }
+        __syncthreads();
+        // (3) trailing update of the lower block triangle right of the panel
+        {
+            int q = 0;
+            for (int i = p + 1; i < np; i++)
+                for (int j = p + 1; j <= i; j++, q++) {
+                    if ((q & 3) != wave) continue;
+                    acc_t acc = acc_t{0, 0, 0, 0};
+                    // C(m = row in block j, n = row in block i) = sum_k L_jp(m,k) L_ip(n,k)
+                    mma_nt16<T>(S, 16 * j, R, S, 16 * i, R, acc, lane);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        T *dst = S + (16 * i + (lane & 15)) * DS + 16 * j + MfmaOp<T>::row(lane, v);
+                        *dst -= acc[v];
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    // off-diagonal blocks of the inverse, block row by block row
+    for (int i = 1; i < np; i++) {
+        if (wave < i) {
+            const int j = wave;
+            acc_t tacc = acc_t{0, 0, 0, 0};
+            for (int k = j; k < i; k++) mma_nn16<T>(S, 16 * i, 16 * k, X, 16 * k, 16 * j, tacc, lane);
+            // stage T = sum_k L_ik X_kj in X's (still zero) block (i, j), then X_ij = -X_ii T
+            store16<T>(X, 16 * i, 16 * j, tacc, lane, (T)1, false);
+        }
+        __syncthreads();
+        if (wave < i) {
+            const int j = wave;
+            acc_t acc = acc_t{0, 0, 0, 0};
+            mma_nn16<T>(X, 16 * i, 16 * i, X, 16 * i, 16 * j, acc, lane);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            store16<T>(X, 16 * i, 16 * j, acc, lane, (T)-1, false);
+        }
+        __syncthreads();
+    }
     if (tid == 0 && sfail >= 0) atomicMin(fail, t.col0 + sfail);
-    __syncthreads();
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int i = e & (NB - 1), c = e >> 6;
-        if (i < nb && c < nb && c <= i) W[t.off + i + (long long)c * t.ld] = A[i][c];
-    }
-    // inverse, row by row: X[i][c] = -(sum_{k=c}^{i-1} L[i][k] X[k][c]) / L[i][i]
-    const int c = tid & (NB - 1), part = tid >> 6;
-    for (int i = 0; i < NB; i++) {
-        T s = 0;
-        for (int k = c + part; k < i; k += 4) s += A[i][k] * X[k][c];
-        red[part][c] = s;
-        __syncthreads();
-        if (part == 0) {
-            const T lii = A[i][i];
-            X[i][c] = (c < i) ? -(red[0][c] + red[1][c] + red[2][c] + red[3][c]) / lii
-                              : ((c == i) ? (T)1 / lii : (T)0);
+    {
+        const int i = tid & (NB - 1), c0 = tid >> 6;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int c = c0 + 4 * q;
+            if (i < nb && c < nb) {
+                if (c <= i) W[t.off + i + (long long)c * t.ld] = S[i * DS + c];
+                W[t.winv + i + (long long)c * nb] = (c <= i) ? X[i * DS + c] : (T)0;
+            }
         }
-        __syncthreads();
-    }
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int i = e & (NB - 1), cc = e >> 6;
-        if (i < nb && cc < nb) W[t.winv + i + (long long)cc * nb] = X[i][cc];
     }
 }
 

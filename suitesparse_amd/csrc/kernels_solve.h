@@ -36,22 +36,40 @@ __global__ void k_perm_scatter(const T *__restrict__ Y, const long long *__restr
     for (int r = 0; r < nrhs; r++) X[pk + r * n] = (double)Y[k + r * n];
 }
 
-// forward, diagonal block: x_b = inv(L_bb) x_b   (one wave per task)
+// Stage the nb x nb inverse of block t (ld nb) into LDS Iv[64][65] with every
+// thread's loads in flight together (256 threads, 16 each).
 template <typename T>
-__global__ __launch_bounds__(64) void k_fwd_diag(const SolveTask *__restrict__ tasks,
-                                                 const T *__restrict__ W, T *__restrict__ X,
-                                                 long long n, int nrhs) {
-    __shared__ T xs[NB];
+__device__ __forceinline__ void stage_inverse(const T *__restrict__ W, long long winv, int nb,
+                                              T (*Iv)[NB + 1], int tid) {
+    const int i = tid & (NB - 1), c0 = tid >> 6;
+    T v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int c = c0 + 4 * q;
+        v[q] = (i < nb && c < nb) ? W[winv + i + (long long)c * nb] : (T)0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) Iv[i][c0 + 4 * q] = v[q];
+}
+
+// forward, diagonal block: x_b = inv(L_bb) x_b
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_diag(const SolveTask *__restrict__ tasks,
+                                                  const T *__restrict__ W, T *__restrict__ X,
+                                                  long long n, int nrhs) {
+    __shared__ T Iv[NB][NB + 1];
+    __shared__ T xs[4][NB];
     const SolveTask t = tasks[blockIdx.x];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
     T *xb = X + t.k1 + t.c0;
-    for (int r = 0; r < nrhs; r++) {
-        if (lane < t.nb) xs[lane] = xb[r * n + lane];
+    stage_inverse(W, t.winv, t.nb, Iv, tid);
+    for (int r0 = 0; r0 < nrhs; r0 += 4) {
+        const int r = r0 + q;
+        if (r < nrhs) xs[q][lane] = (lane < t.nb) ? xb[r * n + lane] : (T)0;
         __syncthreads();
-        if (lane < t.nb) {
+        if (r < nrhs && lane < t.nb) {
             T s = 0;
-            const T *iv = W + t.winv + lane;
-            for (int k = 0; k <= lane; k++) s += iv[(long long)k * t.nb] * xs[k];
+            for (int k = 0; k <= lane; k++) s += Iv[lane][k] * xs[q][k];
             xb[r * n + lane] = s;
         }
         __syncthreads();
@@ -61,6 +79,7 @@ __global__ __launch_bounds__(64) void k_fwd_diag(const SolveTask *__restrict__ t
 // forward, rows [r0, r1) below block t:  x[g(r)] -= L[r, c0:c0+nb] x_b
 // g(r) = k1 + r for the supernode's own columns (direct), Ls[psi + r] below
 // them (atomic: sibling supernodes of one level share ancestor rows).
+// One thread per row; the row's 16-column chunks are loaded together.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_off(const SolveTask *__restrict__ tasks,
                                                  const T *__restrict__ W,
@@ -74,109 +93,111 @@ __global__ __launch_bounds__(256) void k_fwd_off(const SolveTask *__restrict__ t
     const bool active = r < t.r1;
     long long g = 0;
     if (active) g = (r < t.nscol) ? (long long)(t.k1 + r) : (long long)Ls[t.psi + r];
-    const T *Lcol = W + t.lx + (long long)t.c0 * t.nsrow + r;
-    for (int r0 = 0; r0 < nrhs; r0 += 8) {
-        const int nr = min(8, nrhs - r0);
-        for (int e = tid; e < nr * NB; e += 256) {
-            int rr = e / NB, k = e % NB;
-            xs[rr][k] = (k < t.nb) ? xb[(r0 + rr) * n + k] : (T)0;
+    const T *Lrow = W + t.lx + (long long)t.c0 * t.nsrow + r;
+    for (int rb = 0; rb < nrhs; rb += 8) {
+        const int nr = min(8, nrhs - rb);
+        for (int e = tid; e < 8 * NB; e += 256) {
+            const int z = e / NB, k = e % NB;
+            xs[z][k] = (z < nr && k < t.nb) ? xb[(rb + z) * n + k] : (T)0;
         }
         __syncthreads();
         if (active) {
             T acc[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) acc[q] = 0;
-            for (int k = 0; k < t.nb; k++) {
-                const T l = Lcol[(long long)k * t.nsrow];
+            for (int z = 0; z < 8; z++) acc[z] = 0;
+            for (int kc = 0; kc < t.nb; kc += 16) {
+                T l[16];
 #pragma unroll
-                for (int q = 0; q < 8; q++) acc[q] += l * xs[q][k];
+                for (int k = 0; k < 16; k++)
+                    l[k] = (kc + k < t.nb) ? Lrow[(long long)(kc + k) * t.nsrow] : (T)0;
+#pragma unroll
+                for (int k = 0; k < 16; k++)
+#pragma unroll
+                    for (int z = 0; z < 8; z++) acc[z] += l[k] * xs[z][kc + k];
             }
-            for (int q = 0; q < nr; q++) {
-                if (r < t.nscol) X[g + (r0 + q) * n] -= acc[q];
-                else atomic_add(X + g + (r0 + q) * n, -acc[q]);
+            for (int z = 0; z < nr; z++) {
+                if (r < t.nscol) X[g + (rb + z) * n] -= acc[z];
+                else atomic_add(X + g + (rb + z) * n, -acc[z]);
             }
         }
         __syncthreads();
     }
 }
 
+// Sum of 16 per-lane values over the 64 lanes of a wave, transposed: on
+// return v[0] in lane l is the total of index (l >> 2) & 15 (every lane of
+// a group of 4 holds the same total).  Halving butterfly: 17 shuffles.
+template <typename T>
+__device__ __forceinline__ T wave_sum16(T (&v)[16], int lane) {
+#pragma unroll
+    for (int lvl = 0; lvl < 4; lvl++) {
+        const int half = 8 >> lvl;            // 8, 4, 2, 1
+        const int mask = 32 >> lvl;           // 32, 16, 8, 4
+        const bool up = (lane & mask) != 0;
+#pragma unroll
+        for (int i = 0; i < half; i++) {
+            const T keep = up ? v[i + half] : v[i];
+            const T send = up ? v[i] : v[i + half];
+            v[i] = keep + __shfl_xor(send, mask);
+        }
+    }
+    T s = v[0];
+    s += __shfl_xor(s, 2);
+    s += __shfl_xor(s, 1);
+    return s;
+}
+
 // backward, rows [r0, r1) below block t:  x_b -= L[r0:r1, c0:c0+nb]' x[g(r0:r1)]
-// 32-row slabs of L are staged through LDS (coalesced loads along rows), then
-// thread (k, q) sums its column over the slab.
+// One thread per row (coalesced reads of L along rows), per-wave transposed
+// butterfly reduction over 16-column chunks, atomic add into x_b.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_off(const SolveTask *__restrict__ tasks,
                                                  const T *__restrict__ W,
                                                  const int *__restrict__ Ls, T *__restrict__ X,
                                                  long long n, int nrhs) {
-    constexpr int SLAB = 32;
-    __shared__ T Lt[NB][SLAB + 1];
-    __shared__ T xr[8][SLAB];
-    __shared__ T part[4][8][NB];
     const SolveTask t = tasks[blockIdx.x];
-    const int tid = threadIdx.x;
-    const int k = tid & (NB - 1), q = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
     T *xb = X + t.k1 + t.c0;
-    for (int rb = 0; rb < nrhs; rb += 8) {
-        const int nr = min(8, nrhs - rb);
-        T acc[8];
+    const int r = t.r0 + tid;
+    const bool active = r < t.r1;
+    long long g = 0;
+    if (active) g = (r < t.nscol) ? (long long)(t.k1 + r) : (long long)Ls[t.psi + r];
+    const T *Lrow = W + t.lx + (long long)t.c0 * t.nsrow + r;
+    for (int kc = 0; kc < t.nb; kc += 16) {
+        T l[16];
 #pragma unroll
-        for (int z = 0; z < 8; z++) acc[z] = 0;
-        for (int s0 = t.r0; s0 < t.r1; s0 += SLAB) {
-            // stage L[s0:s0+SLAB, c0:c0+nb] and x[g(rows)]
-            for (int e = tid; e < SLAB * NB; e += 256) {
-                int rr = e % SLAB, kk = e / SLAB;
-                int r = s0 + rr;
-                T v = 0;
-                if (r < t.r1 && kk < t.nb) v = W[t.lx + (long long)(t.c0 + kk) * t.nsrow + r];
-                Lt[kk][rr] = v;
-            }
-            for (int e = tid; e < nr * SLAB; e += 256) {
-                int z = e / SLAB, rr = e % SLAB;
-                int r = s0 + rr;
-                T v = 0;
-                if (r < t.r1) {
-                    long long g = (r < t.nscol) ? (long long)(t.k1 + r) : (long long)Ls[t.psi + r];
-                    v = X[g + (rb + z) * n];
-                }
-                xr[z][rr] = v;
-            }
-            __syncthreads();
-            for (int rr = q; rr < SLAB; rr += 4) {
-                const T l = Lt[k][rr];
+        for (int k = 0; k < 16; k++)
+            l[k] = (active && kc + k < t.nb) ? Lrow[(long long)(kc + k) * t.nsrow] : (T)0;
+        for (int z = 0; z < nrhs; z++) {
+            const T xr = active ? X[g + z * n] : (T)0;
+            T v[16];
 #pragma unroll
-                for (int z = 0; z < 8; z++) acc[z] += l * xr[z][rr];
-            }
-            __syncthreads();
+            for (int k = 0; k < 16; k++) v[k] = l[k] * xr;
+            const T s = wave_sum16(v, lane);
+            const int k = kc + ((lane >> 2) & 15);
+            if ((lane & 3) == 0 && k < t.nb) atomic_add(xb + z * n + k, -s);
         }
-#pragma unroll
-        for (int z = 0; z < 8; z++) part[q][z][k] = acc[z];
-        __syncthreads();
-        if (q == 0 && k < t.nb) {
-            for (int z = 0; z < nr; z++) {
-                T s = part[0][z][k] + part[1][z][k] + part[2][z][k] + part[3][z][k];
-                atomic_add(xb + (rb + z) * n + k, -s);
-            }
-        }
-        __syncthreads();
     }
 }
 
 // backward, diagonal block: x_b = inv(L_bb)' x_b
 template <typename T>
-__global__ __launch_bounds__(64) void k_bwd_diag(const SolveTask *__restrict__ tasks,
-                                                 const T *__restrict__ W, T *__restrict__ X,
-                                                 long long n, int nrhs) {
-    __shared__ T xs[NB];
+__global__ __launch_bounds__(256) void k_bwd_diag(const SolveTask *__restrict__ tasks,
+                                                  const T *__restrict__ W, T *__restrict__ X,
+                                                  long long n, int nrhs) {
+    __shared__ T Iv[NB][NB + 1];
+    __shared__ T xs[4][NB];
     const SolveTask t = tasks[blockIdx.x];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
     T *xb = X + t.k1 + t.c0;
-    for (int r = 0; r < nrhs; r++) {
-        if (lane < t.nb) xs[lane] = xb[r * n + lane];
+    stage_inverse(W, t.winv, t.nb, Iv, tid);
+    for (int r0 = 0; r0 < nrhs; r0 += 4) {
+        const int r = r0 + q;
+        if (r < nrhs) xs[q][lane] = (lane < t.nb) ? xb[r * n + lane] : (T)0;
         __syncthreads();
-        if (lane < t.nb) {
+        if (r < nrhs && lane < t.nb) {
             T s = 0;
-            const T *iv = W + t.winv + (long long)lane * t.nb;   // column `lane` of inv
-            for (int k = lane; k < t.nb; k++) s += iv[k] * xs[k];
+            for (int k = lane; k < t.nb; k++) s += Iv[k][lane] * xs[q][k];
             xb[r * n + lane] = s;
         }
         __syncthreads();

@@ -187,6 +187,9 @@ static int dupload(mxc_factor *F, X **p, const std::vector<X> &h, hipStream_t s)
 // ---------------------------------------------------------------------------
 static inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Outer panel width of the in-supernode blocked Cholesky (a multiple of NB).
+static constexpr int64_t PANEL = 256;
+
 // local row of global row g inside supernode s (rows Ls[psi..psi+nsrow) sorted,
 // first nscol of them = k1..k2-1)
 static int64_t local_row(const mxc_symbolic *S, int64_t s, int64_t g) {
@@ -419,7 +422,12 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     trsm.push_back(u);
                     F->flops_trsm += (double)u.M * nb * nb;
                 }
-                if (nscol > r0) {
+                // two-level blocking: a 64-column step updates the rest of its
+                // PANEL-column panel only; the last step of a panel then updates
+                // everything right of the panel with K = PANEL
+                const int64_t pstart = (c0 / PANEL) * PANEL;
+                const int64_t pend = std::min<int64_t>(nscol, pstart + PANEL);
+                if (pend > r0) {
                     UpdDesc u;
                     memset(&u, 0, sizeof(u));
                     u.x_off = psx + r0 + c0 * nsrow;
@@ -427,8 +435,23 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.ldx = u.ldy = (int)nsrow;
                     u.K = (int)nb;
                     u.M = (int)(nsrow - r0);
-                    u.N = (int)(nscol - r0);
+                    u.N = (int)(pend - r0);
                     u.c_off = psx + r0 + r0 * nsrow;
+                    u.ldc = (int)nsrow;
+                    trail.push_back(u);
+                    double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
+                    F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
+                }
+                if (r0 == pend && pend < nscol) {
+                    UpdDesc u;
+                    memset(&u, 0, sizeof(u));
+                    u.x_off = psx + pend + pstart * nsrow;
+                    u.y_off = u.x_off;
+                    u.ldx = u.ldy = (int)nsrow;
+                    u.K = (int)(pend - pstart);
+                    u.M = (int)(nsrow - pend);
+                    u.N = (int)(nscol - pend);
+                    u.c_off = psx + pend + pend * nsrow;
                     u.ldc = (int)nsrow;
                     trail.push_back(u);
                     double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
@@ -489,7 +512,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
             if ((long long)st.size() > b) F->fwd_launches.push_back(Launch{L_FOFF, b, (long long)st.size() - b, 0});
         }
     }
-    const int BROWS = 1024;
+    const int BROWS = 256;
     for (int l = nlev - 1; l >= 0; l--) {
         int64_t maxsteps = 0;
         for (int64_t s : by_level[l])
@@ -935,7 +958,7 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
         tick(F, 5, &tt);
         for (const Launch &L : F->fwd_launches) {
             if (L.kind == L_FDIAG)
-                hipLaunchKernelGGL(k_fwd_diag<T>, dim3((unsigned)L.count), dim3(64), 0, sm,
+                hipLaunchKernelGGL(k_fwd_diag<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
                                    F->d_stask + L.begin, W, Xw, (long long)n, nr);
             else
                 hipLaunchKernelGGL(k_fwd_off<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
@@ -948,7 +971,7 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
         tick(F, 6, &tt);
         for (const Launch &L : F->bwd_launches) {
             if (L.kind == L_BDIAG)
-                hipLaunchKernelGGL(k_bwd_diag<T>, dim3((unsigned)L.count), dim3(64), 0, sm,
+                hipLaunchKernelGGL(k_bwd_diag<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
                                    F->d_stask + L.begin, W, Xw, (long long)n, nr);
             else
                 hipLaunchKernelGGL(k_bwd_off<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
