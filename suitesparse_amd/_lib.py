@@ -47,6 +47,9 @@ _PROTOS = {
     "mxc_residual": (i32, [vp, vp, vp, i64, vp, i32]),
     "mxc_solve_refine": (i32, [vp, vp, i64, vp, i32, f64, i32, vp]),
     "mxc_solve_refine_segments": (i32, [vp, vp, i64, vp, i32, f64, i32, i64, vp, vp]),
+    "mxc_autotune": (i32, [vp, i32, vp]),
+    "mxc_set_tiles": (i32, [vp, vp]),
+    "mxc_get_tiles": (i32, [vp, vp]),
     "mxc_synchronize": (i32, [vp]),
     "mxc_set_option": (i32, [vp, i32, i64]),
     "mxc_set_timing": (i32, [vp, i32]),

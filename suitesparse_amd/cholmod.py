@@ -165,6 +165,23 @@ class Factor:
         """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 1, int(bool(on))))
 
+    def autotune(self, reps: int = 3) -> list:
+        """Pick the tile edge per (rows, cols) bucket by timing; returns the table.
+        Invalidates the numeric factor (factorize again afterwards)."""
+        t = np.zeros(25, dtype=np.int32)
+        check("mxc_autotune", lib().mxc_autotune(self._num, int(reps), _ptr(t)))
+        self.is_numeric = False
+        return t.tolist()
+
+    def set_tiles(self, table):
+        t = np.ascontiguousarray(table, dtype=np.int32)
+        check("mxc_set_tiles", lib().mxc_set_tiles(self._num, _ptr(t)))
+
+    def tiles(self) -> list:
+        t = np.zeros(25, dtype=np.int32)
+        check("mxc_get_tiles", lib().mxc_get_tiles(self._num, _ptr(t)))
+        return t.tolist()
+
     def kernel_flops(self) -> dict:
         """Algorithmic flops of one factorization by kernel class."""
         info = np.zeros(8, dtype=np.int64)

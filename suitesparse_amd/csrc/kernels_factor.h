@@ -73,23 +73,26 @@ __global__ void k_add_diag(const long long *__restrict__ dpos, long long n,
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
 // ---------------------------------------------------------------------------
-// Dense TS x TS update tile on MFMA (see UpdDesc), TS = 64 (4 waves, 256
-// threads) or 32 (1 wave, 64 threads, for the many small (d,s) pairs).  Each
-// wave owns a 32x32 block of C as 2x2 16x16 MFMA blocks.  The MFMA is fed X
-// (C-column side) as its A operand and Y (C-row side) as its B operand, so
-// each lane's 4 accumulators hold 4 C columns of one C row and 16
-// consecutive lanes hold 16 consecutive C rows: stores into the
-// column-major target are 16-element contiguous runs.
+// Dense TS x TS update tile on MFMA (see UpdDesc).  TS = 32: one wave owns the
+// tile; TS = 64 / 128: four waves, each owning a (TS/2) x (TS/2) quarter, as
+// (TS/32)^2 ... i.e. 2x2 or 4x4 blocks of 16x16 MFMA.  The larger tiles do more
+// MFMA work per staged byte, which hides the global-load latency of the next
+// K chunk (prefetched into registers while the current chunk computes from
+// the other LDS buffer).  The MFMA is fed X (C-column side) as its A operand
+// and Y (C-row side) as its B operand, so 16 consecutive lanes hold 16
+// consecutive C rows: stores into the column-major target are contiguous runs.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, int TS>
-__global__ __launch_bounds__(TS * TS / 16) void k_tile(const UpdDesc *__restrict__ descs,
-                                                      const Tile *__restrict__ tiles, T *__restrict__ W,
-                                                      const int *__restrict__ Ls) {
+__global__ __launch_bounds__(TS == 32 ? 64 : 256) void k_tile(const UpdDesc *__restrict__ descs,
+                                                              const Tile *__restrict__ tiles,
+                                                              T *__restrict__ W,
+                                                              const int *__restrict__ Ls) {
     typedef typename MfmaOp<T>::acc_t acc_t;
-    constexpr int NT = TS * TS / 16;            // threads: 64 per 32x32 wave block
-    constexpr int KG = NT / TS;                 // k groups staged per pass
+    constexpr int NT = TS == 32 ? 64 : 256;     // threads
+    constexpr int WB = TS == 32 ? 32 : TS / 2;  // edge of a wave's block of C
+    constexpr int MB = WB / 16;                 // 16x16 MFMA blocks per wave edge
+    constexpr int KG = NT / TS;                 // k rows staged per pass
     constexpr int PER = (BK * TS) / NT;         // elements of X (and of Y) each thread stages
-    constexpr int WPR = TS / 32;                // waves per tile row
     __shared__ T Xs[2][BK][TS + LDS_PAD];
     __shared__ T Ys[2][BK][TS + LDS_PAD];
     __shared__ int relr[TS];
@@ -99,41 +102,43 @@ __global__ __launch_bounds__(TS * TS / 16) void k_tile(const UpdDesc *__restrict
     const UpdDesc d = descs[tl.desc];
     const int i0 = tl.ti * TS, j0 = tl.tj * TS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave % WPR, wc = wave / WPR;
+    const int wr = TS == 32 ? 0 : (wave & 1), wc = TS == 32 ? 0 : (wave >> 1);
 
     if (MODE == MODE_SCATTER) {
-        if (tid < TS) {
-            int i = i0 + tid;
-            int r = -1;
-            if (i < d.M) {
-                int g = Ls[d.ls_off + i];
-                if (g < d.base + d.t_nscol) {
-                    r = g - d.base;
-                } else {
-                    int lo = d.t_psi + d.t_nscol, hi = d.t_psi + d.t_nsrow - 1;
-                    while (lo < hi) {
-                        int mid = (lo + hi) >> 1;
-                        if (Ls[mid] < g) lo = mid + 1; else hi = mid;
+        for (int e = tid; e < 2 * TS; e += NT) {
+            if (e < TS) {
+                int i = i0 + e;
+                int r = -1;
+                if (i < d.M) {
+                    int g = Ls[d.ls_off + i];
+                    if (g < d.base + d.t_nscol) {
+                        r = g - d.base;
+                    } else {
+                        int lo = d.t_psi + d.t_nscol, hi = d.t_psi + d.t_nsrow - 1;
+                        while (lo < hi) {
+                            int mid = (lo + hi) >> 1;
+                            if (Ls[mid] < g) lo = mid + 1; else hi = mid;
+                        }
+                        r = lo - d.t_psi;
                     }
-                    r = lo - d.t_psi;
                 }
+                relr[e] = r;
+            } else {
+                int j = j0 + e - TS;
+                relc[e - TS] = (j < d.N) ? (Ls[d.ls_off + j] - d.base) : -1;
             }
-            relr[tid] = r;
-        } else if (tid < 2 * TS) {
-            int j = j0 + tid - TS;
-            relc[tid - TS] = (j < d.N) ? (Ls[d.ls_off + j] - d.base) : -1;
         }
     }
 
-    acc_t acc[2][2];
+    acc_t acc[MB][MB];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < MB; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+        for (int b = 0; b < MB; b++) acc[a][b] = acc_t{0, 0, 0, 0};
 
     const T *__restrict__ Xp = W + d.x_off;
     const T *__restrict__ Yp = W + d.y_off;
-    // thread t stages row r = t % TS of chunk columns k = t/TS + KG*q, q < PER
+    // thread t stages row r = t % TS of chunk rows k = t/TS + KG*q, q < PER
     const int r = tid % TS, kq = tid / TS;
     const bool rowy = (i0 + r) < d.M, rowx = (j0 + r) < d.N;
     const T *yrow = Yp + (i0 + r);
@@ -160,31 +165,31 @@ __global__ __launch_bounds__(TS * TS / 16) void k_tile(const UpdDesc *__restrict
         if (k0 + BK < d.K) fetch(k0 + BK);      // next chunk in flight during the MFMAs
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
-            T xa[2], yb[2];
+            T xa[MB], yb[MB];
 #pragma unroll
-            for (int m = 0; m < 2; m++) xa[m] = Xs[buf][kk + (lane >> 4)][wc * 32 + m * 16 + (lane & 15)];
+            for (int m = 0; m < MB; m++) xa[m] = Xs[buf][kk + (lane >> 4)][wc * WB + m * 16 + (lane & 15)];
 #pragma unroll
-            for (int n = 0; n < 2; n++) yb[n] = Ys[buf][kk + (lane >> 4)][wr * 32 + n * 16 + (lane & 15)];
+            for (int n = 0; n < MB; n++) yb[n] = Ys[buf][kk + (lane >> 4)][wr * WB + n * 16 + (lane & 15)];
 #pragma unroll
-            for (int m = 0; m < 2; m++)
+            for (int m = 0; m < MB; m++)
 #pragma unroll
-                for (int n = 0; n < 2; n++) acc[m][n] = MfmaOp<T>::mma(xa[m], yb[n], acc[m][n]);
+                for (int n = 0; n < MB; n++) acc[m][n] = MfmaOp<T>::mma(xa[m], yb[n], acc[m][n]);
         }
         buf ^= 1;
     }
     if (MODE == MODE_STORE) __syncthreads();   // in-place TRSM: every read of the panel is done
 
-    // epilogue: acc[m][n][v] = C(i, j), i = i0 + 32wr + 16n + lane%16,
-    //                                   j = j0 + 32wc + 16m + row(lane, v)
+    // epilogue: acc[m][n][v] = C(i, j), i = i0 + WB*wr + 16n + lane%16,
+    //                                   j = j0 + WB*wc + 16m + row(lane, v)
 #pragma unroll
-    for (int m = 0; m < 2; m++) {
+    for (int m = 0; m < MB; m++) {
 #pragma unroll
-        for (int n = 0; n < 2; n++) {
-            const int il = wr * 32 + n * 16 + (lane & 15);
+        for (int n = 0; n < MB; n++) {
+            const int il = wr * WB + n * 16 + (lane & 15);
             const int i = i0 + il;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int jl = wc * 32 + m * 16 + MfmaOp<T>::row(lane, v);
+                const int jl = wc * WB + m * 16 + MfmaOp<T>::row(lane, v);
                 const int j = j0 + jl;
                 if (i >= d.M || j >= d.N) continue;
                 const T c = acc[m][n][v];

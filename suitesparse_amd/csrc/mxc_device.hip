@@ -104,6 +104,11 @@ static inline int tile_bucket(int M, int N) {
 static constexpr int NBUCKET = 25;
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FDIAG = 4, L_FOFF = 5, L_BOFF = 6, L_BDIAG = 7 };
 
+struct StepBlocks {
+    std::vector<DiagTask> diags;
+    std::vector<UpdDesc> trsm, trail;
+};
+
 struct mxc_factor {
     int precision = MXC_FP64;
     int64_t n = 0, nsuper = 0, xsize = 0, nnz_a = 0, nnz_full = 0;
@@ -140,7 +145,13 @@ struct mxc_factor {
     std::vector<Launch> fwd_launches, bwd_launches;
     bool values_loaded = false;
 
-    // tile edge per (M, N) bucket; default 32 for blocks up to 64 x 32
+    // host copy of the factorization's blocks (re-flattened after autotuning)
+    std::vector<std::vector<UpdDesc>> blk_upd;        // per level
+    std::vector<std::vector<StepBlocks>> blk_steps;   // per level, per 64-column step
+    size_t plan_bytes = 0;                            // bytes of d_desc + d_tiles + d_diag
+
+    // tile edge per (M, N) bucket: 32 up to 64 x 32, 128 for M > 128 and N > 32,
+    // else 64 (mxc_set_tile overrides an entry before the plan is built)
     int tune[NBUCKET];
 
     // hipGraph replay: the launch sequence depends only on the pattern, so it is
@@ -199,6 +210,73 @@ static int64_t local_row(const mxc_symbolic *S, int64_t s, int64_t g) {
     const int64_t *it = std::lower_bound(S->Ls + lo, S->Ls + hi, g);
     if (it == S->Ls + hi || *it != g) return -1;
     return (int64_t)(it - S->Ls) - S->Lpi[s];
+}
+
+static void drop_factor_graph(mxc_factor *F) {
+    if (F->x_factor) (void)hipGraphExecDestroy(F->x_factor);
+    if (F->g_factor) (void)hipGraphDestroy(F->g_factor);
+    F->x_factor = nullptr;
+    F->g_factor = nullptr;
+}
+
+// Turn the host block lists into device descriptors, tiles and the launch
+// list, one launch per tile class (from F->tune) per phase.
+static int flatten_upload(mxc_factor *F) {
+    std::vector<UpdDesc> descs;
+    std::vector<Tile> tiles;
+    std::vector<DiagTask> diags;
+    F->factor_launches.clear();
+    int64_t nupd = 0;
+    auto emit = [&](int kind, const std::vector<UpdDesc> &blocks, bool lower_only) {
+        long long total = 0;
+        for (int ts : {32, 64, 128}) {
+            long long b = (long long)tiles.size();
+            for (const UpdDesc &u : blocks) {
+                if (F->tune[tile_bucket(u.M, u.N)] != ts) continue;
+                int di = (int)descs.size();
+                descs.push_back(u);
+                int64_t mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
+                for (int64_t ti = 0; ti < mt; ti++)
+                    for (int64_t tj = 0; tj < nt; tj++) {
+                        if (lower_only && ti < tj) continue;   // tile wholly above the diagonal
+                        tiles.push_back(Tile{di, (int)ti, (int)tj});
+                    }
+            }
+            long long c = (long long)tiles.size() - b;
+            if (c > 0) F->factor_launches.push_back(Launch{kind, b, c, ts});
+            total += c;
+        }
+        return total;
+    };
+    for (size_t l = 0; l < F->blk_upd.size(); l++) {
+        nupd += emit(L_UPD, F->blk_upd[l], true);
+        for (const StepBlocks &sb : F->blk_steps[l]) {
+            long long db = (long long)diags.size();
+            diags.insert(diags.end(), sb.diags.begin(), sb.diags.end());
+            long long dc = (long long)diags.size() - db;
+            if (dc > 0) F->factor_launches.push_back(Launch{L_DIAG, db, dc, 0});
+            emit(L_TRSM, sb.trsm, false);
+            emit(L_TRAIL, sb.trail, true);
+        }
+    }
+    F->nupd_tiles = nupd;
+    if (descs.size() >= (size_t)INT32_MAX || tiles.size() >= (size_t)UINT32_MAX) return MXC_TOO_LARGE;
+    drop_factor_graph(F);
+    if (F->d_desc) (void)hipFree(F->d_desc);
+    if (F->d_tiles) (void)hipFree(F->d_tiles);
+    if (F->d_diag) (void)hipFree(F->d_diag);
+    F->d_desc = nullptr;
+    F->d_tiles = nullptr;
+    F->d_diag = nullptr;
+    F->bytes -= F->plan_bytes;
+    size_t before = F->bytes;
+    int rc;
+    if ((rc = dupload(F, &F->d_desc, descs, F->stream))) return rc;
+    if ((rc = dupload(F, &F->d_tiles, tiles, F->stream))) return rc;
+    if ((rc = dupload(F, &F->d_diag, diags, F->stream))) return rc;
+    F->plan_bytes = F->bytes - before;
+    HIPTRY(hipStreamSynchronize(F->stream));
+    return MXC_OK;
 }
 
 template <typename T>
@@ -348,43 +426,18 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     }
     F->npairs = npairs;
 
-    // ---- flatten: descriptors, tiles, diag tasks, launches.  Each phase's
-    // blocks are split by tile class (32 or 64, from the tuning table) into
-    // one launch per class.
-    std::vector<UpdDesc> descs;
-    std::vector<Tile> tiles;
-    std::vector<DiagTask> diags;
-    F->factor_launches.clear();
-    int64_t nupd = 0;
-    auto emit = [&](int kind, const std::vector<UpdDesc> &blocks, bool lower_only) {
-        long long total = 0;
-        for (int ts : {32, 64}) {
-            long long b = (long long)tiles.size();
-            for (const UpdDesc &u : blocks) {
-                if (F->tune[tile_bucket(u.M, u.N)] != ts) continue;
-                int di = (int)descs.size();
-                descs.push_back(u);
-                int64_t mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
-                for (int64_t ti = 0; ti < mt; ti++)
-                    for (int64_t tj = 0; tj < nt; tj++) {
-                        if (lower_only && ti < tj) continue;   // tile wholly above the diagonal
-                        tiles.push_back(Tile{di, (int)ti, (int)tj});
-                    }
-            }
-            long long c = (long long)tiles.size() - b;
-            if (c > 0) F->factor_launches.push_back(Launch{kind, b, c, ts});
-            total += c;
-        }
-        return total;
-    };
+    // ---- per level: the update blocks, then per 64-column step the diagonal
+    // blocks, TRSM panels and trailing blocks (kept on the host so that the
+    // tile assignment can be redone by the autotuner)
+    F->blk_upd = std::move(pairs_by_level);
+    F->blk_steps.assign(nlev, {});
     for (int l = 0; l < nlev; l++) {
-        nupd += emit(L_UPD, pairs_by_level[l], true);
         int64_t maxsteps = 0;
         for (int64_t s : by_level[l])
             maxsteps = std::max(maxsteps, ceil_div(S->Super[s + 1] - S->Super[s], NB));
+        F->blk_steps[l].resize(maxsteps);
         for (int64_t t = 0; t < maxsteps; t++) {
-            long long db = (long long)diags.size();
-            std::vector<UpdDesc> trsm, trail;
+            StepBlocks &sb = F->blk_steps[l][t];
             for (int64_t s : by_level[l]) {
                 int64_t nscol = S->Super[s + 1] - S->Super[s];
                 int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
@@ -404,7 +457,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                 dt.nb = (int)nb;
                 dt.col0 = (int)(S->Super[s] + c0);
                 dt.pad = 0;
-                diags.push_back(dt);
+                sb.diags.push_back(dt);
                 F->flops_potrf += (double)nb * nb * nb / 3.0;
                 int64_t r0 = c0 + nb;
                 if (nsrow > r0) {
@@ -419,7 +472,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.N = (int)nb;
                     u.c_off = u.y_off;
                     u.ldc = (int)nsrow;
-                    trsm.push_back(u);
+                    sb.trsm.push_back(u);
                     F->flops_trsm += (double)u.M * nb * nb;
                 }
                 // two-level blocking: a 64-column step updates the rest of its
@@ -438,7 +491,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.N = (int)(pend - r0);
                     u.c_off = psx + r0 + r0 * nsrow;
                     u.ldc = (int)nsrow;
-                    trail.push_back(u);
+                    sb.trail.push_back(u);
                     double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
                     F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
                 }
@@ -453,18 +506,13 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.N = (int)(nscol - pend);
                     u.c_off = psx + pend + pend * nsrow;
                     u.ldc = (int)nsrow;
-                    trail.push_back(u);
+                    sb.trail.push_back(u);
                     double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
                     F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
                 }
             }
-            long long dc = (long long)diags.size() - db;
-            if (dc > 0) F->factor_launches.push_back(Launch{L_DIAG, db, dc, 0});
-            emit(L_TRSM, trsm, false);
-            emit(L_TRAIL, trail, true);
         }
     }
-    F->nupd_tiles = nupd;
 
     // ---- solve tasks: forward (levels up, steps up), backward (levels down, steps down)
     std::vector<SolveTask> st;
@@ -546,9 +594,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     if ((rc = dupload(F, &F->d_pos, pos, sm))) return rc;
     if ((rc = dupload(F, &F->d_dpos, dpos, sm))) return rc;
     if ((rc = dupload(F, &F->d_perm, perm64, sm))) return rc;
-    if ((rc = dupload(F, &F->d_desc, descs, sm))) return rc;
-    if ((rc = dupload(F, &F->d_tiles, tiles, sm))) return rc;
-    if ((rc = dupload(F, &F->d_diag, diags, sm))) return rc;
+    if ((rc = flatten_upload(F))) return rc;
     if ((rc = dupload(F, &F->d_stask, st, sm))) return rc;
     if ((rc = dupload(F, &F->d_rowptr, rowptr, sm))) return rc;
     if ((rc = dupload(F, &F->d_col, colv, sm))) return rc;
@@ -590,8 +636,7 @@ static int ensure_work(mxc_factor *F, int64_t nrhs) {
 extern "C" void mxc_factor_free(mxc_factor *F) {
     if (!F) return;
     drop_sweep_graphs(F);
-    if (F->x_factor) (void)hipGraphExecDestroy(F->x_factor);
-    if (F->g_factor) (void)hipGraphDestroy(F->g_factor);
+    drop_factor_graph(F);
     void *ptrs[] = {F->W, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
                     F->d_diag, F->d_stask, F->d_fail, F->d_rowptr, F->d_src, F->d_col,
                     F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_b, F->d_x, F->d_r};
@@ -615,7 +660,7 @@ extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const
     F->precision = precision;
     for (int b = 0; b < NBUCKET; b++) {
         int mb = b / 5, nbk = b % 5;
-        F->tune[b] = (mb <= 2 && nbk <= 2) ? 32 : 64;
+        F->tune[b] = (mb <= 2 && nbk <= 2) ? 32 : (mb == 4 && nbk >= 3) ? 128 : 64;
     }
     hipError_t e = hipSetDevice(g_device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking);
@@ -727,8 +772,11 @@ static void launch_tile(mxc_factor *F, const Launch &L, T *W) {
     if (L.tile == 32)
         hipLaunchKernelGGL((k_tile<T, MODE, 32>), dim3((unsigned)L.count), dim3(64), 0, F->stream,
                            F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
-    else
+    else if (L.tile == 64)
         hipLaunchKernelGGL((k_tile<T, MODE, 64>), dim3((unsigned)L.count), dim3(256), 0, F->stream,
+                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+    else
+        hipLaunchKernelGGL((k_tile<T, MODE, 128>), dim3((unsigned)L.count), dim3(256), 0, F->stream,
                            F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
 }
 
@@ -1243,6 +1291,116 @@ static int solve_refine_impl(mxc_factor *F, const double *B, int64_t nrhs, doubl
     if (location != MXC_DEVICE)
         HIPTRY(hipMemcpyAsync(X, dX, F->n * nrhs * sizeof(double), hipMemcpyDeviceToHost, F->stream));
     HIPTRY(hipStreamSynchronize(F->stream));
+    return MXC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: per (M, N) bucket of update blocks, time each tile class on (a
+// sample of) that bucket's blocks and keep the fastest
+// ---------------------------------------------------------------------------
+template <typename T>
+static int time_tiles(mxc_factor *F, const std::vector<UpdDesc> &blocks, int ts, int reps, float *ms) {
+    std::vector<UpdDesc> descs(blocks);
+    std::vector<Tile> tiles;
+    for (size_t di = 0; di < descs.size(); di++) {
+        const UpdDesc &u = descs[di];
+        int64_t mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
+        for (int64_t ti = 0; ti < mt; ti++)
+            for (int64_t tj = 0; tj < nt; tj++)
+                if (ti >= tj) tiles.push_back(Tile{(int)di, (int)ti, (int)tj});
+    }
+    UpdDesc *d_desc = nullptr;
+    Tile *d_tiles = nullptr;
+    HIPTRY(hipMalloc((void **)&d_desc, descs.size() * sizeof(UpdDesc)));
+    hipError_t e = hipMalloc((void **)&d_tiles, tiles.size() * sizeof(Tile));
+    if (e != hipSuccess) {
+        (void)hipFree(d_desc);
+        return set_hip_err(e, "hipMalloc", __LINE__);
+    }
+    hipStream_t sm = F->stream;
+    (void)hipMemcpyAsync(d_desc, descs.data(), descs.size() * sizeof(UpdDesc), hipMemcpyHostToDevice, sm);
+    (void)hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, sm);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    T *W = (T *)F->W;
+    auto launch = [&]() {
+        if (ts == 32)
+            hipLaunchKernelGGL((k_tile<T, MODE_SCATTER, 32>), dim3((unsigned)tiles.size()), dim3(64), 0, sm,
+                               d_desc, d_tiles, W, F->d_Ls);
+        else if (ts == 64)
+            hipLaunchKernelGGL((k_tile<T, MODE_SCATTER, 64>), dim3((unsigned)tiles.size()), dim3(256), 0, sm,
+                               d_desc, d_tiles, W, F->d_Ls);
+        else
+            hipLaunchKernelGGL((k_tile<T, MODE_SCATTER, 128>), dim3((unsigned)tiles.size()), dim3(256), 0, sm,
+                               d_desc, d_tiles, W, F->d_Ls);
+    };
+    launch();                        // warm-up
+    (void)hipEventRecord(e0, sm);
+    for (int r = 0; r < reps; r++) launch();
+    (void)hipEventRecord(e1, sm);
+    e = hipEventSynchronize(e1);
+    float t = 0;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
+    *ms = t / (float)std::max(reps, 1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(d_desc);
+    (void)hipFree(d_tiles);
+    if (e != hipSuccess) return set_hip_err(e, "autotune launch", __LINE__);
+    HIPTRY(hipGetLastError());
+    return MXC_OK;
+}
+
+template <typename T>
+static int autotune_impl(mxc_factor *F, int reps, int *table) {
+    std::vector<std::vector<UpdDesc>> byb(NBUCKET);
+    for (const auto &lv : F->blk_upd)
+        for (const UpdDesc &u : lv) byb[tile_bucket(u.M, u.N)].push_back(u);
+    const size_t cap = 4096;         // blocks timed per bucket (evenly strided sample)
+    for (int b = 0; b < NBUCKET; b++) {
+        if (byb[b].empty()) continue;
+        std::vector<UpdDesc> sample;
+        size_t stride = std::max<size_t>(1, byb[b].size() / cap);
+        for (size_t q = 0; q < byb[b].size() && sample.size() < cap; q += stride) sample.push_back(byb[b][q]);
+        float best = -1;
+        int best_ts = F->tune[b];
+        for (int ts : {32, 64, 128}) {
+            float ms = 0;
+            int rc = time_tiles<T>(F, sample, ts, reps, &ms);
+            if (rc) return rc;
+            if (best < 0 || ms < best) {
+                best = ms;
+                best_ts = ts;
+            }
+        }
+        F->tune[b] = best_ts;
+    }
+    F->values_loaded = false;        // the timing launches scribbled on L
+    int rc = flatten_upload(F);
+    if (rc) return rc;
+    if (table) for (int b = 0; b < NBUCKET; b++) table[b] = F->tune[b];
+    return MXC_OK;
+}
+
+extern "C" int mxc_autotune(mxc_factor *F, int reps, int *table) {
+    if (!F || reps < 1) return MXC_INVALID;
+    HIPTRY(hipSetDevice(g_device));
+    return F->precision == MXC_FP64 ? autotune_impl<double>(F, reps, table)
+                                    : autotune_impl<float>(F, reps, table);
+}
+
+extern "C" int mxc_set_tiles(mxc_factor *F, const int *table) {
+    if (!F || !table) return MXC_INVALID;
+    for (int b = 0; b < NBUCKET; b++)
+        if (table[b] != 32 && table[b] != 64 && table[b] != 128) return MXC_INVALID;
+    for (int b = 0; b < NBUCKET; b++) F->tune[b] = table[b];
+    return flatten_upload(F);
+}
+
+extern "C" int mxc_get_tiles(const mxc_factor *F, int *table) {
+    if (!F || !table) return MXC_INVALID;
+    for (int b = 0; b < NBUCKET; b++) table[b] = F->tune[b];
     return MXC_OK;
 }
 
