@@ -46,8 +46,26 @@ def relres(A, B, X) -> float:
     return float((np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)).max())
 
 
+TUNING = ROOT / "suitesparse_amd" / "tuning"
+
+
+def tile_table(name: str, precision: str, mode: str, L):
+    """Install the tile table for this workload: 'auto' runs the autotuner and
+    saves its table; otherwise a saved table is loaded when one exists."""
+    path = TUNING / f"{name}_{precision}.json"
+    if mode == "auto":
+        t0 = time.perf_counter()
+        table = L.autotune(reps=3)
+        path.write_text(json.dumps({"table": table, "tune_s": time.perf_counter() - t0}))
+        return "autotuned"
+    if mode == "saved" and path.exists():
+        L.set_tiles(json.loads(path.read_text())["table"])
+        return "saved"
+    return "default"
+
+
 def gpu_factor_solve(A, perm, B, precision, steps, warmup, refine=False, tol=1e-10,
-                     time_updates=False):
+                     time_updates=False, tune=("", "none")):
     """Run `steps` timed factor+solve steps; returns timings and the last X."""
     import torch
     from suitesparse_amd import cholmod
@@ -64,6 +82,7 @@ def gpu_factor_solve(A, perm, B, precision, steps, warmup, refine=False, tol=1e-
     t0 = time.perf_counter()
     L._numeric(cm)
     t_plan = time.perf_counter() - t0
+    tiles = tile_table(tune[0], precision, tune[1], L) if tune[0] else "default"
     info = None
 
     def step():
@@ -114,7 +133,7 @@ def gpu_factor_solve(A, perm, B, precision, steps, warmup, refine=False, tol=1e-
     res = {
         "L": L, "lat": lat, "t_total": ttot, "t_factor": tf, "t_solve": ts,
         "t_analyze": t_an, "t_plan": t_plan, "X": np.asarray(Xh), "timing": timing,
-        "refine": info, "info": L.info(), "breakdown": breakdown,
+        "refine": info, "info": L.info(), "breakdown": breakdown, "tiles": tiles,
     }
     return res
 
@@ -159,6 +178,8 @@ def main():
     ap.add_argument("--mixed-side", type=int, default=96)
     ap.add_argument("--only-mixed", action="store_true",
                     help="skip the 64^3 headline (profiling of the 96^3 runs)")
+    ap.add_argument("--tiles", choices=["saved", "auto", "default"], default="saved",
+                    help="tile table: the saved autotuned one (default), retune, or built-in")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -179,7 +200,8 @@ def main():
     A, B, xt = problems.workload_diffusion3d_64()
     perm = load_perm("diffusion3d_64_metis")
     log("[bench] FP64 factor+solve on GPU")
-    g = gpu_factor_solve(A, perm, B, "fp64", args.steps, args.warmup, time_updates=True)
+    g = gpu_factor_solve(A, perm, B, "fp64", args.steps, args.warmup, time_updates=True,
+                         tune=("diffusion3d_64", args.tiles))
     L = g["L"]
     fl, lnz, nrhs = L.fl, L.lnz, B.shape[1]
     ms = g["t_total"] / args.steps * 1e3
@@ -215,7 +237,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3)},
         "t_factor_ms": round(g["t_factor"] * 1e3, 3), "t_solve_ms": round(g["t_solve"] * 1e3, 3),
-        "factor_gflops": round(fl / g["t_factor"] / 1e9, 2),
+        "factor_gflops": round(fl / g["t_factor"] / 1e9, 2), "tiles": g["tiles"],
         "relres": rr, "fwd_err": fe, "roofline": roof,
         "breakdown_ms": {k: (round(v, 3) if k != "launches" else v)
                          for k, v in (g["breakdown"] or {}).items()},
@@ -254,7 +276,7 @@ def mixed_section(args, out):
         for prec in ("fp64", "fp32"):
             log(f"[bench] {side}^3 {prec}")
             r = gpu_factor_solve(A3, perm3, B3, prec, max(1, args.steps // 2), 1,
-                                 refine=(prec == "fp32"))
+                                 refine=(prec == "fp32"), tune=(f"stencil27_{side}", args.tiles))
             tts = r["t_total"] / max(1, args.steps // 2)
             rr3 = relres(A3, B3, r["X"])
             mix[prec] = {"time_to_solution_ms": round(tts * 1e3, 2),
@@ -262,7 +284,7 @@ def mixed_section(args, out):
                          "t_solve_ms": round(r["t_solve"] * 1e3, 2),
                          "factor_gflops": round(r["L"].fl / r["t_factor"] / 1e9, 1),
                          "factor_solve_gflops": round((r["L"].fl + 4 * r["L"].lnz * B3.shape[1]) / tts / 1e9, 1),
-                         "relres": rr3, "refine": r["refine"]}
+                         "relres": rr3, "refine": r["refine"], "tiles": r["tiles"]}
             log(f"[bench]   {mix[prec]}")
             del r
             import gc

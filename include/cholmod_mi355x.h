@@ -164,14 +164,15 @@ int mxc_solve_refine_segments(mxc_factor *F, const double *B, int64_t nrhs, doub
                               double tol, int maxiter, int64_t nseg, const int64_t *segoff,
                               double *out);
 
-/* Tile autotuner.  Update blocks are bucketed by rows M (<=16, <=32, <=64,
- * <=128, >128) and columns N (<=8, <=16, <=32, <=64, >64): bucket 5*mb + nb.
+/* Tile autotuner.  Blocks are bucketed by rows M (<=16, <=32, <=64, <=128,
+ * >128), columns N (<=8, <=16, <=32, <=64, >64) and inner dimension K (<=32,
+ * <=128, >128): bucket (5*mb + nb)*3 + kb, one table of 75 buckets for each
+ * kind of block (0 descendant update, 1 in-supernode trailing update, 2 TRSM).
  * mxc_autotune times every tile edge (32, 64, 128) on a sample of each
- * bucket's descendant-update blocks (reps launches each, HIP events), keeps
- * the fastest per bucket, rebuilds the plan, and returns the table (25
- * entries; may be NULL).  It scribbles on the numeric factor: call it before
- * mxc_factorize.  mxc_set_tiles installs a saved table; mxc_get_tiles reads
- * the current one. */
+ * bucket's blocks (reps launches each, HIP events), keeps the fastest, rebuilds
+ * the plan and returns the table (3*75 entries, kind-major; may be NULL).  It
+ * scribbles on the numeric factor: call it before mxc_factorize.
+ * mxc_set_tiles installs a saved table; mxc_get_tiles reads the current one. */
 int mxc_autotune(mxc_factor *F, int reps, int *table);
 int mxc_set_tiles(mxc_factor *F, const int *table);
 int mxc_get_tiles(const mxc_factor *F, int *table);

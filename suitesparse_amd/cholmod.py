@@ -168,7 +168,7 @@ class Factor:
     def autotune(self, reps: int = 3) -> list:
         """Pick the tile edge per (rows, cols) bucket by timing; returns the table.
         Invalidates the numeric factor (factorize again afterwards)."""
-        t = np.zeros(25, dtype=np.int32)
+        t = np.zeros(225, dtype=np.int32)
         check("mxc_autotune", lib().mxc_autotune(self._num, int(reps), _ptr(t)))
         self.is_numeric = False
         return t.tolist()
@@ -178,7 +178,7 @@ class Factor:
         check("mxc_set_tiles", lib().mxc_set_tiles(self._num, _ptr(t)))
 
     def tiles(self) -> list:
-        t = np.zeros(25, dtype=np.int32)
+        t = np.zeros(225, dtype=np.int32)
         check("mxc_get_tiles", lib().mxc_get_tiles(self._num, _ptr(t)))
         return t.tolist()
 

@@ -95,13 +95,18 @@ struct Launch {
     int tile;          // tile edge of k_tile launches (32 or 64)
 };
 
-// Tile-size buckets over (rows M, columns N) of an update block: 5 x 5 classes.
-static inline int tile_bucket(int M, int N) {
+// Tile-size buckets over (rows M, columns N, inner dimension K) of a block:
+// 5 x 5 x 3 classes, one table per kind of block (descendant update, trailing
+// update, TRSM).
+static inline int tile_bucket(int M, int N, int K) {
     int mb = M <= 16 ? 0 : M <= 32 ? 1 : M <= 64 ? 2 : M <= 128 ? 3 : 4;
     int nb = N <= 8 ? 0 : N <= 16 ? 1 : N <= 32 ? 2 : N <= 64 ? 3 : 4;
-    return mb * 5 + nb;
+    int kb = K <= 32 ? 0 : K <= 128 ? 1 : 2;
+    return (mb * 5 + nb) * 3 + kb;
 }
-static constexpr int NBUCKET = 25;
+static constexpr int NBUCKET = 75;
+static constexpr int NKIND = 3;     // 0 descendant update, 1 trailing, 2 TRSM
+static inline int kind_of(int launch_kind) { return launch_kind == 0 ? 0 : launch_kind == 3 ? 1 : 2; }
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FDIAG = 4, L_FOFF = 5, L_BOFF = 6, L_BDIAG = 7 };
 
 struct StepBlocks {
@@ -152,7 +157,7 @@ struct mxc_factor {
 
     // tile edge per (M, N) bucket: 32 up to 64 x 32, 128 for M > 128 and N > 32,
     // else 64 (mxc_set_tile overrides an entry before the plan is built)
-    int tune[NBUCKET];
+    int tune[NKIND][NBUCKET];
 
     // hipGraph replay: the launch sequence depends only on the pattern, so it is
     // captured once and replayed for every refactorization / solve
@@ -232,7 +237,7 @@ static int flatten_upload(mxc_factor *F) {
         for (int ts : {32, 64, 128}) {
             long long b = (long long)tiles.size();
             for (const UpdDesc &u : blocks) {
-                if (F->tune[tile_bucket(u.M, u.N)] != ts) continue;
+                if (F->tune[kind_of(kind)][tile_bucket(u.M, u.N, u.K)] != ts) continue;
                 int di = (int)descs.size();
                 descs.push_back(u);
                 int64_t mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
@@ -658,10 +663,11 @@ extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const
     mxc_factor *F = new (std::nothrow) mxc_factor();
     if (!F) return MXC_OUT_OF_MEMORY;
     F->precision = precision;
-    for (int b = 0; b < NBUCKET; b++) {
-        int mb = b / 5, nbk = b % 5;
-        F->tune[b] = (mb <= 2 && nbk <= 2) ? 32 : (mb == 4 && nbk >= 3) ? 128 : 64;
-    }
+    for (int k = 0; k < NKIND; k++)
+        for (int b = 0; b < NBUCKET; b++) {
+            int mb = b / 15, nbk = (b / 3) % 5;
+            F->tune[k][b] = (mb <= 2 && nbk <= 2) ? 32 : 64;
+        }
     hipError_t e = hipSetDevice(g_device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -1299,7 +1305,8 @@ static int solve_refine_impl(mxc_factor *F, const double *B, int64_t nrhs, doubl
 // sample of) that bucket's blocks and keep the fastest
 // ---------------------------------------------------------------------------
 template <typename T>
-static int time_tiles(mxc_factor *F, const std::vector<UpdDesc> &blocks, int ts, int reps, float *ms) {
+static int time_tiles(mxc_factor *F, int kind, const std::vector<UpdDesc> &blocks, int ts, int reps,
+                      float *ms) {
     std::vector<UpdDesc> descs(blocks);
     std::vector<Tile> tiles;
     for (size_t di = 0; di < descs.size(); di++) {
@@ -1307,7 +1314,7 @@ static int time_tiles(mxc_factor *F, const std::vector<UpdDesc> &blocks, int ts,
         int64_t mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
         for (int64_t ti = 0; ti < mt; ti++)
             for (int64_t tj = 0; tj < nt; tj++)
-                if (ti >= tj) tiles.push_back(Tile{(int)di, (int)ti, (int)tj});
+                if (kind == 2 || ti >= tj) tiles.push_back(Tile{(int)di, (int)ti, (int)tj});
     }
     UpdDesc *d_desc = nullptr;
     Tile *d_tiles = nullptr;
@@ -1324,22 +1331,23 @@ static int time_tiles(mxc_factor *F, const std::vector<UpdDesc> &blocks, int ts,
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     T *W = (T *)F->W;
+    Launch L{kind == 0 ? L_UPD : kind == 1 ? L_TRAIL : L_TRSM, 0, (long long)tiles.size(), ts};
+    UpdDesc *save_desc = F->d_desc;
+    Tile *save_tiles = F->d_tiles;
+    F->d_desc = d_desc;
+    F->d_tiles = d_tiles;
     auto launch = [&]() {
-        if (ts == 32)
-            hipLaunchKernelGGL((k_tile<T, MODE_SCATTER, 32>), dim3((unsigned)tiles.size()), dim3(64), 0, sm,
-                               d_desc, d_tiles, W, F->d_Ls);
-        else if (ts == 64)
-            hipLaunchKernelGGL((k_tile<T, MODE_SCATTER, 64>), dim3((unsigned)tiles.size()), dim3(256), 0, sm,
-                               d_desc, d_tiles, W, F->d_Ls);
-        else
-            hipLaunchKernelGGL((k_tile<T, MODE_SCATTER, 128>), dim3((unsigned)tiles.size()), dim3(256), 0, sm,
-                               d_desc, d_tiles, W, F->d_Ls);
+        if (kind == 0) launch_tile<T, MODE_SCATTER>(F, L, W);
+        else if (kind == 1) launch_tile<T, MODE_TRAIL>(F, L, W);
+        else launch_tile<T, MODE_STORE>(F, L, W);
     };
     launch();                        // warm-up
     (void)hipEventRecord(e0, sm);
     for (int r = 0; r < reps; r++) launch();
     (void)hipEventRecord(e1, sm);
     e = hipEventSynchronize(e1);
+    F->d_desc = save_desc;
+    F->d_tiles = save_tiles;
     float t = 0;
     if (e == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
     *ms = t / (float)std::max(reps, 1);
@@ -1354,32 +1362,44 @@ static int time_tiles(mxc_factor *F, const std::vector<UpdDesc> &blocks, int ts,
 
 template <typename T>
 static int autotune_impl(mxc_factor *F, int reps, int *table) {
-    std::vector<std::vector<UpdDesc>> byb(NBUCKET);
-    for (const auto &lv : F->blk_upd)
-        for (const UpdDesc &u : lv) byb[tile_bucket(u.M, u.N)].push_back(u);
     const size_t cap = 4096;         // blocks timed per bucket (evenly strided sample)
-    for (int b = 0; b < NBUCKET; b++) {
-        if (byb[b].empty()) continue;
-        std::vector<UpdDesc> sample;
-        size_t stride = std::max<size_t>(1, byb[b].size() / cap);
-        for (size_t q = 0; q < byb[b].size() && sample.size() < cap; q += stride) sample.push_back(byb[b][q]);
-        float best = -1;
-        int best_ts = F->tune[b];
-        for (int ts : {32, 64, 128}) {
-            float ms = 0;
-            int rc = time_tiles<T>(F, sample, ts, reps, &ms);
-            if (rc) return rc;
-            if (best < 0 || ms < best) {
-                best = ms;
-                best_ts = ts;
+    for (int kind = 0; kind < NKIND; kind++) {
+        std::vector<std::vector<UpdDesc>> byb(NBUCKET);
+        for (size_t l = 0; l < F->blk_upd.size(); l++) {
+            if (kind == 0) {
+                for (const UpdDesc &u : F->blk_upd[l]) byb[tile_bucket(u.M, u.N, u.K)].push_back(u);
+            } else {
+                for (const StepBlocks &sb : F->blk_steps[l])
+                    for (const UpdDesc &u : (kind == 1 ? sb.trail : sb.trsm))
+                        byb[tile_bucket(u.M, u.N, u.K)].push_back(u);
             }
         }
-        F->tune[b] = best_ts;
+        for (int b = 0; b < NBUCKET; b++) {
+            if (byb[b].empty()) continue;
+            std::vector<UpdDesc> sample;
+            size_t stride = std::max<size_t>(1, byb[b].size() / cap);
+            for (size_t q = 0; q < byb[b].size() && sample.size() < cap; q += stride)
+                sample.push_back(byb[b][q]);
+            float best = -1;
+            int best_ts = F->tune[kind][b];
+            for (int ts : {32, 64, 128}) {
+                float ms = 0;
+                int rc = time_tiles<T>(F, kind, sample, ts, reps, &ms);
+                if (rc) return rc;
+                if (best < 0 || ms < best) {
+                    best = ms;
+                    best_ts = ts;
+                }
+            }
+            F->tune[kind][b] = best_ts;
+        }
     }
     F->values_loaded = false;        // the timing launches scribbled on L
     int rc = flatten_upload(F);
     if (rc) return rc;
-    if (table) for (int b = 0; b < NBUCKET; b++) table[b] = F->tune[b];
+    if (table)
+        for (int k = 0; k < NKIND; k++)
+            for (int b = 0; b < NBUCKET; b++) table[k * NBUCKET + b] = F->tune[k][b];
     return MXC_OK;
 }
 
@@ -1392,15 +1412,17 @@ extern "C" int mxc_autotune(mxc_factor *F, int reps, int *table) {
 
 extern "C" int mxc_set_tiles(mxc_factor *F, const int *table) {
     if (!F || !table) return MXC_INVALID;
-    for (int b = 0; b < NBUCKET; b++)
-        if (table[b] != 32 && table[b] != 64 && table[b] != 128) return MXC_INVALID;
-    for (int b = 0; b < NBUCKET; b++) F->tune[b] = table[b];
+    for (int q = 0; q < NKIND * NBUCKET; q++)
+        if (table[q] != 32 && table[q] != 64 && table[q] != 128) return MXC_INVALID;
+    for (int k = 0; k < NKIND; k++)
+        for (int b = 0; b < NBUCKET; b++) F->tune[k][b] = table[k * NBUCKET + b];
     return flatten_upload(F);
 }
 
 extern "C" int mxc_get_tiles(const mxc_factor *F, int *table) {
     if (!F || !table) return MXC_INVALID;
-    for (int b = 0; b < NBUCKET; b++) table[b] = F->tune[b];
+    for (int k = 0; k < NKIND; k++)
+        for (int b = 0; b < NBUCKET; b++) table[k * NBUCKET + b] = F->tune[k][b];
     return MXC_OK;
 }
 

@@ -145,6 +145,15 @@ class Server:
                     return req, p
                 continue
             need = self._estimate(req)
+            if need > self.budget:
+                # can never fit: fail it (CHOLMOD_OUT_OF_MEMORY) instead of waiting forever
+                del self._queue[idx]
+                req.status = -2
+                req.info = {"error": "plan larger than the HBM budget"}
+                req.start = req.done = self.now()
+                self.completed.append(req)
+                self._cv.notify_all()
+                return self._take()
             if self._reserved + need > self.budget:
                 self._evict(self._reserved + need - self.budget)
             if self._reserved + need <= self.budget:
