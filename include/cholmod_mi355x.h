@@ -139,6 +139,12 @@ int mxc_solve(mxc_factor *F, const double *B, int64_t nrhs, double *X, int locat
 int mxc_super_lsolve(mxc_factor *F, double *X, int64_t nrhs, int location);
 int mxc_super_ltsolve(mxc_factor *F, double *X, int64_t nrhs, int location);
 
+/* Reciprocal condition estimate of the last factorization, as
+ * cholmod_l_rcond (CHOLMOD/Cholesky/cholmod_rcond.c:51-112) computes it for an
+ * LL' factor: (min diag(L) / max diag(L))^2; 0 if the factorization stopped
+ * early (minor < n) or a diagonal entry is NaN; 1 when n = 0. */
+int mxc_rcond(mxc_factor *F, double *rcond);
+
 /* R = B - A X in FP64 with the values of the last mxc_factorize. */
 int mxc_residual(mxc_factor *F, const double *B, const double *X, int64_t nrhs, double *R,
                  int location);

@@ -401,6 +401,20 @@ def solve_refine(L: Factor, B, cm: Common, tol: float = 1e-10, maxiter: int = 10
     return X, {"iterations": int(out[0]), "relres": float(out[1]), "relres0": float(out[2])}
 
 
+def rcond(L: Factor, cm: Common) -> float:
+    """cholmod_l_rcond: (min diag(L) / max diag(L))^2 of the current factor."""
+    cm.status = CHOLMOD_OK
+    if L is None or not L.is_numeric:
+        cm.status = CHOLMOD_INVALID
+        return -1.0
+    out = C.c_double(0)
+    st = lib().mxc_rcond(L._num, C.byref(out))
+    if st != 0:
+        cm.status = st
+        return -1.0
+    return out.value
+
+
 def residual(L: Factor, B, X):
     """R = B - A X in FP64 on the device (values of the last factorization)."""
     B2 = np.asfortranarray(np.asarray(B, dtype=np.float64).reshape(L.n, -1))

@@ -72,6 +72,40 @@ __global__ void k_add_diag(const long long *__restrict__ dpos, long long n,
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
+// Min / max of the diagonal of L (rcond diagnostic): each block writes
+// (min, max, has_nan) of its share to out[3*blockIdx.x ...].
+template <typename T>
+__global__ void k_diag_minmax(const T *__restrict__ W, const long long *__restrict__ dpos, long long n,
+                              double *__restrict__ out) {
+    __shared__ double smin[256], smax[256];
+    __shared__ int snan;
+    if (threadIdx.x == 0) snan = 0;
+    __syncthreads();
+    double lo = INFINITY, hi = -INFINITY;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const double v = (double)W[dpos[k]];
+        if (isnan(v)) snan = 1;
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[3 * blockIdx.x] = smin[0];
+        out[3 * blockIdx.x + 1] = smax[0];
+        out[3 * blockIdx.x + 2] = snan;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Dense TS x TS update tile on MFMA (see UpdDesc).  TS = 32: one wave owns the
 // tile; TS = 64 / 128: four waves, each owning a (TS/2) x (TS/2) quarter, as

@@ -149,6 +149,7 @@ struct mxc_factor {
     std::vector<Launch> factor_launches;   // per level: UPD, then (DIAG, TRSM, TRAIL)*
     std::vector<Launch> fwd_launches, bwd_launches;
     bool values_loaded = false;
+    int64_t minor = 0;                     // L->minor of the last factorization
 
     // host copy of the factorization's blocks (re-flattened after autotuning)
     std::vector<std::vector<UpdDesc>> blk_upd;        // per level
@@ -887,9 +888,11 @@ extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, doub
     HIPTRY(hipStreamSynchronize(sm));
     F->values_loaded = true;
     if (fail == 0x7fffffff) {
+        F->minor = F->n;
         if (minor) *minor = F->n;
         return MXC_OK;
     }
+    F->minor = fail;
     rc = F->precision == MXC_FP64 ? finish_not_posdef<double>(F, fail, beta)
                                   : finish_not_posdef<float>(F, fail, beta);
     if (minor) *minor = fail;
@@ -1446,6 +1449,40 @@ extern "C" int mxc_solve_refine_segments(mxc_factor *F, const double *B, int64_t
                                          int location, double tol, int maxiter, int64_t nseg,
                                          const int64_t *segoff, double *out) {
     return solve_refine_impl(F, B, nrhs, X, location, tol, maxiter, nseg, segoff, out);
+}
+
+// (min diag(L) / max diag(L))^2, 0 if any diagonal entry is NaN or the
+// factorization stopped early, 1 for n = 0 (CHOLMOD/Cholesky/cholmod_rcond.c:51-112,
+// t_cholmod_rcond_worker.c:53-143).
+extern "C" int mxc_rcond(mxc_factor *F, double *rcond) {
+    if (!F || !rcond) return MXC_INVALID;
+    if (!F->values_loaded) return MXC_INVALID;
+    if (F->n == 0) { *rcond = 1; return MXC_OK; }
+    if (F->minor < F->n) { *rcond = 0; return MXC_OK; }
+    HIPTRY(hipSetDevice(g_device));
+    const int G = 256;
+    double *d_out = nullptr;
+    HIPTRY(hipMalloc((void **)&d_out, sizeof(double) * 3 * G));
+    if (F->precision == MXC_FP64)
+        hipLaunchKernelGGL(k_diag_minmax<double>, dim3(G), dim3(256), 0, F->stream, (const double *)F->W,
+                           F->d_dpos, (long long)F->n, d_out);
+    else
+        hipLaunchKernelGGL(k_diag_minmax<float>, dim3(G), dim3(256), 0, F->stream, (const float *)F->W,
+                           F->d_dpos, (long long)F->n, d_out);
+    std::vector<double> h(3 * G);
+    hipError_t e = hipMemcpyAsync(h.data(), d_out, sizeof(double) * 3 * G, hipMemcpyDeviceToHost, F->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(F->stream);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return set_hip_err(e, "rcond", __LINE__);
+    double lo = INFINITY, hi = -INFINITY;
+    for (int b = 0; b < G; b++) {
+        if (h[3 * b + 2] != 0) { *rcond = 0; return MXC_OK; }
+        lo = std::min(lo, h[3 * b]);
+        hi = std::max(hi, h[3 * b + 1]);
+    }
+    double r = lo / hi;
+    *rcond = r * r;
+    return MXC_OK;
 }
 
 extern "C" int mxc_synchronize(mxc_factor *F) {

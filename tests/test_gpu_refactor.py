@@ -87,3 +87,29 @@ def test_device_resident_inputs(gpu, ref_lib):
     Z = cholmod.solve(cholmod.CHOLMOD_Lt, L, B, cm)
     Zo = restate.ltsolve(sym, Lx, B)
     assert np.linalg.norm(Z - Zo) <= 1e-12 * np.linalg.norm(Zo)
+
+
+def test_rcond_matches_reference_definition(gpu, ref_lib):
+    """rcond = (min diag L / max diag L)^2 as cholmod_l_rcond computes it, on the
+    reference's own factor; 0 for a not-positive-definite factorization."""
+    A = problems.grid2d_lognormal(30, np.random.default_rng(9))
+    R = ref_lib.RefFactor(A, ref_lib.AMD)
+    sym = R.symbolic()
+    R.factorize()
+    Lref = R.Lx()
+    d = np.concatenate([Lref[sym["Lpx"][s] + np.arange(sym["Super"][s + 1] - sym["Super"][s]) *
+                             (sym["Lpi"][s + 1] - sym["Lpi"][s] + 1)] for s in range(len(sym["Super"]) - 1)])
+    expect = (d.min() / d.max()) ** 2
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+        cm = cholmod.Common(precision=prec)
+        L = cholmod.analyze_p(A, sym["Perm"], cm)
+        assert cholmod.factorize(A, L, cm)
+        rc = cholmod.rcond(L, cm)
+        assert abs(rc - expect) <= tol * expect, (prec, rc, expect)
+    x = A.x.copy()
+    x[A.p[:-1][10]] = -1.0
+    A2 = problems.SymMatrix(n=A.n, p=A.p, i=A.i, x=x)
+    cm = cholmod.Common()
+    L = cholmod.analyze_p(A2, sym["Perm"], cm)
+    assert cholmod.factorize(A2, L, cm) and cm.status == cholmod.CHOLMOD_NOT_POSDEF
+    assert cholmod.rcond(L, cm) == 0.0
