@@ -113,8 +113,9 @@ int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const int64_t *A
 
 void mxc_factor_free(mxc_factor *F);
 
-/* info[0..7] = n, nsuper, xsize, levels, update pairs, update tiles,
- *              device bytes held, precision;
+/* info[0..8] = n, nsuper, xsize, levels, update pairs, update tiles,
+ *              device bytes held, precision, pivots perturbed by the
+ *              low-precision safeguard in the last factorization;
  * dinfo[0..5] = fl, lnz, and the algorithmic flops of one factorization by
  *               kernel class: descendant updates (dsyrk + dgemm counts), diagonal
  *               POTRF, panel TRSM, in-supernode trailing updates */
@@ -151,8 +152,11 @@ int mxc_residual(mxc_factor *F, const double *B, const double *X, int64_t nrhs, 
 
 /* Solve with FP64 iterative refinement: x_0 = A\b through the factor, then
  * r = b - A x (FP64), x += A\r until ||r||_2 / ||b||_2 <= tol for every
- * column or maxiter corrections.  out[0] = corrections done, out[1] = final
- * max relative residual, out[2] = initial max relative residual. */
+ * column or maxiter corrections.  If a low-precision factor does not reach tol
+ * (e.g. its pivots broke down: see MXC_OPT_PIVOT_RHO_E9), an FP64 factor of
+ * the same matrix is built, factored and used instead.  out[0] = corrections
+ * done, out[1] = final max relative residual, out[2] = initial max relative
+ * residual, out[3] = 1 if the FP64 fallback factor was used. */
 int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location,
                      double tol, int maxiter, double *out);
 
@@ -160,6 +164,13 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * solve sweeps into hipGraphs on first use and replay them afterwards (the
  * launch sequence depends only on the pattern); 0 enqueues every launch. */
 #define MXC_OPT_GRAPHS 1
+/* MXC_OPT_PIVOT_RHO_E9: pivot-ratio threshold rho in units of 1e-9 (FP32
+ * factors default to rho = 1e-6, FP64 to 0).  With rho > 0, a diagonal-block
+ * pivot d_jj <= rho * a_jj (a_jj the assembled diagonal of A) is raised to
+ * rho * a_jj and counted (mxc_factor_info info[8]); the refinement, or the
+ * FP64 fallback factor, corrects the perturbation.  rho = 0 keeps CHOLMOD's
+ * semantics: a pivot <= 0 stops with MXC_NOT_POSDEF. */
+#define MXC_OPT_PIVOT_RHO_E9 2
 int mxc_set_option(mxc_factor *F, int option, int64_t value);
 
 /* As mxc_solve_refine for a block-diagonal batch of independent systems: rows

@@ -147,10 +147,11 @@ class Factor:
         return out
 
     def info(self) -> dict:
-        info = np.zeros(8, dtype=np.int64)
+        info = np.zeros(9, dtype=np.int64)
         dinfo = np.zeros(6)
         check("mxc_factor_info", lib().mxc_factor_info(self._num, _ptr(info), _ptr(dinfo)))
-        keys = ("n", "nsuper", "xsize", "levels", "pairs", "update_tiles", "device_bytes", "precision")
+        keys = ("n", "nsuper", "xsize", "levels", "pairs", "update_tiles", "device_bytes", "precision",
+                "perturbed_pivots")
         return dict(zip(keys, (int(v) for v in info)))
 
     def set_timing(self, on, classes=None):
@@ -160,6 +161,10 @@ class Factor:
         if on:
             mask = 1 if classes is None else sum(1 << c for c in classes) | (1 << 30)
         check("mxc_set_timing", lib().mxc_set_timing(self._num, int(mask)))
+
+    def set_pivot_rho(self, rho: float):
+        """Pivot-ratio threshold of the low-precision safeguard (0 = CHOLMOD semantics)."""
+        check("mxc_set_option", lib().mxc_set_option(self._num, 2, int(round(rho * 1e9))))
 
     def set_graphs(self, on: bool):
         """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""
@@ -184,7 +189,7 @@ class Factor:
 
     def kernel_flops(self) -> dict:
         """Algorithmic flops of one factorization by kernel class."""
-        info = np.zeros(8, dtype=np.int64)
+        info = np.zeros(9, dtype=np.int64)
         dinfo = np.zeros(6)
         check("mxc_factor_info", lib().mxc_factor_info(self._num, _ptr(info), _ptr(dinfo)))
         return {"update": float(dinfo[2]), "potrf": float(dinfo[3]), "trsm": float(dinfo[4]),
@@ -387,7 +392,7 @@ def solve_refine(L: Factor, B, cm: Common, tol: float = 1e-10, maxiter: int = 10
         Bc = np.asfortranarray(np.asarray(B, dtype=np.float64).reshape(L.n, -1))
         X = np.zeros_like(Bc, order="F")
         loc = _lib.HOST
-    out = np.zeros(3)
+    out = np.zeros(4)
     if segments is None:
         st = lib().mxc_solve_refine(L._num, _ptr(Bc), Bc.shape[1], _ptr(X), loc, float(tol),
                                     int(maxiter), _ptr(out))
@@ -398,7 +403,8 @@ def solve_refine(L: Factor, B, cm: Common, tol: float = 1e-10, maxiter: int = 10
     if st != 0:
         cm.status = st
         return None, None
-    return X, {"iterations": int(out[0]), "relres": float(out[1]), "relres0": float(out[2])}
+    return X, {"iterations": int(out[0]), "relres": float(out[1]), "relres0": float(out[2]),
+               "fp64_fallback": bool(out[3])}
 
 
 def rcond(L: Factor, cm: Common) -> float:

@@ -63,11 +63,17 @@ __global__ void k_scatter_a(const double *__restrict__ Ax, const long long *__re
 // serves every beta
 template <typename T>
 __global__ void k_add_diag(const long long *__restrict__ dpos, long long n,
-                           const double *__restrict__ beta, T *__restrict__ W) {
+                           const double *__restrict__ beta, T *__restrict__ W,
+                           double *__restrict__ adiag) {
     const double b = *beta;
-    if (b == 0.0) return;
     long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) W[dpos[k]] += (T)b;
+    if (k >= n) return;
+    T v = W[dpos[k]];
+    if (b != 0.0) {
+        v += (T)b;
+        W[dpos[k]] = v;
+    }
+    adiag[k] = (double)v;           // assembled diagonal, for the pivot-ratio test
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -117,7 +123,7 @@ __global__ void k_diag_minmax(const T *__restrict__ W, const long long *__restri
 // consecutive C rows: stores into the column-major target are contiguous runs.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, int TS>
-__global__ __launch_bounds__(TS == 32 ? 64 : 256) void k_tile(const UpdDesc *__restrict__ descs,
+__global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile(const UpdDesc *__restrict__ descs,
                                                               const Tile *__restrict__ tiles,
                                                               T *__restrict__ W,
                                                               const int *__restrict__ Ls) {
@@ -302,9 +308,15 @@ __device__ __forceinline__ void store16(T *S, int r0, int c0, const typename Mfm
     }
 }
 
+// rho > 0 (low-precision factor): a pivot d_jj with d_jj <= rho * a_jj (a_jj the
+// assembled diagonal of A) signals breakdown; it is replaced by rho * a_jj and
+// counted in *npert, and the FP64 refinement (or the FP64 fallback factor)
+// repairs the perturbation.  rho = 0: exact semantics, a pivot <= 0 stops.
 template <typename T>
 __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__ tasks,
-                                                    T *__restrict__ W, int *__restrict__ fail) {
+                                                    T *__restrict__ W, int *__restrict__ fail,
+                                                    const double *__restrict__ adiag, double rho,
+                                                    int *__restrict__ npert) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     __shared__ T S[NB * DS];       // the block, row-major
     __shared__ T X[NB * DS];       // its inverse
@@ -342,6 +354,13 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
 #pragma unroll
             for (int j = 0; j < 16; j++) {
                 T djj = lane_bcast(a[j], j);
+                if (rho > 0 && R + j < nb) {
+                    const T floor_j = (T)(rho * adiag[t.col0 + R + j]);
+                    if (djj <= floor_j && floor_j > (T)0) {
+                        djj = floor_j;
+                        if (lane == 0) atomicAdd(npert, 1);
+                    }
+                }
                 if (!(djj > (T)0)) {
                     if (lane == 0 && R + j < nb && sfail < 0) sfail = R + j;
                     djj = 1;

@@ -136,7 +136,8 @@ struct mxc_factor {
     Tile *d_tiles = nullptr;
     DiagTask *d_diag = nullptr;
     SolveTask *d_stask = nullptr;
-    int *d_fail = nullptr;
+    int *d_fail = nullptr;            // [0] first failing column, [1] perturbed pivots
+    double *d_adiag = nullptr;        // assembled diagonal of A (n)
     long long *d_rowptr = nullptr, *d_src = nullptr;
     int *d_col = nullptr;
     double *d_vals = nullptr, *d_Ax = nullptr;
@@ -150,7 +151,11 @@ struct mxc_factor {
     std::vector<Launch> fwd_launches, bwd_launches;
     bool values_loaded = false;
     int64_t minor = 0;                     // L->minor of the last factorization
-
+    int64_t nperturbed = 0;                // pivots raised by the low-precision safeguard
+    double rho = 0;                        // pivot-ratio threshold (0: exact semantics)
+    // FP64 fallback factor of a low-precision factor whose refinement stalls
+    mxc_factor *fallback = nullptr;
+    std::vector<int64_t> Ls_host, Ap_host, Ai_host;
     // host copy of the factorization's blocks (re-flattened after autotuning)
     std::vector<std::vector<UpdDesc>> blk_upd;        // per level
     std::vector<std::vector<StepBlocks>> blk_steps;   // per level, per 64-column step
@@ -607,7 +612,11 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     if ((rc = dupload(F, &F->d_src, src, sm))) return rc;
     if ((rc = dalloc(F, &F->d_vals, (size_t)std::max<int64_t>(F->nnz_full, 1)))) return rc;
     if ((rc = dalloc(F, &F->d_Ax, (size_t)std::max<int64_t>(nnz, 1)))) return rc;
-    if ((rc = dalloc(F, &F->d_fail, 1))) return rc;
+    if ((rc = dalloc(F, &F->d_fail, 2))) return rc;
+    if ((rc = dalloc(F, &F->d_adiag, (size_t)std::max<int64_t>(n, 1)))) return rc;
+    F->Ls_host.assign(S->Ls, S->Ls + S->ssize);
+    F->Ap_host.assign(Ap, Ap + n + 1);
+    F->Ai_host.assign(Ai, Ai + nnz);
     if ((rc = dalloc(F, &F->d_norm, 64))) return rc;
     if ((rc = dalloc(F, &F->d_beta, 1))) return rc;
     HIPTRY(hipStreamSynchronize(sm));
@@ -643,8 +652,9 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
     if (!F) return;
     drop_sweep_graphs(F);
     drop_factor_graph(F);
+    if (F->fallback) mxc_factor_free(F->fallback);
     void *ptrs[] = {F->W, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
-                    F->d_diag, F->d_stask, F->d_fail, F->d_rowptr, F->d_src, F->d_col,
+                    F->d_diag, F->d_stask, F->d_fail, F->d_adiag, F->d_rowptr, F->d_src, F->d_col,
                     F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_b, F->d_x, F->d_r};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -664,6 +674,7 @@ extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const
     mxc_factor *F = new (std::nothrow) mxc_factor();
     if (!F) return MXC_OUT_OF_MEMORY;
     F->precision = precision;
+    F->rho = precision == MXC_FP32 ? 1e-6 : 0.0;
     for (int k = 0; k < NKIND; k++)
         for (int b = 0; b < NBUCKET; b++) {
             int mb = b / 15, nbk = (b / 3) % 5;
@@ -696,6 +707,7 @@ extern "C" int mxc_factor_info(const mxc_factor *F, int64_t *info, double *dinfo
         info[5] = F->nupd_tiles;
         info[6] = (int64_t)F->bytes;
         info[7] = F->precision;
+        info[8] = F->nperturbed;
     }
     if (dinfo) {
         dinfo[0] = F->fl;
@@ -796,13 +808,14 @@ static int enqueue_factor(mxc_factor *F) {
     size_t t0 = 0, tt = 0;
     tick(F, 8, &t0);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_fail, INT_MAX);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_fail + 1, 0);
     // assembly: L = 0, scatter A, + beta on the diagonal; also refresh the CSR values
     tick(F, 0, &tt);
     hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(F->xsize, 256)), dim3(256), 0, sm, W, (long long)F->xsize, (T)0);
     hipLaunchKernelGGL(k_scatter_a<T>, dim3(grid_for(F->nnz_a, 256)), dim3(256), 0, sm, F->d_Ax,
                        F->d_pos, (long long)F->nnz_a, W);
     hipLaunchKernelGGL(k_add_diag<T>, dim3(grid_for(F->n, 256, 1LL << 30)), dim3(256), 0, sm,
-                       F->d_dpos, (long long)F->n, (const double *)F->d_beta, W);
+                       F->d_dpos, (long long)F->n, (const double *)F->d_beta, W, F->d_adiag);
     hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(F->nnz_full, 256)), dim3(256), 0, sm,
                        F->d_Ax, F->d_src, (long long)F->nnz_full, F->d_vals);
     tock(F, 0, tt);
@@ -816,7 +829,8 @@ static int enqueue_factor(mxc_factor *F) {
                 break;
             case L_DIAG:
                 hipLaunchKernelGGL(k_potrf_diag<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
-                                   F->d_diag + L.begin, W, F->d_fail);
+                                   F->d_diag + L.begin, W, F->d_fail, (const double *)F->d_adiag,
+                                   F->rho, F->d_fail + 1);
                 F->counts[1]++;
                 break;
             case L_TRSM:
@@ -883,9 +897,11 @@ extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, doub
     HIPTRY(hipMemcpyAsync(F->d_beta, &beta, sizeof(double), hipMemcpyHostToDevice, sm));
     int rc = F->precision == MXC_FP64 ? run_factor<double>(F) : run_factor<float>(F);
     if (rc) return rc;
-    int fail = 0;
-    HIPTRY(hipMemcpyAsync(&fail, F->d_fail, sizeof(int), hipMemcpyDeviceToHost, sm));
+    int flags[2] = {0, 0};
+    HIPTRY(hipMemcpyAsync(flags, F->d_fail, sizeof(int) * 2, hipMemcpyDeviceToHost, sm));
     HIPTRY(hipStreamSynchronize(sm));
+    int fail = flags[0];
+    F->nperturbed = flags[1];
     F->values_loaded = true;
     if (fail == 0x7fffffff) {
         F->minor = F->n;
@@ -1277,6 +1293,32 @@ static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *
     return MXC_OK;
 }
 
+// Build (once) and factor an FP64 twin of a low-precision factor, from the
+// kept symbolic arrays and the values of the last factorization.
+static int fp64_fallback(mxc_factor *F) {
+    if (!F->fallback) {
+        mxc_symbolic S;
+        memset(&S, 0, sizeof(S));
+        S.n = F->n;
+        S.nsuper = F->nsuper;
+        S.ssize = (int64_t)F->Ls_host.size();
+        S.xsize = F->xsize;
+        S.fl = F->fl;
+        S.lnz = F->lnz;
+        S.Perm = F->Perm.data();
+        S.Super = F->Super.data();
+        S.Lpi = F->Lpi.data();
+        S.Lpx = F->Lpx.data();
+        S.Ls = F->Ls_host.data();
+        int rc = mxc_factor_create(&S, F->Ap_host.data(), F->Ai_host.data(), MXC_FP64, &F->fallback);
+        if (rc) return rc;
+    }
+    double beta = 0;
+    HIPTRY(hipMemcpy(&beta, F->d_beta, sizeof(double), hipMemcpyDeviceToHost));
+    int64_t minor = 0;
+    return mxc_factorize(F->fallback, F->d_Ax, MXC_DEVICE, beta, &minor);   // NOT_POSDEF propagates
+}
+
 static int solve_refine_impl(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location,
                              double tol, int maxiter, int64_t nseg, const int64_t *segoff, double *out) {
     if (!F || !B || !X || nrhs < 0 || nrhs > 64) return MXC_INVALID;
@@ -1294,9 +1336,31 @@ static int solve_refine_impl(mxc_factor *F, const double *B, int64_t nrhs, doubl
     const double *dB;
     if ((rc = stage_in(F, B, nrhs, location, &dB))) return rc;
     double *dX = location == MXC_DEVICE ? X : F->d_x;
-    rc = F->precision == MXC_FP64 ? refine_device<double>(F, dB, nrhs, dX, tol, maxiter, nseg, segoff, out)
-                                  : refine_device<float>(F, dB, nrhs, dX, tol, maxiter, nseg, segoff, out);
+    double o[3] = {0, 0, 0};
+    rc = F->precision == MXC_FP64 ? refine_device<double>(F, dB, nrhs, dX, tol, maxiter, nseg, segoff, o)
+                                  : refine_device<float>(F, dB, nrhs, dX, tol, maxiter, nseg, segoff, o);
     if (rc) return rc;
+    int used_fallback = 0;
+    if (F->precision != MXC_FP64 && !(o[1] <= tol)) {
+        // the low-precision factor did not reach tol: refactor in FP64 and refine with it
+        if ((rc = fp64_fallback(F))) return rc;
+        mxc_factor *G = F->fallback;
+        if ((rc = ensure_work(G, nrhs))) return rc;
+        HIPTRY(hipStreamSynchronize(F->stream));
+        double o2[3] = {0, 0, 0};
+        rc = refine_device<double>(G, dB, nrhs, dX, tol, maxiter, nseg, segoff, o2);
+        if (rc) return rc;
+        HIPTRY(hipStreamSynchronize(G->stream));
+        o[0] += o2[0];
+        o[1] = o2[1];
+        used_fallback = 1;
+    }
+    if (out) {
+        out[0] = o[0];
+        out[1] = o[1];
+        out[2] = o[2];
+        out[3] = used_fallback;
+    }
     if (location != MXC_DEVICE)
         HIPTRY(hipMemcpyAsync(X, dX, F->n * nrhs * sizeof(double), hipMemcpyDeviceToHost, F->stream));
     HIPTRY(hipStreamSynchronize(F->stream));
@@ -1434,6 +1498,11 @@ extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
     switch (option) {
         case MXC_OPT_GRAPHS:
             F->use_graphs = value != 0;
+            return MXC_OK;
+        case MXC_OPT_PIVOT_RHO_E9:
+            if (value < 0) return MXC_INVALID;
+            F->rho = (double)value * 1e-9;
+            drop_factor_graph(F);
             return MXC_OK;
         default:
             return MXC_INVALID;

@@ -87,3 +87,46 @@ def test_factor_fp32_refine(name, make, ordering, gpu, ref_lib):
     assert info["relres"] <= 1e-10
     r = B - A.matvec(X)
     assert np.linalg.norm(r, axis=0).max() / np.linalg.norm(B, axis=0).min() <= 1e-9
+
+
+def _near_singular_pairs(m, delta, seed=0):
+    """Block-diagonal SPD matrix of 2x2 blocks [[1, 1-delta], [1-delta, 1]] (pivot
+    ~2*delta: cond ~ 1/delta), weakly coupled along a chain and symmetrically scaled."""
+    import scipy.sparse as sp
+    n = 2 * m
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for b in range(m):
+        i = 2 * b
+        rows += [i, i + 1, i + 1]
+        cols += [i, i, i + 1]
+        vals += [1.0, 1.0 - delta, 1.0]
+        if b + 1 < m:                       # tiny coupling to the next block
+            rows.append(i + 2)
+            cols.append(i)
+            vals.append(1e-3 * delta)
+    T = sp.csc_matrix((vals, (rows, cols)), shape=(n, n))
+    d = np.exp(rng.uniform(-1, 1, n))
+    T = sp.diags(d) @ T @ sp.diags(d)
+    T = sp.tril(T).tocsc()
+    T.sort_indices()
+    return problems.SymMatrix(n=n, p=T.indptr.astype(np.int64), i=T.indices.astype(np.int64),
+                              x=T.data, name=f"pairs{m}")
+
+
+def test_fp32_breakdown_falls_back_to_fp64(gpu):
+    """FP32 pivots that break down (pivot ratio ~1e-9 < rho = 1e-6) are raised and
+    counted; refinement through the perturbed FP32 factor cannot reach 1e-10, so
+    the FP64 twin factor is used and does."""
+    A = _near_singular_pairs(200, 1e-9)
+    cm = cholmod.Common(precision="fp32")
+    L = cholmod.analyze_p(A, np.arange(A.n), cm)
+    assert cholmod.factorize(A, L, cm) and cm.status == 0
+    assert L.info()["perturbed_pivots"] > 0
+    B = np.random.default_rng(1).uniform(-1, 1, (A.n, 2))
+    X, info = cholmod.solve_refine(L, B, cm, tol=1e-10, maxiter=5)
+    print(info, flush=True)
+    assert info["fp64_fallback"]
+    assert info["relres"] <= 1e-10
+    R = B - A.matvec(X)
+    assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) <= 1e-9
