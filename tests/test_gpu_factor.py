@@ -123,7 +123,13 @@ def test_fp32_breakdown_falls_back_to_fp64(gpu):
     L = cholmod.analyze_p(A, np.arange(A.n), cm)
     assert cholmod.factorize(A, L, cm) and cm.status == 0
     assert L.info()["perturbed_pivots"] > 0
-    B = np.random.default_rng(1).uniform(-1, 1, (A.n, 2))
+    # B = A X_true with X_true = O(1): then ||x|| ~ ||b||, so FP64's backward error
+    # (~eps ||A|| ||x|| / ||b||) is far below 1e-10, while b's components along the
+    # ~2e-9 eigenvectors (relative size ~1e-9) stall refinement through the
+    # perturbed FP32 factor (contraction 1 - 2e-9/1e-6 per step).  A random B
+    # instead has ||x|| ~ 1e9 ||b|| and no FP64 solve could reach 1e-10.
+    Xt = np.random.default_rng(1).uniform(-1, 1, (A.n, 2))
+    B = np.asfortranarray(A.matvec(Xt))
     X, info = cholmod.solve_refine(L, B, cm, tol=1e-10, maxiter=5)
     print(info, flush=True)
     assert info["fp64_fallback"]
