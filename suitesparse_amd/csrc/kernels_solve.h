@@ -7,7 +7,8 @@
 // steps of CHOLMOD/Cholesky/t_cholmod_psolve_worker.c:40-110.  Diagonal
 // blocks are applied through their inverses (kept from the factorization),
 // so every step is a small dense mat-vec; off-diagonal rows of a supernode are
-// split across workgroups.  X is column-major n x nrhs (ld n), permuted order.
+// split across workgroups, and one launch carries one 64-column step of every
+// supernode of a level.  X is column-major n x nrhs (ld n), permuted order.
 #pragma once
 #include "mxc_kernels.h"
 
@@ -36,177 +37,295 @@ __global__ void k_perm_scatter(const T *__restrict__ Y, const long long *__restr
     for (int r = 0; r < nrhs; r++) X[pk + r * n] = (double)Y[k + r * n];
 }
 
-// Stage the nb x nb inverse of block t (ld nb) into LDS Iv[64][65] with every
-// thread's loads in flight together (256 threads, 16 each).
-template <typename T>
-__device__ __forceinline__ void stage_inverse(const T *__restrict__ W, long long winv, int nb,
-                                              T (*Iv)[NB + 1], int tid) {
-    const int i = tid & (NB - 1), c0 = tid >> 6;
-    T v[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int c = c0 + 4 * q;
-        v[q] = (i < nb && c < nb) ? W[winv + i + (long long)c * nb] : (T)0;
-    }
-#pragma unroll
-    for (int q = 0; q < 16; q++) Iv[i][c0 + 4 * q] = v[q];
+// Value of v in lane addr / 4 (ds_bpermute, 32 bits at a time).
+__device__ __forceinline__ float lane_read(float v, int addr) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_read(double v, int addr) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)u);
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(u >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// forward, diagonal block: x_b = inv(L_bb) x_b
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_diag(const SolveTask *__restrict__ tasks,
-                                                  const T *__restrict__ W, T *__restrict__ X,
-                                                  long long n, int nrhs) {
-    __shared__ T Iv[NB][NB + 1];
-    __shared__ T xs[4][NB];
-    const SolveTask t = tasks[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
-    T *xb = X + t.k1 + t.c0;
-    stage_inverse(W, t.winv, t.nb, Iv, tid);
-    for (int r0 = 0; r0 < nrhs; r0 += 4) {
-        const int r = r0 + q;
-        if (r < nrhs) xs[q][lane] = (lane < t.nb) ? xb[r * n + lane] : (T)0;
-        __syncthreads();
-        if (r < nrhs && lane < t.nb) {
-            T s = 0;
-            for (int k = 0; k <= lane; k++) s += Iv[lane][k] * xs[q][k];
-            xb[r * n + lane] = s;
-        }
-        __syncthreads();
-    }
-}
-
-// forward, rows [r0, r1) below block t:  x[g(r)] -= L[r, c0:c0+nb] x_b
-// g(r) = k1 + r for the supernode's own columns (direct), Ls[psi + r] below
-// them (atomic: sibling supernodes of one level share ancestor rows).
-// One thread per row; the row's 16-column chunks are loaded together.
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_off(const SolveTask *__restrict__ tasks,
-                                                 const T *__restrict__ W,
-                                                 const int *__restrict__ Ls, T *__restrict__ X,
-                                                 long long n, int nrhs) {
-    __shared__ T xs[8][NB];
-    const SolveTask t = tasks[blockIdx.x];
-    const int tid = threadIdx.x;
-    const T *xb = X + t.k1 + t.c0;
-    const int r = t.r0 + tid;
-    const bool active = r < t.r1;
-    long long g = 0;
-    if (active) g = (r < t.nscol) ? (long long)(t.k1 + r) : (long long)Ls[t.psi + r];
-    const T *Lrow = W + t.lx + (long long)t.c0 * t.nsrow + r;
-    for (int rb = 0; rb < nrhs; rb += 8) {
-        const int nr = min(8, nrhs - rb);
-        for (int e = tid; e < 8 * NB; e += 256) {
-            const int z = e / NB, k = e % NB;
-            xs[z][k] = (z < nr && k < t.nb) ? xb[(rb + z) * n + k] : (T)0;
-        }
-        __syncthreads();
-        if (active) {
-            T acc[8];
+// One halving step: lanes with bit 4*HALF set keep the upper HALF values,
+// the others the lower HALF, and each adds its partner's copy of them.
+template <int HALF, typename T>
+__device__ __forceinline__ void fold_half(T (&v)[16], int lane) {
+    constexpr int MASK = 4 * HALF;
+    const bool up = (lane & MASK) != 0;
+    const int addr = (lane ^ MASK) << 2;
 #pragma unroll
-            for (int z = 0; z < 8; z++) acc[z] = 0;
-            for (int kc = 0; kc < t.nb; kc += 16) {
-                T l[16];
-#pragma unroll
-                for (int k = 0; k < 16; k++)
-                    l[k] = (kc + k < t.nb) ? Lrow[(long long)(kc + k) * t.nsrow] : (T)0;
-#pragma unroll
-                for (int k = 0; k < 16; k++)
-#pragma unroll
-                    for (int z = 0; z < 8; z++) acc[z] += l[k] * xs[z][kc + k];
-            }
-            for (int z = 0; z < nr; z++) {
-                if (r < t.nscol) X[g + (rb + z) * n] -= acc[z];
-                else atomic_add(X + g + (rb + z) * n, -acc[z]);
-            }
-        }
-        __syncthreads();
+    for (int i = 0; i < HALF; i++) {
+        const T keep = up ? v[i + HALF] : v[i];
+        const T send = up ? v[i] : v[i + HALF];
+        v[i] = keep + lane_read(send, addr);
     }
 }
 
 // Sum of 16 per-lane values over the 64 lanes of a wave, transposed: on
-// return v[0] in lane l is the total of index (l >> 2) & 15 (every lane of
-// a group of 4 holds the same total).  Halving butterfly: 17 shuffles.
+// return lane l holds the total of index (l >> 2) & 15 (every lane of a
+// group of 4 holds the same total).  Halving butterfly: 17 lane exchanges.
 template <typename T>
 __device__ __forceinline__ T wave_sum16(T (&v)[16], int lane) {
-#pragma unroll
-    for (int lvl = 0; lvl < 4; lvl++) {
-        const int half = 8 >> lvl;            // 8, 4, 2, 1
-        const int mask = 32 >> lvl;           // 32, 16, 8, 4
-        const bool up = (lane & mask) != 0;
-#pragma unroll
-        for (int i = 0; i < half; i++) {
-            const T keep = up ? v[i + half] : v[i];
-            const T send = up ? v[i] : v[i + half];
-            v[i] = keep + __shfl_xor(send, mask);
-        }
-    }
+    fold_half<8>(v, lane);
+    fold_half<4>(v, lane);
+    fold_half<2>(v, lane);
+    fold_half<1>(v, lane);
     T s = v[0];
-    s += __shfl_xor(s, 2);
-    s += __shfl_xor(s, 1);
+    s += lane_read(s, (lane ^ 2) << 2);
+    s += lane_read(s, (lane ^ 1) << 2);
     return s;
 }
 
-// backward, rows [r0, r1) below block t:  x_b -= L[r0:r1, c0:c0+nb]' x[g(r0:r1)]
-// One thread per row (coalesced reads of L along rows), per-wave transposed
-// butterfly reduction over 16-column chunks, atomic add into x_b.
+// Right-hand sides per pass of a sweep workgroup.
+constexpr int ZC = 8;
+
+// One launch = one 64-column step of every supernode of a level.  The
+// forward sweep of supernode s (t_cholmod_super_solve_worker.c:20-303: dtrsv
+// on the diagonal block, dgemv below it, scatter) is pipelined by one block:
+//   diag workgroups, block t:   x_t = inv(L_tt) (x_t - L[t, t-1] x_{t-1})
+//   off  workgroups, block t-1: x[g(r)] -= L[r, t-1] x_{t-1}  for the rows r
+//                               below block t (r >= nscol: ancestor rows, Ls)
+// Both read x_{t-1}, final after the previous launch; diag writes block t's
+// rows, off writes the rows below it, so no two workgroups of a launch touch
+// one row, except ancestor rows shared by supernodes of one level (atomics).
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_off(const SolveTask *__restrict__ tasks,
-                                                 const T *__restrict__ W,
-                                                 const int *__restrict__ Ls, T *__restrict__ X,
-                                                 long long n, int nrhs) {
-    const SolveTask t = tasks[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63;
+__global__ __launch_bounds__(256) void k_sweep_fwd(const SweepTask *__restrict__ tasks, int ndiag,
+                                                   const T *__restrict__ W,
+                                                   const int *__restrict__ Ls, T *__restrict__ X,
+                                                   long long n, int nrhs) {
+    __shared__ T xs[ZC][NB];
+    __shared__ T red[4][ZC][NB];
+    const SweepTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     T *xb = X + t.k1 + t.c0;
+    if ((int)blockIdx.x < ndiag) {
+        const int nb = t.nb, nbp = t.r0;
+        // lane i: row c0 + i of block t; wave w: columns 16w .. 16w+15
+        const T *Lb = W + t.lx + (long long)(t.c0 - NB) * t.nsrow + t.c0 + lane;
+        T lp[16], iv[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int k = 16 * w + j;
+            lp[j] = (lane < nb && k < nbp) ? Lb[(long long)k * t.nsrow] : (T)0;
+            iv[j] = (lane < nb && k < nb) ? W[t.winv + lane + (long long)k * nb] : (T)0;
+        }
+        for (int z0 = 0; z0 < nrhs; z0 += ZC) {
+            const int zc = min(ZC, nrhs - z0);
+            for (int e = tid; e < ZC * NB; e += 256) {
+                const int z = e >> 6, k = e & 63;
+                xs[z][k] = (z < zc && k < nbp) ? xb[(long long)(z0 + z) * n + k - NB] : (T)0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int z = 0; z < ZC; z++) {
+                T s = 0;
+#pragma unroll
+                for (int j = 0; j < 16; j++) s += lp[j] * xs[z][16 * w + j];
+                red[w][z][lane] = s;
+            }
+            __syncthreads();
+            for (int z = w; z < ZC; z += 4) {
+                T v = 0;
+                if (z < zc && lane < nb)
+                    v = xb[(long long)(z0 + z) * n + lane] -
+                        ((red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]));
+                xs[z][lane] = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int z = 0; z < ZC; z++) {
+                T s = 0;
+#pragma unroll
+                for (int j = 0; j < 16; j++) s += iv[j] * xs[z][16 * w + j];
+                red[w][z][lane] = s;
+            }
+            __syncthreads();
+            for (int z = w; z < zc; z += 4)
+                if (lane < nb)
+                    xb[(long long)(z0 + z) * n + lane] =
+                        (red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]);
+            __syncthreads();
+        }
+        return;
+    }
+    // off: one thread per row, the row's nb values loaded up front
+    const int nb = t.nb;
     const int r = t.r0 + tid;
     const bool active = r < t.r1;
     long long g = 0;
     if (active) g = (r < t.nscol) ? (long long)(t.k1 + r) : (long long)Ls[t.psi + r];
     const T *Lrow = W + t.lx + (long long)t.c0 * t.nsrow + r;
-    for (int kc = 0; kc < t.nb; kc += 16) {
-        T l[16];
+    T l[NB];
 #pragma unroll
-        for (int k = 0; k < 16; k++)
-            l[k] = (active && kc + k < t.nb) ? Lrow[(long long)(kc + k) * t.nsrow] : (T)0;
-        for (int z = 0; z < nrhs; z++) {
-            const T xr = active ? X[g + z * n] : (T)0;
-            T v[16];
-#pragma unroll
-            for (int k = 0; k < 16; k++) v[k] = l[k] * xr;
-            const T s = wave_sum16(v, lane);
-            const int k = kc + ((lane >> 2) & 15);
-            if ((lane & 3) == 0 && k < t.nb) atomic_add(xb + z * n + k, -s);
+    for (int k = 0; k < NB; k++) l[k] = (active && k < nb) ? Lrow[(long long)k * t.nsrow] : (T)0;
+    for (int z0 = 0; z0 < nrhs; z0 += ZC) {
+        const int zc = min(ZC, nrhs - z0);
+        for (int e = tid; e < ZC * NB; e += 256) {
+            const int z = e >> 6, k = e & 63;
+            xs[z][k] = (z < zc && k < nb) ? xb[(long long)(z0 + z) * n + k] : (T)0;
         }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int z = 0; z < ZC; z++) {
+                if (z < zc) {
+                    T s0 = 0, s1 = 0;
+#pragma unroll
+                    for (int k = 0; k < NB; k += 2) {
+                        s0 += l[k] * xs[z][k];
+                        s1 += l[k + 1] * xs[z][k + 1];
+                    }
+                    T *dst = X + g + (long long)(z0 + z) * n;
+                    if (r < t.nscol) *dst -= s0 + s1;
+                    else atomic_add(dst, -(s0 + s1));
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
-// backward, diagonal block: x_b = inv(L_bb)' x_b
+// Columns [c, c + 16) of one row of a supernode (zero beyond nb).
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_diag(const SolveTask *__restrict__ tasks,
-                                                  const T *__restrict__ W, T *__restrict__ X,
-                                                  long long n, int nrhs) {
-    __shared__ T Iv[NB][NB + 1];
-    __shared__ T xs[4][NB];
-    const SolveTask t = tasks[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+__device__ __forceinline__ void load_row16(T (&lc)[16], const T *Lrow, int c, int nb, int nsrow,
+                                           bool active) {
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+        lc[j] = (active && c + j < nb) ? Lrow[(long long)(c + j) * nsrow] : (T)0;
+}
+
+// dst[q] = sum over the wave's lanes of lc[q] * x  (q < 16, written by lanes 4q)
+template <typename T>
+__device__ __forceinline__ void col_sums16(const T (&lc)[16], T x, T *dst, int lane) {
+    T v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = lc[j] * x;
+    const T s = wave_sum16(v, lane);
+    if ((lane & 3) == 0) dst[(lane >> 2) & 15] = s;
+}
+
+// Backward sweep (t_cholmod_super_solve_worker.c:309-580: gather, dgemv with
+// L', dtrsv with L_tt'), pipelined the same way, steps in descending order:
+//   diag workgroups, block t:   x_t = inv(L_tt)' (x_t - sum of partials_t
+//                                                 - L[t+1, t]' x_{t+1})
+//   off  workgroups, block t-1: partial = L[r, t-1]' x[g(r)] over their rows
+//                               r below block t (one slot per workgroup)
+// Partials are written, not accumulated: no atomics, and the order of the
+// sums is fixed, so the sweep is deterministic.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_bwd(const SweepTask *__restrict__ tasks, int ndiag,
+                                                   const T *__restrict__ W,
+                                                   const int *__restrict__ Ls, T *__restrict__ X,
+                                                   T *__restrict__ part, long long n, int nrhs) {
+    __shared__ T xs[ZC][NB];
+    __shared__ T red[4][ZC][NB];
+    const SweepTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int qcol = (lane >> 2) & 15;          // column of a wave_sum16 total
     T *xb = X + t.k1 + t.c0;
-    stage_inverse(W, t.winv, t.nb, Iv, tid);
-    for (int r0 = 0; r0 < nrhs; r0 += 4) {
-        const int r = r0 + q;
-        if (r < nrhs) xs[q][lane] = (lane < t.nb) ? xb[r * n + lane] : (T)0;
-        __syncthreads();
-        if (r < nrhs && lane < t.nb) {
-            T s = 0;
-            for (int k = lane; k < t.nb; k++) s += Iv[k][lane] * xs[q][k];
-            xb[r * n + lane] = s;
+    if ((int)blockIdx.x < ndiag) {
+        const int nb = t.nb, nbn = t.r0;
+        // lane j: row c0 + 64 + j (block t+1) / row j of the inverse;
+        // wave w: columns 16w .. 16w+15 of block t
+        const T *Lb = W + t.lx + (long long)t.c0 * t.nsrow + t.c0 + NB + lane;
+        T ln[16], iv[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int k = 16 * w + j;
+            ln[j] = (lane < nbn && k < nb) ? Lb[(long long)k * t.nsrow] : (T)0;
+            iv[j] = (lane < nb && k < nb) ? W[t.winv + lane + (long long)k * nb] : (T)0;
         }
+        for (int z0 = 0; z0 < nrhs; z0 += ZC) {
+            const int zc = min(ZC, nrhs - z0);
+            for (int e = tid; e < ZC * NB; e += 256) {
+                const int z = e >> 6, k = e & 63;
+                xs[z][k] = (z < zc && k < nbn) ? xb[(long long)(z0 + z) * n + NB + k] : (T)0;
+            }
+            __syncthreads();
+            for (int z = 0; z < zc; z++) {
+                T v[16];
+                const T xj = xs[z][lane];
+#pragma unroll
+                for (int j = 0; j < 16; j++) v[j] = ln[j] * xj;
+                const T s = wave_sum16(v, lane);
+                if ((lane & 3) == 0) red[0][z][16 * w + qcol] = s;
+            }
+            __syncthreads();
+            for (int z = w; z < ZC; z += 4) {
+                T v = 0;
+                if (z < zc && lane < nb) {
+                    const T *pp = part + ((long long)t.p0 * nrhs + z0 + z) * NB + lane;
+                    const long long stride = (long long)nrhs * NB;
+                    T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+                    int p = 0;
+                    for (; p + 4 <= t.np; p += 4) {
+                        a0 += pp[(p + 0) * stride];
+                        a1 += pp[(p + 1) * stride];
+                        a2 += pp[(p + 2) * stride];
+                        a3 += pp[(p + 3) * stride];
+                    }
+                    for (; p < t.np; p++) a0 += pp[p * stride];
+                    v = xb[(long long)(z0 + z) * n + lane] - red[0][z][lane] - ((a0 + a1) + (a2 + a3));
+                }
+                xs[z][lane] = v;
+            }
+            __syncthreads();
+            for (int z = 0; z < zc; z++) {
+                T v[16];
+                const T xk = xs[z][lane];
+#pragma unroll
+                for (int j = 0; j < 16; j++) v[j] = iv[j] * xk;
+                const T s = wave_sum16(v, lane);
+                if ((lane & 3) == 0) red[0][z][16 * w + qcol] = s;
+            }
+            __syncthreads();
+            for (int z = w; z < zc; z += 4)
+                if (lane < nb) xb[(long long)(z0 + z) * n + lane] = red[0][z][lane];
+            __syncthreads();
+        }
+        return;
+    }
+    // off: one thread per row, transposed wave sums, one partial per workgroup.
+    // The row is held as four 16-column chunks so that every register index
+    // is a compile-time constant.
+    const int nb = t.nb;
+    const int r = t.r0 + tid;
+    const bool active = r < t.r1;
+    long long g = 0;
+    if (active) g = (r < t.nscol) ? (long long)(t.k1 + r) : (long long)Ls[t.psi + r];
+    const T *Lrow = W + t.lx + (long long)t.c0 * t.nsrow + r;
+    T l0[16], l1[16], l2[16], l3[16];
+    load_row16(l0, Lrow, 0, nb, t.nsrow, active);
+    load_row16(l1, Lrow, 16, nb, t.nsrow, active);
+    load_row16(l2, Lrow, 32, nb, t.nsrow, active);
+    load_row16(l3, Lrow, 48, nb, t.nsrow, active);
+    for (int z0 = 0; z0 < nrhs; z0 += ZC) {
+        const int zc = min(ZC, nrhs - z0);
+        T xr[ZC];
+#pragma unroll
+        for (int z = 0; z < ZC; z++) xr[z] = (active && z < zc) ? X[g + (long long)(z0 + z) * n] : (T)0;
+#pragma unroll
+        for (int z = 0; z < ZC; z++) {
+            if (z < zc) {
+                col_sums16(l0, xr[z], &red[w][z][0], lane);
+                col_sums16(l1, xr[z], &red[w][z][16], lane);
+                col_sums16(l2, xr[z], &red[w][z][32], lane);
+                col_sums16(l3, xr[z], &red[w][z][48], lane);
+            }
+        }
+        __syncthreads();
+        for (int z = w; z < zc; z += 4)
+            part[((long long)t.p0 * nrhs + z0 + z) * NB + lane] =
+                (red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]);
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
 // FP64 residual R = B - A X with A held as full CSR (both triangles), values
-// gathered from tril(A) by src.  One wave per 64 rows, one thread per row.
+// gathered from tril(A) by src.
 // ---------------------------------------------------------------------------
 __global__ void k_gather_vals(const double *__restrict__ Ax, const long long *__restrict__ src,
                               long long nnz, double *__restrict__ vals) {
@@ -215,18 +334,43 @@ __global__ void k_gather_vals(const double *__restrict__ Ax, const long long *__
     for (; p < nnz; p += stride) vals[p] = Ax[src[p]];
 }
 
-__global__ void k_residual(const long long *__restrict__ rowptr, const int *__restrict__ col,
-                           const double *__restrict__ vals, const double *__restrict__ B,
-                           const double *__restrict__ X, double *__restrict__ R, long long n,
-                           int nrhs) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const long long p0 = rowptr[i], p1 = rowptr[i + 1];
-    for (int r = 0; r < nrhs; r++) {
-        const double *x = X + r * n;
-        double s = 0;
-        for (long long p = p0; p < p1; p++) s += vals[p] * x[col[p]];
-        R[i + r * n] = B[i + r * n] - s;
+__global__ __launch_bounds__(256) void k_residual(const long long *__restrict__ rowptr,
+                                                  const int *__restrict__ col,
+                                                  const double *__restrict__ vals,
+                                                  const double *__restrict__ B,
+                                                  const double *__restrict__ X,
+                                                  double *__restrict__ R, long long n, int nrhs) {
+    // eight lanes per row: the row's values and columns are read coalesced,
+    // four right-hand sides per pass over the row
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const int sub = threadIdx.x & 7;
+    const bool ok = i < n;
+    long long p0 = 0, p1 = 0;
+    if (ok) {
+        p0 = rowptr[i];
+        p1 = rowptr[i + 1];
+    }
+    for (int z0 = 0; z0 < nrhs; z0 += 4) {
+        const int zc = min(4, nrhs - z0);
+        double s[4] = {0, 0, 0, 0};
+        for (long long p = p0 + sub; p < p1; p += 8) {
+            const double v = vals[p];
+            const double *x = X + col[p] + (long long)z0 * n;
+#pragma unroll
+            for (int z = 0; z < 4; z++)
+                if (z < zc) s[z] += v * x[(long long)z * n];
+        }
+#pragma unroll
+        for (int z = 0; z < 4; z++) {
+            double u = s[z];
+            u += __shfl_xor(u, 4);
+            u += __shfl_xor(u, 2);
+            u += __shfl_xor(u, 1);
+            if (ok && sub == 0 && z < zc) {
+                const long long q = i + (long long)(z0 + z) * n;
+                R[q] = B[q] - u;
+            }
+        }
     }
 }
 

@@ -92,7 +92,7 @@ struct Launch {
     int kind;          // see enum below
     long long begin;   // first task
     long long count;   // number of tasks (grid size)
-    int tile;          // tile edge of k_tile launches (32 or 64)
+    int tile;          // k_tile: tile edge; sweep steps: number of diag workgroups
 };
 
 // Tile-size buckets over (rows M, columns N, inner dimension K) of a block:
@@ -107,7 +107,7 @@ static inline int tile_bucket(int M, int N, int K) {
 static constexpr int NBUCKET = 75;
 static constexpr int NKIND = 3;     // 0 descendant update, 1 trailing, 2 TRSM
 static inline int kind_of(int launch_kind) { return launch_kind == 0 ? 0 : launch_kind == 3 ? 1 : 2; }
-enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FDIAG = 4, L_FOFF = 5, L_BOFF = 6, L_BDIAG = 7 };
+enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5 };
 
 struct StepBlocks {
     std::vector<DiagTask> diags;
@@ -135,7 +135,7 @@ struct mxc_factor {
     UpdDesc *d_desc = nullptr;
     Tile *d_tiles = nullptr;
     DiagTask *d_diag = nullptr;
-    SolveTask *d_stask = nullptr;
+    SweepTask *d_stask = nullptr;
     int *d_fail = nullptr;            // [0] first failing column, [1] perturbed pivots
     double *d_adiag = nullptr;        // assembled diagonal of A (n)
     long long *d_rowptr = nullptr, *d_src = nullptr;
@@ -144,6 +144,8 @@ struct mxc_factor {
     double *d_norm = nullptr;
     double *d_beta = nullptr;
     void *d_work = nullptr;             // T[n * work_nrhs] solve vector
+    void *d_part = nullptr;             // T[nslots * 64 * work_nrhs] backward partial sums
+    int64_t nslots = 0;
     double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr;   // FP64 n * work_nrhs
     int64_t work_nrhs = 0;
 
@@ -525,17 +527,19 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
         }
     }
 
-    // ---- solve tasks: forward (levels up, steps up), backward (levels down, steps down)
-    std::vector<SolveTask> st;
+    // ---- solve tasks.  Per level, launch t carries the diag workgroups of
+    // block t and the off workgroups of block t-1 of every supernode of the
+    // level (t = 0 .. max steps; forward ascending, backward descending).
+    std::vector<SweepTask> st;
     F->fwd_launches.clear();
     F->bwd_launches.clear();
     const int ROWS = 256;
-    auto make_task = [&](int64_t s, int64_t t, int64_t r0, int64_t r1) {
-        SolveTask x;
+    auto make_task = [&](int64_t s, int64_t blk) {
+        SweepTask x{};
         int64_t nscol = S->Super[s + 1] - S->Super[s];
         x.lx = S->Lpx[s];
         int64_t wv = winv_base[s];
-        for (int64_t cc = 0; cc < t * NB; cc += NB) {
+        for (int64_t cc = 0; cc < blk * NB; cc += NB) {
             int64_t b2 = std::min<int64_t>(NB, nscol - cc);
             wv += b2 * b2;
         }
@@ -544,55 +548,70 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
         x.nscol = (int)nscol;
         x.k1 = (int)S->Super[s];
         x.psi = (int)S->Lpi[s];
-        x.c0 = (int)(t * NB);
-        x.nb = (int)std::min<int64_t>(NB, nscol - t * NB);
-        x.r0 = (int)r0;
-        x.r1 = (int)r1;
+        x.c0 = (int)(blk * NB);
+        x.nb = (int)std::min<int64_t>(NB, nscol - blk * NB);
         return x;
+    };
+    auto nsteps = [&](int64_t s) { return ceil_div(S->Super[s + 1] - S->Super[s], NB); };
+    // off workgroups of block t-1 cover the rows below block t
+    auto push_off = [&](int64_t s, int64_t t, int *slot) {
+        int64_t nscol = S->Super[s + 1] - S->Super[s];
+        int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
+        for (int64_t r = std::min<int64_t>(nscol, (t + 1) * NB); r < nsrow; r += ROWS) {
+            SweepTask x = make_task(s, t - 1);
+            x.r0 = (int)r;
+            x.r1 = (int)std::min<int64_t>(nsrow, r + ROWS);
+            if (slot) x.p0 = (*slot)++;
+            st.push_back(x);
+        }
     };
     for (int l = 0; l < nlev; l++) {
         int64_t maxsteps = 0;
-        for (int64_t s : by_level[l])
-            maxsteps = std::max(maxsteps, ceil_div(S->Super[s + 1] - S->Super[s], NB));
-        for (int64_t t = 0; t < maxsteps; t++) {
+        for (int64_t s : by_level[l]) maxsteps = std::max(maxsteps, nsteps(s));
+        for (int64_t t = 0; t <= maxsteps; t++) {
             long long b = (long long)st.size();
             for (int64_t s : by_level[l])
-                if (t * NB < S->Super[s + 1] - S->Super[s]) st.push_back(make_task(s, t, 0, 0));
-            if ((long long)st.size() > b) F->fwd_launches.push_back(Launch{L_FDIAG, b, (long long)st.size() - b, 0});
-            b = (long long)st.size();
-            for (int64_t s : by_level[l]) {
-                int64_t nscol = S->Super[s + 1] - S->Super[s];
-                int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
-                if (t * NB >= nscol) continue;
-                int64_t r0 = std::min<int64_t>(nscol, t * NB + NB);
-                for (int64_t r = r0; r < nsrow; r += ROWS)
-                    st.push_back(make_task(s, t, r, std::min<int64_t>(nsrow, r + ROWS)));
-            }
-            if ((long long)st.size() > b) F->fwd_launches.push_back(Launch{L_FOFF, b, (long long)st.size() - b, 0});
+                if (t < nsteps(s)) {
+                    SweepTask x = make_task(s, t);
+                    x.r0 = t >= 1 ? NB : 0;
+                    st.push_back(x);
+                }
+            int ndiag = (int)((long long)st.size() - b);
+            for (int64_t s : by_level[l])
+                if (t >= 1 && t <= nsteps(s)) push_off(s, t, nullptr);
+            if ((long long)st.size() > b)
+                F->fwd_launches.push_back(Launch{L_FSTEP, b, (long long)st.size() - b, ndiag});
         }
     }
-    const int BROWS = 256;
+    int nslot = 0;
+    std::vector<int> pend_p0(ns, 0), pend_np(ns, 0);
     for (int l = nlev - 1; l >= 0; l--) {
         int64_t maxsteps = 0;
-        for (int64_t s : by_level[l])
-            maxsteps = std::max(maxsteps, ceil_div(S->Super[s + 1] - S->Super[s], NB));
-        for (int64_t t = maxsteps - 1; t >= 0; t--) {
+        for (int64_t s : by_level[l]) maxsteps = std::max(maxsteps, nsteps(s));
+        for (int64_t t = maxsteps; t >= 0; t--) {
             long long b = (long long)st.size();
-            for (int64_t s : by_level[l]) {
-                int64_t nscol = S->Super[s + 1] - S->Super[s];
-                int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
-                if (t * NB >= nscol) continue;
-                int64_t r0 = std::min<int64_t>(nscol, t * NB + NB);
-                for (int64_t r = r0; r < nsrow; r += BROWS)
-                    st.push_back(make_task(s, t, r, std::min<int64_t>(nsrow, r + BROWS)));
-            }
-            if ((long long)st.size() > b) F->bwd_launches.push_back(Launch{L_BOFF, b, (long long)st.size() - b, 0});
-            b = (long long)st.size();
             for (int64_t s : by_level[l])
-                if (t * NB < S->Super[s + 1] - S->Super[s]) st.push_back(make_task(s, t, 0, 0));
-            if ((long long)st.size() > b) F->bwd_launches.push_back(Launch{L_BDIAG, b, (long long)st.size() - b, 0});
+                if (t < nsteps(s)) {
+                    SweepTask x = make_task(s, t);
+                    int64_t nscol = S->Super[s + 1] - S->Super[s];
+                    x.r0 = (t + 1 < nsteps(s)) ? (int)std::min<int64_t>(NB, nscol - (t + 1) * NB) : 0;
+                    x.p0 = pend_p0[s];
+                    x.np = pend_np[s];
+                    st.push_back(x);
+                }
+            int ndiag = (int)((long long)st.size() - b);
+            for (int64_t s : by_level[l])
+                if (t >= 1 && t <= nsteps(s)) {
+                    int first = nslot;
+                    push_off(s, t, &nslot);
+                    pend_p0[s] = first;
+                    pend_np[s] = nslot - first;
+                }
+            if ((long long)st.size() > b)
+                F->bwd_launches.push_back(Launch{L_BSTEP, b, (long long)st.size() - b, ndiag});
         }
     }
+    F->nslots = nslot;
 
     // ---- device upload
     std::vector<int> Ls32(S->ssize);
@@ -636,11 +655,13 @@ static int ensure_work(mxc_factor *F, int64_t nrhs) {
     drop_sweep_graphs(F);
     size_t esz = F->precision == MXC_FP64 ? sizeof(double) : sizeof(float);
     if (F->d_work) { (void)hipFree(F->d_work); F->d_work = nullptr; }
+    if (F->d_part) { (void)hipFree(F->d_part); F->d_part = nullptr; }
     if (F->d_b) { (void)hipFree(F->d_b); F->d_b = nullptr; }
     if (F->d_x) { (void)hipFree(F->d_x); F->d_x = nullptr; }
     if (F->d_r) { (void)hipFree(F->d_r); F->d_r = nullptr; }
     size_t len = (size_t)std::max<int64_t>(F->n, 1) * (size_t)nrhs;
     HIPTRY(hipMalloc(&F->d_work, len * esz));
+    HIPTRY(hipMalloc(&F->d_part, (size_t)std::max<int64_t>(F->nslots, 1) * NB * (size_t)nrhs * esz));
     HIPTRY(hipMalloc((void **)&F->d_b, len * sizeof(double)));
     HIPTRY(hipMalloc((void **)&F->d_x, len * sizeof(double)));
     HIPTRY(hipMalloc((void **)&F->d_r, len * sizeof(double)));
@@ -655,7 +676,7 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
     if (F->fallback) mxc_factor_free(F->fallback);
     void *ptrs[] = {F->W, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
                     F->d_diag, F->d_stask, F->d_fail, F->d_adiag, F->d_rowptr, F->d_src, F->d_col,
-                    F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_b, F->d_x, F->d_r};
+                    F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_part, F->d_b, F->d_x, F->d_r};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
@@ -1030,12 +1051,8 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     if (fwd) {
         tick(F, 5, &tt);
         for (const Launch &L : F->fwd_launches) {
-            if (L.kind == L_FDIAG)
-                hipLaunchKernelGGL(k_fwd_diag<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
-                                   F->d_stask + L.begin, W, Xw, (long long)n, nr);
-            else
-                hipLaunchKernelGGL(k_fwd_off<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
-                                   F->d_stask + L.begin, W, F->d_Ls, Xw, (long long)n, nr);
+            hipLaunchKernelGGL(k_sweep_fwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
+                               F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (long long)n, nr);
             F->counts[4]++;
         }
         tock(F, 5, tt);
@@ -1043,12 +1060,9 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     if (bwd) {
         tick(F, 6, &tt);
         for (const Launch &L : F->bwd_launches) {
-            if (L.kind == L_BDIAG)
-                hipLaunchKernelGGL(k_bwd_diag<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
-                                   F->d_stask + L.begin, W, Xw, (long long)n, nr);
-            else
-                hipLaunchKernelGGL(k_bwd_off<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
-                                   F->d_stask + L.begin, W, F->d_Ls, Xw, (long long)n, nr);
+            hipLaunchKernelGGL(k_sweep_bwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
+                               F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (T *)F->d_part,
+                               (long long)n, nr);
             F->counts[4]++;
         }
         tock(F, 6, tt);
@@ -1183,7 +1197,7 @@ extern "C" int mxc_super_ltsolve(mxc_factor *F, double *X, int64_t nrhs, int loc
 // residual and iterative refinement
 // ---------------------------------------------------------------------------
 static void launch_residual(mxc_factor *F, const double *dB, const double *dX, double *dR, int64_t nrhs) {
-    hipLaunchKernelGGL(k_residual, dim3(grid_for(F->n, 256, 1LL << 30)), dim3(256), 0, F->stream,
+    hipLaunchKernelGGL(k_residual, dim3(grid_for(F->n * 8, 256, 1LL << 30)), dim3(256), 0, F->stream,
                        F->d_rowptr, F->d_col, F->d_vals, dB, dX, dR, (long long)F->n, (int)nrhs);
 }
 

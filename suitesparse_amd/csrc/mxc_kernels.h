@@ -52,13 +52,21 @@ struct DiagTask {
 };
 
 // Solve work item: column block (s, t) and a row range [r0, r1) of s.
-struct SolveTask {
+// One workgroup of a supernodal solve step (forward or backward sweep).
+//   diag: block t of supernode s, columns [c0, c0 + nb); r0 = width of the
+//         neighbouring block folded in (forward: block t-1, backward: the rows
+//         of block t+1; 0 = none); backward: reduces partial slots [p0, p0+np).
+//   off:  block columns [c0, c0 + nb) against rows [r0, r1) of the supernode
+//         (rows >= nscol map through Ls); backward: writes partial slot p0.
+struct SweepTask {
     long long lx;      // W offset of supernode s (Lpx[s])
-    long long winv;    // inverse of block t (ld nb)
+    long long winv;    // diag: inverse of the block (nb x nb, ld nb)
     int nsrow, nscol;
     int k1, psi;
     int c0, nb;
     int r0, r1;
+    int p0, np;
+    int pad[2];
 };
 
 enum { MODE_SCATTER = 0, MODE_TRAIL = 1, MODE_STORE = 2 };
