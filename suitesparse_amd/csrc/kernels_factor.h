@@ -262,6 +262,23 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
 // ---------------------------------------------------------------------------
 constexpr int DS = NB + 1;     // LDS row stride of the 64x64 block
 
+// 1/sqrt(x) for x > 0: the hardware estimate refined by two Newton steps
+// y <- y + y (1 - x y^2) / 2 (each roughly doubles the correct bits).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+        const double e = fma(-x * y, y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+__device__ __forceinline__ float rsqrt_nr(float x) {
+    float y = __builtin_amdgcn_rsqf(x);
+    const float e = fmaf(-x * y, y, 1.0f);
+    return fmaf(0.5f * y, e, y);
+}
+
 // value of v in lane k (k wave-uniform), through a scalar register
 __device__ __forceinline__ double lane_bcast(double v, int k) {
     int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
@@ -346,17 +363,23 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
 
     for (int p = 0; p < np; p++) {
         const int R = 16 * p;
-        // (1) POTRF + inverse of the 16x16 diagonal block, one wave, lane i < 16 owns row i
+        // (1) POTRF + inverse of the 16x16 diagonal block, one wave, lane i < 16 owns row i.
+        // Each pivot costs one reciprocal square root (no divide on the serial path):
+        // L_jj = d_jj r_j and L_ij = a_ij r_j with r_j = 1/sqrt(d_jj); the inverse
+        // reuses r_j = 1/L_jj.
         if (wave == 0) {
-            T a[16];
+            T a[16], rinv[16];
 #pragma unroll
             for (int c = 0; c < 16; c++) a[c] = (lane < 16) ? S[(R + lane) * DS + R + c] : (T)0;
+            // pivot floors of the panel's columns (low-precision safeguard), lane j holds column j's
+            T floor_l = 0;
+            if (rho > 0 && lane < 16 && R + lane < nb) floor_l = (T)(rho * adiag[t.col0 + R + lane]);
 #pragma unroll
             for (int j = 0; j < 16; j++) {
                 T djj = lane_bcast(a[j], j);
-                if (rho > 0 && R + j < nb) {
-                    const T floor_j = (T)(rho * adiag[t.col0 + R + j]);
-                    if (djj <= floor_j && floor_j > (T)0) {
+                if (rho > 0) {
+                    const T floor_j = lane_bcast(floor_l, j);
+                    if (R + j < nb && djj <= floor_j && floor_j > (T)0) {
                         djj = floor_j;
                         if (lane == 0) atomicAdd(npert, 1);
                     }
@@ -365,8 +388,9 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
                     if (lane == 0 && R + j < nb && sfail < 0) sfail = R + j;
                     djj = 1;
                 }
-                const T dj = sqrt(djj);
-                const T l = (lane > j) ? a[j] / dj : ((lane == j) ? dj : (T)0);
+                const T rj = rsqrt_nr(djj);
+                rinv[j] = rj;
+                const T l = (lane > j) ? a[j] * rj : ((lane == j) ? djj * rj : (T)0);
                 a[j] = l;
 #pragma unroll
                 for (int c = j + 1; c < 16; c++) a[c] -= l * lane_bcast(l, c);
@@ -386,7 +410,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
                 T sacc = (i == lane) ? (T)1 : (T)0;
 #pragma unroll
                 for (int k = 0; k < i; k++) sacc -= S[(R + i) * DS + R + k] * x[k];
-                x[i] = sacc / S[(R + i) * DS + R + i];
+                x[i] = sacc * rinv[i];
             }
             if (lane < 16) {
 #pragma unroll
