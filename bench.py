@@ -47,6 +47,83 @@ def relres(A, B, X) -> float:
 
 
 TUNING = ROOT / "suitesparse_amd" / "tuning"
+PROFILES = ROOT / "profiles"
+TRAFFIC_FILE = PROFILES / "traffic_update_diffusion3d_64_fp64.md"
+UPDATE_KERNEL = "k_tile<double, 0,"          # descendant update + scatter, FP64, every tile edge
+CALIB_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def calib_copy(mib: int):
+    """One device-to-device copy of exactly `mib` MiB (read mib, write mib): the
+    reference dispatch that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def _counter_rows(d: pathlib.Path, counter: str):
+    """(kernel name, value) per dispatch from a rocprofv3 --pmc output folder."""
+    import csv
+    files = sorted(d.rglob("*counter_collection.csv"))
+    if not files:
+        raise FileNotFoundError(f"no counter_collection.csv under {d}")
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    rows.append((r["Kernel_Name"], float(r["Counter_Value"]), int(r["Dispatch_Id"])))
+    return rows
+
+
+def traffic_summary(fetch_dir: str, write_dir: str, calib_mib: int, cmd: str) -> dict:
+    """HBM bytes per launch of the update kernel from two --pmc runs: the average
+    of FETCH_SIZE (WRITE_SIZE) over its dispatches, scaled by what the counter
+    reads for the calibration copy of calib_mib MiB (known: calib_mib * 1024 KiB
+    read and written)."""
+    res = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = _counter_rows(pathlib.Path(d), counter)
+        cal = max((v for k, v, _ in rows if k.startswith(CALIB_KERNEL)), default=0.0)
+        upd = [v for k, v, _ in rows if UPDATE_KERNEL in k]
+        if cal <= 0 or not upd:
+            raise RuntimeError(f"{counter}: calibration copy or update dispatches missing")
+        scale = calib_mib * 1024.0 / cal
+        res[counter] = {"calib_kib": cal, "scale": scale, "dispatches": len(upd),
+                        "avg_kib": sum(upd) / len(upd),
+                        "bytes_per_launch": sum(upd) / len(upd) * scale * 1024.0}
+    total = res["FETCH_SIZE"]["bytes_per_launch"] + res["WRITE_SIZE"]["bytes_per_launch"]
+    lines = ["# HBM traffic of the descendant-update kernel, 64^3 7-point diffusion, FP64", "",
+             "Two counter passes over the same command (repository root, one MI355X):", "", "```"]
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        lines.append(f"timeout -s KILL 180 rocprofv3 --pmc {counter} -d {d} -o run -- {cmd}")
+    lines += ["```", "",
+              f"Calibration: one device-to-device copy of {calib_mib} MiB ({CALIB_KERNEL}) in the same "
+              "process; scale = known KiB / counter KiB.", "",
+              "| counter | calibration copy (KiB) | scale | update dispatches | avg per dispatch (KiB) | bytes per launch (scaled) |",
+              "|---|---|---|---|---|---|"]
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        r = res[counter]
+        lines.append(f"| {counter} | {r['calib_kib']:.0f} | {r['scale']:.4f} | {r['dispatches']} | "
+                     f"{r['avg_kib']:.1f} | {r['bytes_per_launch']:.0f} |")
+    lines += ["", f"traffic_bytes_per_launch: {total:.0f}", ""]
+    TRAFFIC_FILE.write_text("\n".join(lines))
+    return {"traffic_bytes_per_launch": total, **res}
+
+
+def read_traffic():
+    """Bytes per update launch from the committed counter summary, or None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def tile_table(name: str, precision: str, mode: str, L):
@@ -180,7 +257,18 @@ def main():
                     help="skip the 64^3 headline (profiling of the 96^3 runs)")
     ap.add_argument("--tiles", choices=["saved", "auto", "default"], default="saved",
                     help="tile table: the saved autotuned one (default), retune, or built-in")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one device copy of this size first (counter calibration)")
+    ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="no GPU: turn two rocprofv3 --pmc outputs into the traffic summary")
+    ap.add_argument("--traffic-cmd", default="python3 bench.py --steps 1 --warmup 1 --no-cpu "
+                    "--no-mixed --calib-copy-mib 1024")
     args = ap.parse_args()
+    if args.traffic_summary:
+        r = traffic_summary(args.traffic_summary[0], args.traffic_summary[1],
+                            args.calib_copy_mib or 1024, args.traffic_cmd)
+        print(json.dumps(r))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(the factorization would shard along the elimination tree, see DESIGN.md)",
@@ -188,6 +276,9 @@ def main():
         sys.exit(2)
 
     from suitesparse_amd import problems
+
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib)
 
     if args.only_mixed:
         out = {"metric": "mixed-precision time-to-solution (96^3 27-pt)", "value": None}
@@ -223,7 +314,8 @@ def main():
         achieved = uf / (upd_ms / 1e3) / 1e12 if upd_ms > 0 else 0.0
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_PEAK_TFLOPS["f64"],
                 "unit": "TFLOP/s", "frac": round(achieved / MFMA_PEAK_TFLOPS["f64"], 4),
-                "traffic": None, "kernel": "k_tile<double,SCATTER> (descendant update + scatter)",
+                "traffic": read_traffic(),
+                "kernel": "k_tile<double,SCATTER> (descendant update + scatter)",
                 "launches_per_step": nl, "avg_launch_ms": round(upd_ms / max(nl, 1), 4),
                 "flops_per_step": uf}
     out = {

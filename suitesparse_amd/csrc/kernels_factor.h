@@ -325,11 +325,12 @@ __device__ __forceinline__ void store16(T *S, int r0, int c0, const typename Mfm
     }
 }
 
-// rho > 0 (low-precision factor): a pivot d_jj with d_jj <= rho * a_jj (a_jj the
-// assembled diagonal of A) signals breakdown; it is replaced by rho * a_jj and
-// counted in *npert, and the FP64 refinement (or the FP64 fallback factor)
-// repairs the perturbation.  rho = 0: exact semantics, a pivot <= 0 stops.
-template <typename T>
+// SAFE (rho > 0, low-precision factor): a pivot d_jj with d_jj <= rho * a_jj
+// (a_jj the assembled diagonal of A) signals breakdown; it is replaced by
+// rho * a_jj and counted in *npert, and the FP64 refinement (or the FP64
+// fallback factor) repairs the perturbation.  !SAFE: exact semantics, a pivot
+// <= 0 stops.
+template <typename T, bool SAFE>
 __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__ tasks,
                                                     T *__restrict__ W, int *__restrict__ fail,
                                                     const double *__restrict__ adiag, double rho,
@@ -373,27 +374,33 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
             for (int c = 0; c < 16; c++) a[c] = (lane < 16) ? S[(R + lane) * DS + R + c] : (T)0;
             // pivot floors of the panel's columns (low-precision safeguard), lane j holds column j's
             T floor_l = 0;
-            if (rho > 0 && lane < 16 && R + lane < nb) floor_l = (T)(rho * adiag[t.col0 + R + lane]);
+            if (SAFE && lane < 16 && R + lane < nb) floor_l = (T)(rho * adiag[t.col0 + R + lane]);
+            // wave-uniform bookkeeping, written once after the panel
+            int npj = 0;        // pivots raised to their floor
+            int badj = 16;      // first non-positive pivot of the panel
 #pragma unroll
             for (int j = 0; j < 16; j++) {
                 T djj = lane_bcast(a[j], j);
-                if (rho > 0) {
+                const bool real = R + j < nb;
+                if (SAFE) {
                     const T floor_j = lane_bcast(floor_l, j);
-                    if (R + j < nb && djj <= floor_j && floor_j > (T)0) {
-                        djj = floor_j;
-                        if (lane == 0) atomicAdd(npert, 1);
-                    }
+                    const bool low = real && djj <= floor_j && floor_j > (T)0;
+                    djj = low ? floor_j : djj;
+                    npj += low ? 1 : 0;
                 }
-                if (!(djj > (T)0)) {
-                    if (lane == 0 && R + j < nb && sfail < 0) sfail = R + j;
-                    djj = 1;
-                }
+                const bool bad = !(djj > (T)0);
+                badj = (bad && real && badj == 16) ? j : badj;
+                djj = bad ? (T)1 : djj;
                 const T rj = rsqrt_nr(djj);
                 rinv[j] = rj;
                 const T l = (lane > j) ? a[j] * rj : ((lane == j) ? djj * rj : (T)0);
                 a[j] = l;
 #pragma unroll
                 for (int c = j + 1; c < 16; c++) a[c] -= l * lane_bcast(l, c);
+            }
+            if (lane == 0) {
+                if (SAFE && npj > 0) atomicAdd(npert, npj);
+                if (badj < 16 && sfail < 0) sfail = R + badj;
             }
             if (lane < 16) {
 #pragma unroll

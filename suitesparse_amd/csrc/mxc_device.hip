@@ -849,9 +849,14 @@ static int enqueue_factor(mxc_factor *F) {
                 F->counts[0]++;
                 break;
             case L_DIAG:
-                hipLaunchKernelGGL(k_potrf_diag<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
-                                   F->d_diag + L.begin, W, F->d_fail, (const double *)F->d_adiag,
-                                   F->rho, F->d_fail + 1);
+                if (F->rho > 0)
+                    hipLaunchKernelGGL((k_potrf_diag<T, true>), dim3((unsigned)L.count), dim3(256), 0,
+                                       sm, F->d_diag + L.begin, W, F->d_fail,
+                                       (const double *)F->d_adiag, F->rho, F->d_fail + 1);
+                else
+                    hipLaunchKernelGGL((k_potrf_diag<T, false>), dim3((unsigned)L.count), dim3(256), 0,
+                                       sm, F->d_diag + L.begin, W, F->d_fail,
+                                       (const double *)F->d_adiag, F->rho, F->d_fail + 1);
                 F->counts[1]++;
                 break;
             case L_TRSM:
