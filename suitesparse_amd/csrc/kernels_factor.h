@@ -122,6 +122,19 @@ __global__ void k_diag_minmax(const T *__restrict__ W, const long long *__restri
 // and Y (C-row side) as its B operand, so 16 consecutive lanes hold 16
 // consecutive C rows: stores into the column-major target are contiguous runs.
 // ---------------------------------------------------------------------------
+// Workgroup b of a launch is dispatched to XCD b % 8.  Tiles are dealt to
+// the XCDs in chunks of 8 consecutive list entries instead (round robin over
+// chunks), so the tiles of a chunk, which share an operand strip, run on one
+// XCD and share its L2, while the chunks still spread evenly over the XCDs.
+// The tail that does not fill a round of 8 chunks keeps its own order.
+__device__ __forceinline__ int xcd_tile(int b, int g) {
+    constexpr int C = 8;
+    const int full = g / (8 * C) * (8 * C);
+    if (b >= full) return b;
+    const int x = b & 7, k = b >> 3;
+    return (x + 8 * (k / C)) * C + (k % C);
+}
+
 template <typename T, int MODE, int TS>
 __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile(const UpdDesc *__restrict__ descs,
                                                               const Tile *__restrict__ tiles,
@@ -138,7 +151,7 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
     __shared__ int relr[TS];
     __shared__ int relc[TS];
 
-    const Tile tl = tiles[blockIdx.x];
+    const Tile tl = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
     const UpdDesc d = descs[tl.desc];
     const int i0 = tl.ti * TS, j0 = tl.tj * TS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;

@@ -273,7 +273,7 @@ static int flatten_upload(mxc_factor *F) {
         }
     }
     F->nupd_tiles = nupd;
-    if (descs.size() >= (size_t)INT32_MAX || tiles.size() >= (size_t)UINT32_MAX) return MXC_TOO_LARGE;
+    if (descs.size() >= (size_t)INT32_MAX || tiles.size() >= (size_t)INT32_MAX) return MXC_TOO_LARGE;
     drop_factor_graph(F);
     if (F->d_desc) (void)hipFree(F->d_desc);
     if (F->d_tiles) (void)hipFree(F->d_tiles);
