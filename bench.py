@@ -257,6 +257,13 @@ def main():
                     help="skip the 64^3 headline (profiling of the 96^3 runs)")
     ap.add_argument("--tiles", choices=["saved", "auto", "default"], default="saved",
                     help="tile table: the saved autotuned one (default), retune, or built-in")
+    ap.add_argument("--batch", action="store_true",
+                    help="also run the batched workload (4,096 2D systems, FP64 and FP32+IR)")
+    ap.add_argument("--serving", type=int, default=0, metavar="N",
+                    help="also run a serving stream of N requests (FP32+IR)")
+    ap.add_argument("--serving-rate", type=float, default=100.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--serving-slots", type=int, default=8)
+    ap.add_argument("--serving-budget-gib", type=float, default=64.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this size first (counter calibration)")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -350,6 +357,10 @@ def main():
 
     if not args.no_mixed:
         mixed_section(args, out)
+    if args.batch:
+        batch_section(args, out)
+    if args.serving:
+        serving_section(args, out)
     print(json.dumps(out))
 
 
@@ -384,6 +395,94 @@ def mixed_section(args, out):
         mix["speedup_fp32ir_vs_fp64"] = round(mix["fp64"]["time_to_solution_ms"] /
                                               mix["fp32"]["time_to_solution_ms"], 3)
         out["mixed_precision"] = mix
+
+
+def batch_section(args, out):
+    """4,096 independent 2D 5-point systems (side U{16..128}, log-normal weights,
+    AMD), 4 RHS each, factored and solved as one block-diagonal forest: FP64,
+    and FP32 + per-system FP64 refinement to 1e-10."""
+    import gc
+    import torch
+    from suitesparse_amd import batch, cholmod, problems
+    log("[bench] batched: generating 4096 2D systems")
+    sides, rng = problems.batch_sides(4096)
+    mats = [problems.grid2d_lognormal(int(s), rng) for s in sides]
+    nt = sum(m.n for m in mats)
+    B = np.asfortranarray(rng.uniform(-1.0, 1.0, (nt, 4)))
+    t0 = time.perf_counter()
+    cm = cholmod.Common(precision="fp64")
+    b = batch.analyze_batch(mats, cm)
+    t_an = time.perf_counter() - t0
+    assert cm.start()
+    L = b.L
+    dev = torch.device("cuda:0")
+    Ax = torch.as_tensor(b.A.x, device=dev)
+    Bd = torch.as_tensor(B, device=dev).t().contiguous().t()
+    seg = b.offsets
+    res = {"workload": "batch_4096_grid2d_side16..128_amd_4rhs", "systems": len(mats), "n_total": nt,
+           "fl": L.fl, "lnz": L.lnz, "t_analyze_s": round(t_an, 2)}
+    steps = max(1, args.steps)
+    for prec in ("fp64", "fp32"):
+        cmp = cholmod.Common(precision=prec)
+        cmp.start()
+        t0 = time.perf_counter()
+        L._numeric(cmp)
+        t_plan = time.perf_counter() - t0
+
+        def step():
+            ok = cholmod.factorize(b.A, L, cmp, values=Ax)
+            assert ok and cmp.status == 0, cmp.status
+            if prec == "fp32":
+                return cholmod.solve_refine(L, Bd, cmp, tol=1e-10, maxiter=20, segments=seg)
+            return cholmod.solve(cholmod.CHOLMOD_A, L, Bd, cmp), None
+
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            X, info = step()
+        torch.cuda.synchronize()
+        tt = (time.perf_counter() - t0) / steps
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        cholmod.factorize(b.A, L, cmp, values=Ax)
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t1
+        Xh = X.t().contiguous().t().cpu().numpy()
+        R = B - b.A.matvec(Xh)
+        rn = np.sqrt(np.add.reduceat(R * R, seg[:-1], axis=0))
+        bn = np.sqrt(np.add.reduceat(B * B, seg[:-1], axis=0))
+        rel = float((rn / bn).max())
+        res[prec] = {"time_ms": round(tt * 1e3, 2), "t_factor_ms": round(tf * 1e3, 2),
+                     "t_solve_ms": round((tt - tf) * 1e3, 2), "t_plan_s": round(t_plan, 2),
+                     "systems_per_s": round(len(mats) / tt, 1),
+                     "factor_gflops": round(L.fl / tf / 1e9, 1),
+                     "max_relres_per_system": rel, "refine": info}
+        log(f"[bench]   batch {prec}: {res[prec]}")
+        del X, Xh, R
+        gc.collect()
+    res["speedup_fp32ir_vs_fp64"] = round(res["fp64"]["time_ms"] / res["fp32"]["time_ms"], 3)
+    out["batched"] = res
+
+
+def serving_section(args, out):
+    """Poisson stream of requests (2D 5-point side 32..512 / 3D 7-point side
+    16..64, log-normal weights, 30% same-pattern refactorizations, 1..8 RHS),
+    FP32 + FP64 refinement to 1e-10, admission by device bytes against a
+    fixed HBM budget; latency is arrival -> result on the device."""
+    from suitesparse_amd import serving
+    n = args.serving
+    log(f"[bench] serving: {n} requests at {args.serving_rate}/s, {args.serving_slots} slots")
+    reqs = serving.request_specs(n)
+    arrivals = serving.poisson_arrivals(n, args.serving_rate)
+    srv = serving.Server(slots=args.serving_slots, hbm_budget=int(args.serving_budget_gib * (1 << 30)),
+                         precision="fp32", tol=1e-10, keep_results=False)
+    st = serving.run_stream(srv, reqs, arrivals)
+    st.update({"workload": f"serving_{n}req_poisson{args.serving_rate:g}_2d32..512_3d16..64_amd_fp32ir",
+               "rate_per_s": args.serving_rate, "slots": args.serving_slots,
+               "hbm_budget_gib": args.serving_budget_gib})
+    log(f"[bench]   serving: {st}")
+    out["serving"] = st
 
 
 if __name__ == "__main__":

@@ -191,10 +191,7 @@ def revalue(A: SymMatrix, rng: np.random.Generator, sigma: float = 0.5,
     x = np.empty_like(A.x)
     w = np.exp(sigma * rng.standard_normal(int(off.sum())))
     x[off] = -w
-    diag = np.full(n, shift)
-    np.add.at(diag, A.i[off], w)
-    np.add.at(diag, col[off], w)
-    x[~off] = diag
+    x[~off] = shift + np.bincount(A.i[off], w, n) + np.bincount(col[off], w, n)
     return SymMatrix(n=n, p=A.p, i=A.i, x=x, name=A.name + "_rev")
 
 

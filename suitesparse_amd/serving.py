@@ -41,6 +41,8 @@ class Request:
     B: np.ndarray                  # n x nrhs
     key: str                       # pattern identity (same key => same pattern)
     arrival: float = 0.0           # seconds from the start of the stream
+    spec: tuple | None = None      # (kind, side, nrhs): drawn when submitted (request_specs)
+    nrhs: int = 0                  # kept when the matrices are dropped after completion
     # results
     X: np.ndarray | None = None
     info: dict | None = None
@@ -71,8 +73,9 @@ def plan_bytes_estimate(xsize: int, nsuper_cols_sq: int, nnz: int, n: int, nrhs:
 
 class Server:
     def __init__(self, slots: int = 8, hbm_budget: int = 64 << 30, precision: str = "fp32",
-                 tol: float = 1e-10, maxiter: int = 20):
+                 tol: float = 1e-10, maxiter: int = 20, keep_results: bool = True):
         self.slots = slots
+        self.keep_results = keep_results   # False: drop A, B and X once a request is done
         self.budget = int(hbm_budget)
         self.precision = precision
         self.tol = tol
@@ -200,6 +203,9 @@ class Server:
                 req.status = -5
                 req.info = {"error": repr(e)}
             req.done = self.now()
+            req.nrhs = req.B.shape[1] if req.B is not None else 0
+            if not self.keep_results:
+                req.A = req.B = req.X = None
             with self._cv:
                 if plan is not None:
                     plan.users -= 1
@@ -271,27 +277,120 @@ def poisson_arrivals(count: int, rate: float, seed: int = 4) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate, size=count))
 
 
-def run_stream(server: Server, reqs, arrivals) -> dict:
-    """Feed requests at their arrival times; return latency / throughput stats."""
+class PatternBank:
+    """One matrix per (kind, side) pattern; each request redraws the values
+    (log-normal edge weights, diagonal = row sum + shift) on that pattern.
+    The index arrays of a pattern are built once."""
+
+    def __init__(self, sigma: float = 0.5, shift: float = 1e-3):
+        self.sigma, self.shift = sigma, shift
+        self._pats = {}
+        self._lock = threading.Lock()
+
+    def _pattern(self, kind: str, side: int):
+        key = (kind, side)
+        with self._lock:
+            return self._pattern_locked(key, kind, side)
+
+    def _pattern_locked(self, key, kind: str, side: int):
+        if key not in self._pats:
+            from . import problems
+            gen = problems.grid2d_lognormal if kind == "2d" else problems.grid3d_lognormal
+            A = gen(side, np.random.default_rng(0), sigma=self.sigma, shift=self.shift)
+            col = np.repeat(np.arange(A.n), np.diff(A.p))
+            off = np.flatnonzero(A.i != col)
+            dpos = np.flatnonzero(A.i == col)
+            self._pats[key] = (A, off, dpos, A.i[off], col[off])
+        return self._pats[key]
+
+    def draw(self, kind: str, side: int, rng: np.random.Generator) -> SymMatrix:
+        A, off, dpos, ri, ci = self._pattern(kind, side)
+        w = np.exp(self.sigma * rng.standard_normal(off.size))
+        x = np.empty(A.x.size)
+        x[off] = -w
+        x[dpos] = self.shift + np.bincount(ri, w, A.n) + np.bincount(ci, w, A.n)
+        return SymMatrix(n=A.n, p=A.p, i=A.i, x=x, name=A.name)
+
+
+def request_specs(count: int, seed: int = 4, side2d=(32, 512), side3d=(16, 64),
+                  refactor_frac: float = 0.3, nrhs=(1, 8)):
+    """The serving workload without its matrices: half 2D 5-point (side
+    U{32..512}), half 3D 7-point (side U{16..64}); a refactor_frac share
+    reuses the pattern of an earlier request; nrhs U{1..8}.  Values are drawn
+    when the request is submitted (materialize)."""
+    rng = np.random.default_rng(seed)
+    reqs, seen = [], []
+    for rid in range(count):
+        if seen and rng.random() < refactor_frac:
+            kind, side = seen[rng.integers(len(seen))]
+        elif rng.random() < 0.5:
+            kind, side = "2d", int(rng.integers(side2d[0], side2d[1] + 1))
+        else:
+            kind, side = "3d", int(rng.integers(side3d[0], side3d[1] + 1))
+        seen.append((kind, side))
+        k = int(rng.integers(nrhs[0], nrhs[1] + 1))
+        reqs.append(Request(rid=rid, A=None, B=None, key=f"{kind}{side}", spec=(kind, side, k)))
+    return reqs
+
+
+def materialize(req: Request, bank: PatternBank, seed: int = 4):
+    """Draw the request's values and right-hand sides (per-request seed)."""
+    kind, side, k = req.spec
+    rng = np.random.default_rng([seed, req.rid])
+    req.A = bank.draw(kind, side, rng)
+    req.B = np.asfortranarray(rng.uniform(-1.0, 1.0, (req.A.n, k)))
+
+
+def run_stream(server: Server, reqs, arrivals, bank: PatternBank | None = None,
+               lookahead: int = 64, producers: int = 8) -> dict:
+    """Feed requests at their arrival times; return latency / throughput stats.
+    Requests built by request_specs are materialized ahead of their arrival
+    by a pool of producer threads, in arrival order, at most `lookahead` ahead."""
+    import concurrent.futures
+    import queue
+    ready: "queue.Queue" = queue.Queue(maxsize=lookahead)
+    bank = bank or PatternBank()
+
+    def one(req):
+        if req.A is None:
+            materialize(req, bank)
+        return req
+
+    def produce():
+        with concurrent.futures.ThreadPoolExecutor(max_workers=producers) as ex:
+            for c0 in range(0, len(reqs), 16):
+                for req in ex.map(one, reqs[c0:c0 + 16]):
+                    ready.put(req)
+
+    prod = threading.Thread(target=produce, daemon=True)
+    prod.start()
     server.start()
-    for req, t in zip(reqs, arrivals):
+    late = 0
+    for t in arrivals:
+        req = ready.get()
         req.arrival = float(t)
         delay = req.arrival - server.now()
         if delay > 0:
             time.sleep(delay)
+        elif delay < -0.005:
+            late += 1                     # producer behind the arrival clock
         server.submit(req)
+    prod.join()
     server.drain()
     done = sorted(server.completed, key=lambda r: r.rid)
     lat = np.array([r.done - r.arrival for r in done]) * 1e3
     span = max(r.done for r in done) - min(r.arrival for r in done)
     ok = [r for r in done if r.status == 0]
+    nr = [r.nrhs for r in ok]
     return {
         "requests": len(done), "ok": len(ok),
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "mean_ms": float(lat.mean()), "solves_per_s": len(ok) / span,
-        "rhs_per_s": sum(r.B.shape[1] for r in ok) / span,
+        "rhs_per_s": sum(nr) / span,
         "plan_reuse": sum(r.reused_plan for r in done) / max(1, len(done)),
         "evictions": server.evictions,
         "max_relres": max((r.info or {}).get("relres", 0.0) for r in ok) if ok else None,
+        "fp64_fallbacks": sum(bool((r.info or {}).get("fp64_fallback")) for r in ok),
+        "late_submissions": late,
         "wall_s": span,
     }
