@@ -324,6 +324,154 @@ __global__ __launch_bounds__(256) void k_sweep_bwd(const SweepTask *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Small supernodes (nscol <= 64 and at most SMALL_ROWS rows below): one wave
+// solves one supernode completely (diagonal block through its inverse, then
+// the rows below), four supernodes per workgroup, so the wide bottom levels
+// of a forest need one workgroup per four supernodes and one launch per level.
+// Waves share no data; the kernels have no workgroup barrier.
+// ---------------------------------------------------------------------------
+constexpr int SMALL_ROWS = 128;
+constexpr int ZS = 4;          // right-hand sides per pass
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_small_fwd(const SweepTask *__restrict__ tasks, int ntask,
+                                                         const T *__restrict__ W,
+                                                         const int *__restrict__ Ls, T *__restrict__ X,
+                                                         long long n, int nrhs) {
+    __shared__ T xs[4][ZS][NB];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ti = blockIdx.x * 4 + w;
+    if (ti >= ntask) return;
+    const SweepTask t = tasks[ti];
+    const int nb = t.nb;
+    T *xb = X + t.k1;
+    for (int z0 = 0; z0 < nrhs; z0 += ZS) {
+        const int zc = min(ZS, nrhs - z0);
+#pragma unroll
+        for (int z = 0; z < ZS; z++)
+            xs[w][z][lane] = (z < zc && lane < nb) ? xb[(long long)(z0 + z) * n + lane] : (T)0;
+        wave_sync_lds();
+        // y = inv(L_bb) x_b: lane i forms row i
+        T y[ZS] = {};
+        for (int kc = 0; kc < nb; kc += 16) {
+            T iv[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                iv[j] = (lane < nb && kc + j <= lane) ? W[t.winv + lane + (long long)(kc + j) * nb] : (T)0;
+#pragma unroll
+            for (int z = 0; z < ZS; z++)
+#pragma unroll
+                for (int j = 0; j < 16; j++) y[z] += iv[j] * xs[w][z][(kc + j) & 63];
+        }
+        wave_sync_lds();
+#pragma unroll
+        for (int z = 0; z < ZS; z++) {
+            if (z < zc && lane < nb) xb[(long long)(z0 + z) * n + lane] = y[z];
+            xs[w][z][lane] = y[z];
+        }
+        wave_sync_lds();
+        // rows below (ancestor rows, shared with other supernodes: atomics)
+        for (int r0 = nb; r0 < t.nsrow; r0 += 64) {
+            const int r = r0 + lane;
+            const bool active = r < t.nsrow;
+            const T *Lrow = W + t.lx + r;
+            T acc[ZS] = {};
+            for (int kc = 0; kc < nb; kc += 16) {
+                T l[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+                    l[j] = (active && kc + j < nb) ? Lrow[(long long)(kc + j) * t.nsrow] : (T)0;
+#pragma unroll
+                for (int z = 0; z < ZS; z++)
+#pragma unroll
+                    for (int j = 0; j < 16; j++) acc[z] += l[j] * xs[w][z][(kc + j) & 63];
+            }
+            if (active) {
+                const long long g = Ls[t.psi + r];
+#pragma unroll
+                for (int z = 0; z < ZS; z++)
+                    if (z < zc) atomic_add(X + g + (long long)(z0 + z) * n, -acc[z]);
+            }
+        }
+        wave_sync_lds();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_small_bwd(const SweepTask *__restrict__ tasks, int ntask,
+                                                         const T *__restrict__ W,
+                                                         const int *__restrict__ Ls, T *__restrict__ X,
+                                                         long long n, int nrhs) {
+    __shared__ T ps[4][ZS][NB];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ti = blockIdx.x * 4 + w;
+    if (ti >= ntask) return;
+    const SweepTask t = tasks[ti];
+    const int nb = t.nb;
+    const int qcol = (lane >> 2) & 15;
+    T *xb = X + t.k1;
+    for (int z0 = 0; z0 < nrhs; z0 += ZS) {
+        const int zc = min(ZS, nrhs - z0);
+        // p_k = sum over the rows below of L[r, k] x[g(r)], 16 columns at a time
+        for (int kc = 0; kc < nb; kc += 16) {
+            T v[ZS][16] = {};
+            for (int r0 = nb; r0 < t.nsrow; r0 += 64) {
+                const int r = r0 + lane;
+                const bool active = r < t.nsrow;
+                const T *Lrow = W + t.lx + r;
+                T xr[ZS];
+                const long long g = active ? (long long)Ls[t.psi + r] : 0;
+#pragma unroll
+                for (int z = 0; z < ZS; z++)
+                    xr[z] = (active && z < zc) ? X[g + (long long)(z0 + z) * n] : (T)0;
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const T l = (active && kc + j < nb) ? Lrow[(long long)(kc + j) * t.nsrow] : (T)0;
+#pragma unroll
+                    for (int z = 0; z < ZS; z++) v[z][j] += l * xr[z];
+                }
+            }
+#pragma unroll
+            for (int z = 0; z < ZS; z++) {
+                const T s = wave_sum16(v[z], lane);
+                if ((lane & 3) == 0) ps[w][z][(kc + qcol) & 63] = s;
+            }
+        }
+        wave_sync_lds();
+        // v = x_b - p, then x_b = inv(L_bb)' v: lane k forms entry k
+#pragma unroll
+        for (int z = 0; z < ZS; z++) {
+            const T xv = (z < zc && lane < nb) ? xb[(long long)(z0 + z) * n + lane] : (T)0;
+            const T pv = (lane < nb && nb < t.nsrow) ? ps[w][z][lane] : (T)0;
+            ps[w][z][lane] = xv - pv;
+        }
+        wave_sync_lds();
+        T y[ZS] = {};
+        for (int ic = 0; ic < nb; ic += 16) {
+            T iv[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                iv[j] = (lane < nb && ic + j < nb && ic + j >= lane) ? W[t.winv + (ic + j) + (long long)lane * nb]
+                                                                     : (T)0;
+#pragma unroll
+            for (int z = 0; z < ZS; z++)
+#pragma unroll
+                for (int j = 0; j < 16; j++) y[z] += iv[j] * ps[w][z][(ic + j) & 63];
+        }
+#pragma unroll
+        for (int z = 0; z < ZS; z++)
+            if (z < zc && lane < nb) xb[(long long)(z0 + z) * n + lane] = y[z];
+        wave_sync_lds();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 residual R = B - A X with A held as full CSR (both triangles), values
 // gathered from tril(A) by src.
 // ---------------------------------------------------------------------------

@@ -107,7 +107,7 @@ static inline int tile_bucket(int M, int N, int K) {
 static constexpr int NBUCKET = 75;
 static constexpr int NKIND = 3;     // 0 descendant update, 1 trailing, 2 TRSM
 static inline int kind_of(int launch_kind) { return launch_kind == 0 ? 0 : launch_kind == 3 ? 1 : 2; }
-enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5 };
+enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7 };
 
 struct StepBlocks {
     std::vector<DiagTask> diags;
@@ -565,20 +565,34 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
             st.push_back(x);
         }
     };
+    // small supernodes: one wave solves the whole supernode (one launch per level)
+    auto is_small = [&](int64_t s) {
+        int64_t nscol = S->Super[s + 1] - S->Super[s];
+        int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
+        return nscol <= NB && nsrow - nscol <= SMALL_ROWS;
+    };
+    auto push_small = [&](int l, std::vector<Launch> &out, int kind) {
+        long long b = (long long)st.size();
+        for (int64_t s : by_level[l])
+            if (is_small(s)) st.push_back(make_task(s, 0));
+        if ((long long)st.size() > b) out.push_back(Launch{kind, b, (long long)st.size() - b, 0});
+    };
     for (int l = 0; l < nlev; l++) {
+        push_small(l, F->fwd_launches, L_FSMALL);
         int64_t maxsteps = 0;
-        for (int64_t s : by_level[l]) maxsteps = std::max(maxsteps, nsteps(s));
+        for (int64_t s : by_level[l])
+            if (!is_small(s)) maxsteps = std::max(maxsteps, nsteps(s));
         for (int64_t t = 0; t <= maxsteps; t++) {
             long long b = (long long)st.size();
             for (int64_t s : by_level[l])
-                if (t < nsteps(s)) {
+                if (!is_small(s) && t < nsteps(s)) {
                     SweepTask x = make_task(s, t);
                     x.r0 = t >= 1 ? NB : 0;
                     st.push_back(x);
                 }
             int ndiag = (int)((long long)st.size() - b);
             for (int64_t s : by_level[l])
-                if (t >= 1 && t <= nsteps(s)) push_off(s, t, nullptr);
+                if (!is_small(s) && t >= 1 && t <= nsteps(s)) push_off(s, t, nullptr);
             if ((long long)st.size() > b)
                 F->fwd_launches.push_back(Launch{L_FSTEP, b, (long long)st.size() - b, ndiag});
         }
@@ -586,12 +600,14 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     int nslot = 0;
     std::vector<int> pend_p0(ns, 0), pend_np(ns, 0);
     for (int l = nlev - 1; l >= 0; l--) {
+        push_small(l, F->bwd_launches, L_BSMALL);
         int64_t maxsteps = 0;
-        for (int64_t s : by_level[l]) maxsteps = std::max(maxsteps, nsteps(s));
+        for (int64_t s : by_level[l])
+            if (!is_small(s)) maxsteps = std::max(maxsteps, nsteps(s));
         for (int64_t t = maxsteps; t >= 0; t--) {
             long long b = (long long)st.size();
             for (int64_t s : by_level[l])
-                if (t < nsteps(s)) {
+                if (!is_small(s) && t < nsteps(s)) {
                     SweepTask x = make_task(s, t);
                     int64_t nscol = S->Super[s + 1] - S->Super[s];
                     x.r0 = (t + 1 < nsteps(s)) ? (int)std::min<int64_t>(NB, nscol - (t + 1) * NB) : 0;
@@ -601,7 +617,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                 }
             int ndiag = (int)((long long)st.size() - b);
             for (int64_t s : by_level[l])
-                if (t >= 1 && t <= nsteps(s)) {
+                if (!is_small(s) && t >= 1 && t <= nsteps(s)) {
                     int first = nslot;
                     push_off(s, t, &nslot);
                     pend_p0[s] = first;
@@ -1056,8 +1072,12 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     if (fwd) {
         tick(F, 5, &tt);
         for (const Launch &L : F->fwd_launches) {
-            hipLaunchKernelGGL(k_sweep_fwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
-                               F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (long long)n, nr);
+            if (L.kind == L_FSMALL)
+                hipLaunchKernelGGL(k_sweep_small_fwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,
+                                   sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xw, (long long)n, nr);
+            else
+                hipLaunchKernelGGL(k_sweep_fwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
+                                   F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (long long)n, nr);
             F->counts[4]++;
         }
         tock(F, 5, tt);
@@ -1065,9 +1085,13 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     if (bwd) {
         tick(F, 6, &tt);
         for (const Launch &L : F->bwd_launches) {
-            hipLaunchKernelGGL(k_sweep_bwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
-                               F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (T *)F->d_part,
-                               (long long)n, nr);
+            if (L.kind == L_BSMALL)
+                hipLaunchKernelGGL(k_sweep_small_bwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,
+                                   sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xw, (long long)n, nr);
+            else
+                hipLaunchKernelGGL(k_sweep_bwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
+                                   F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (T *)F->d_part,
+                                   (long long)n, nr);
             F->counts[4]++;
         }
         tock(F, 6, tt);
