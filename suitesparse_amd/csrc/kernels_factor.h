@@ -338,6 +338,70 @@ __device__ __forceinline__ void store16(T *S, int r0, int c0, const typename Mfm
     }
 }
 
+// POTRF + inverse of the 16x16 diagonal block at S[R.., R..] (row stride ld)
+// by one wave, lane i < 16 owning row i; the inverse (zero above the diagonal)
+// goes to X in the same layout.  Each pivot costs one reciprocal square root
+// (no divide on the serial path): L_jj = d_jj r_j and L_ij = a_ij r_j with
+// r_j = 1/sqrt(d_jj); the inverse reuses r_j = 1/L_jj.  Returns the first
+// non-positive pivot of the block (16 if none), wave-uniform.
+template <typename T, bool SAFE>
+__device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, int lane,
+                                            const double *__restrict__ adiag, int col0, double rho,
+                                            int *__restrict__ npert) {
+    T a[16], rinv[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) a[c] = (lane < 16) ? S[(R + lane) * ld + R + c] : (T)0;
+    // pivot floors of the panel's columns (low-precision safeguard), lane j holds column j's
+    T floor_l = 0;
+    if (SAFE && lane < 16 && R + lane < nb) floor_l = (T)(rho * adiag[col0 + R + lane]);
+    // wave-uniform bookkeeping, written once after the panel
+    int npj = 0;        // pivots raised to their floor
+    int badj = 16;      // first non-positive pivot of the panel
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        T djj = lane_bcast(a[j], j);
+        const bool real = R + j < nb;
+        if (SAFE) {
+            const T floor_j = lane_bcast(floor_l, j);
+            const bool low = real && djj <= floor_j && floor_j > (T)0;
+            djj = low ? floor_j : djj;
+            npj += low ? 1 : 0;
+        }
+        const bool bad = !(djj > (T)0);
+        badj = (bad && real && badj == 16) ? j : badj;
+        djj = bad ? (T)1 : djj;
+        const T rj = rsqrt_nr(djj);
+        rinv[j] = rj;
+        const T l = (lane > j) ? a[j] * rj : ((lane == j) ? djj * rj : (T)0);
+        a[j] = l;
+#pragma unroll
+        for (int c = j + 1; c < 16; c++) a[c] -= l * lane_bcast(l, c);
+    }
+    if (SAFE && lane == 0 && npj > 0) atomicAdd(npert, npj);
+    if (lane < 16) {
+#pragma unroll
+        for (int c = 0; c < 16; c++)
+            if (c <= lane) S[(R + lane) * ld + R + c] = a[c];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // inverse: lane c < 16 forms column c by forward substitution
+    T x[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        T sacc = (i == lane) ? (T)1 : (T)0;
+#pragma unroll
+        for (int k = 0; k < i; k++) sacc -= S[(R + i) * ld + R + k] * x[k];
+        x[i] = sacc * rinv[i];
+    }
+    if (lane < 16) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) X[(R + i) * ld + R + lane] = (i >= lane) ? x[i] : (T)0;
+    }
+    return badj;
+}
+
 // SAFE (rho > 0, low-precision factor): a pivot d_jj with d_jj <= rho * a_jj
 // (a_jj the assembled diagonal of A) signals breakdown; it is replaced by
 // rho * a_jj and counted in *npert, and the FP64 refinement (or the FP64
@@ -377,65 +441,10 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
 
     for (int p = 0; p < np; p++) {
         const int R = 16 * p;
-        // (1) POTRF + inverse of the 16x16 diagonal block, one wave, lane i < 16 owns row i.
-        // Each pivot costs one reciprocal square root (no divide on the serial path):
-        // L_jj = d_jj r_j and L_ij = a_ij r_j with r_j = 1/sqrt(d_jj); the inverse
-        // reuses r_j = 1/L_jj.
+        // (1) POTRF + inverse of the 16x16 diagonal block, one wave
         if (wave == 0) {
-            T a[16], rinv[16];
-#pragma unroll
-            for (int c = 0; c < 16; c++) a[c] = (lane < 16) ? S[(R + lane) * DS + R + c] : (T)0;
-            // pivot floors of the panel's columns (low-precision safeguard), lane j holds column j's
-            T floor_l = 0;
-            if (SAFE && lane < 16 && R + lane < nb) floor_l = (T)(rho * adiag[t.col0 + R + lane]);
-            // wave-uniform bookkeeping, written once after the panel
-            int npj = 0;        // pivots raised to their floor
-            int badj = 16;      // first non-positive pivot of the panel
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                T djj = lane_bcast(a[j], j);
-                const bool real = R + j < nb;
-                if (SAFE) {
-                    const T floor_j = lane_bcast(floor_l, j);
-                    const bool low = real && djj <= floor_j && floor_j > (T)0;
-                    djj = low ? floor_j : djj;
-                    npj += low ? 1 : 0;
-                }
-                const bool bad = !(djj > (T)0);
-                badj = (bad && real && badj == 16) ? j : badj;
-                djj = bad ? (T)1 : djj;
-                const T rj = rsqrt_nr(djj);
-                rinv[j] = rj;
-                const T l = (lane > j) ? a[j] * rj : ((lane == j) ? djj * rj : (T)0);
-                a[j] = l;
-#pragma unroll
-                for (int c = j + 1; c < 16; c++) a[c] -= l * lane_bcast(l, c);
-            }
-            if (lane == 0) {
-                if (SAFE && npj > 0) atomicAdd(npert, npj);
-                if (badj < 16 && sfail < 0) sfail = R + badj;
-            }
-            if (lane < 16) {
-#pragma unroll
-                for (int c = 0; c < 16; c++)
-                    if (c <= lane) S[(R + lane) * DS + R + c] = a[c];
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            // inverse: lane c < 16 forms column c by forward substitution
-            T x[16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                T sacc = (i == lane) ? (T)1 : (T)0;
-#pragma unroll
-                for (int k = 0; k < i; k++) sacc -= S[(R + i) * DS + R + k] * x[k];
-                x[i] = sacc * rinv[i];
-            }
-            if (lane < 16) {
-#pragma unroll
-                for (int i = 0; i < 16; i++) X[(R + i) * DS + R + lane] = (i >= lane) ? x[i] : (T)0;
-            }
+            const int badj = potrf16_wave<T, SAFE>(S, X, DS, R, nb, lane, adiag, t.col0, rho, npert);
+            if (lane == 0 && badj < 16 && sfail < 0) sfail = R + badj;
         }
         __syncthreads();
         // (2) panel TRSM: rows below, 16-row tiles t = p+1..np-1, one wave each
@@ -496,6 +505,51 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
             if (i < nb && c < nb) {
                 if (c <= i) W[t.off + i + (long long)c * t.ld] = S[i * DS + c];
                 W[t.winv + i + (long long)c * nb] = (c <= i) ? X[i * DS + c] : (T)0;
+            }
+        }
+    }
+}
+
+
+// Diagonal blocks of at most 16 columns (the many tiny supernodes of the
+// lower etree levels): one wave per block, four blocks per workgroup, the
+// same POTRF + inverse as panel 0 of k_potrf_diag.
+template <typename T, bool SAFE>
+__global__ __launch_bounds__(256) void k_potrf_small(const DiagTask *__restrict__ tasks, int ntask,
+                                                     T *__restrict__ W, int *__restrict__ fail,
+                                                     const double *__restrict__ adiag, double rho,
+                                                     int *__restrict__ npert) {
+    constexpr int SS = 17;
+    __shared__ T S[4][16 * SS];
+    __shared__ T X[4][16 * SS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ti = blockIdx.x * 4 + w;
+    if (ti >= ntask) return;
+    const DiagTask t = tasks[ti];
+    const int nb = t.nb;
+    T *Sw = S[w], *Xw = X[w];
+    if (lane < 16) {
+        T v[16];
+#pragma unroll
+        for (int c = 0; c < 16; c++)
+            v[c] = (lane < nb && c <= lane) ? W[t.off + lane + (long long)c * t.ld] : (T)0;
+#pragma unroll
+        for (int c = 0; c < 16; c++) Sw[lane * SS + c] = (lane >= nb && lane == c) ? (T)1 : v[c];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int badj = potrf16_wave<T, SAFE>(Sw, Xw, SS, 0, nb, lane, adiag, t.col0, rho, npert);
+    if (lane == 0 && badj < nb) atomicMin(fail, t.col0 + badj);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (lane < nb) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+            if (c < nb) {
+                if (c <= lane) W[t.off + lane + (long long)c * t.ld] = Sw[lane * SS + c];
+                W[t.winv + lane + (long long)c * nb] = (c <= lane) ? Xw[lane * SS + c] : (T)0;
             }
         }
     }

@@ -107,7 +107,8 @@ static inline int tile_bucket(int M, int N, int K) {
 static constexpr int NBUCKET = 75;
 static constexpr int NKIND = 3;     // 0 descendant update, 1 trailing, 2 TRSM
 static inline int kind_of(int launch_kind) { return launch_kind == 0 ? 0 : launch_kind == 3 ? 1 : 2; }
-enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7 };
+enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7,
+       L_DIAG_SMALL = 8 };
 
 struct StepBlocks {
     std::vector<DiagTask> diags;
@@ -264,10 +265,14 @@ static int flatten_upload(mxc_factor *F) {
     for (size_t l = 0; l < F->blk_upd.size(); l++) {
         nupd += emit(L_UPD, F->blk_upd[l], true);
         for (const StepBlocks &sb : F->blk_steps[l]) {
-            long long db = (long long)diags.size();
-            diags.insert(diags.end(), sb.diags.begin(), sb.diags.end());
-            long long dc = (long long)diags.size() - db;
-            if (dc > 0) F->factor_launches.push_back(Launch{L_DIAG, db, dc, 0});
+            // blocks of at most 16 columns: one wave each; wider blocks: a workgroup each
+            for (int small = 1; small >= 0; small--) {
+                long long db = (long long)diags.size();
+                for (const DiagTask &d : sb.diags)
+                    if ((d.nb <= 16) == (small == 1)) diags.push_back(d);
+                long long dc = (long long)diags.size() - db;
+                if (dc > 0) F->factor_launches.push_back(Launch{small ? L_DIAG_SMALL : L_DIAG, db, dc, 0});
+            }
             emit(L_TRSM, sb.trsm, false);
             emit(L_TRAIL, sb.trail, true);
         }
@@ -857,7 +862,8 @@ static int enqueue_factor(mxc_factor *F) {
                        F->d_Ax, F->d_src, (long long)F->nnz_full, F->d_vals);
     tock(F, 0, tt);
     for (const Launch &L : F->factor_launches) {
-        const int cls = L.kind == L_UPD ? 1 : L.kind == L_DIAG ? 2 : L.kind == L_TRSM ? 3 : 4;
+        const int cls = L.kind == L_UPD ? 1 : (L.kind == L_DIAG || L.kind == L_DIAG_SMALL) ? 2
+                      : L.kind == L_TRSM ? 3 : 4;
         tick(F, cls, &tt);
         switch (L.kind) {
             case L_UPD:
@@ -875,6 +881,19 @@ static int enqueue_factor(mxc_factor *F) {
                                        (const double *)F->d_adiag, F->rho, F->d_fail + 1);
                 F->counts[1]++;
                 break;
+            case L_DIAG_SMALL: {
+                const unsigned g = (unsigned)((L.count + 3) / 4);
+                if (F->rho > 0)
+                    hipLaunchKernelGGL((k_potrf_small<T, true>), dim3(g), dim3(256), 0, sm,
+                                       F->d_diag + L.begin, (int)L.count, W, F->d_fail,
+                                       (const double *)F->d_adiag, F->rho, F->d_fail + 1);
+                else
+                    hipLaunchKernelGGL((k_potrf_small<T, false>), dim3(g), dim3(256), 0, sm,
+                                       F->d_diag + L.begin, (int)L.count, W, F->d_fail,
+                                       (const double *)F->d_adiag, F->rho, F->d_fail + 1);
+                F->counts[1]++;
+                break;
+            }
             case L_TRSM:
                 launch_tile<T, MODE_STORE>(F, L, W);
                 F->counts[2]++;
