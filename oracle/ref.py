@@ -51,6 +51,8 @@ def lib():
         L.ref_residual.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int64, _f64p]
         L.ref_free.argtypes = [C.c_void_p]
         L.ref_free.restype = None
+        L.ref_read_sparse.argtypes = [C.c_char_p] + [C.c_void_p] * 4
+        L.ref_read_sparse.restype = C.c_int
         _lib = L
     return _lib
 
@@ -61,6 +63,21 @@ def set_threads(n: int) -> int:
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def read_sparse(path) -> dict:
+    """cholmod_l_read_sparse2 on a file: nrow, ncol, stype, xtype and CSC p, i, x."""
+    dims = np.zeros(5, dtype=np.int64)
+    b = str(path).encode()
+    if lib().ref_read_sparse(b, _ptr(dims), None, None, None) != 0:
+        raise RuntimeError(f"reference reader failed on {path}")
+    nrow, ncol, nnz, stype, xtype = (int(v) for v in dims)
+    p = np.zeros(ncol + 1, dtype=np.int64)
+    i = np.zeros(max(nnz, 1), dtype=np.int64)
+    x = np.zeros(max(nnz, 1))
+    lib().ref_read_sparse(b, _ptr(dims), _ptr(p), _ptr(i), _ptr(x))
+    return {"nrow": nrow, "ncol": ncol, "stype": stype, "xtype": xtype, "p": p, "i": i[:nnz],
+            "x": x[:nnz]}
 
 
 class RefFactor:

@@ -11,9 +11,11 @@
 //   cholmod_l_factorize                      cholmod.h:2775
 //   cholmod_l_solve                          cholmod.h:2823
 //   cholmod_l_sdmult (residual)              cholmod.h (MatrixOps)
+//   cholmod_l_read_sparse2 (file reader)     CHOLMOD/Check/cholmod_read.c
 // Input matrices are the lower triangle of a symmetric matrix in CSC form
 // (stype = -1), int64 indices, as CHOLMOD's int64 ("_l_") interface takes.
 
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -223,4 +225,43 @@ void ref_free(void *vh) {
     cholmod_l_free_sparse(&h->A, &h->cm);
     cholmod_l_finish(&h->cm);
     free(h);
+}
+
+// cholmod_l_read_sparse2 (CHOLMOD/Check/cholmod_read.c, cholmod.h) on a file,
+// with the reader's defaults (prefer_binary FALSE: symmetric pattern-only
+// matrices get diag = 1 + degree and off-diagonals -1).  dims receives nrow,
+// ncol, nnz, stype, xtype (CHOLMOD_PATTERN 0 / REAL 1 / COMPLEX 2).  With Ap
+// NULL only dims is filled; otherwise the packed, sorted CSC arrays are copied
+// (Ax only for a real matrix).  Returns 0, or -1 if the reader fails.
+int ref_read_sparse(const char *path, int64_t *dims, int64_t *Ap, int64_t *Ai, double *Ax) {
+    cholmod_common cm;
+    cholmod_l_start(&cm);
+    cm.print = 0;
+    FILE *f = fopen(path, "r");
+    if (!f) {
+        cholmod_l_finish(&cm);
+        return -1;
+    }
+    cholmod_sparse *A = cholmod_l_read_sparse2(f, CHOLMOD_DOUBLE, &cm);
+    fclose(f);
+    if (!A) {
+        cholmod_l_finish(&cm);
+        return -1;
+    }
+    if (!A->packed || !A->sorted) cholmod_l_sort(A, &cm);
+    const int64_t *P = (const int64_t *)A->p, *I = (const int64_t *)A->i;
+    const int64_t nnz = P[A->ncol];
+    dims[0] = (int64_t)A->nrow;
+    dims[1] = (int64_t)A->ncol;
+    dims[2] = nnz;
+    dims[3] = A->stype;
+    dims[4] = A->xtype;
+    if (Ap) {
+        memcpy(Ap, P, (A->ncol + 1) * sizeof(int64_t));
+        memcpy(Ai, I, nnz * sizeof(int64_t));
+        if (Ax && A->xtype == CHOLMOD_REAL) memcpy(Ax, A->x, nnz * sizeof(double));
+    }
+    cholmod_l_free_sparse(&A, &cm);
+    cholmod_l_finish(&cm);
+    return 0;
 }
