@@ -166,6 +166,10 @@ class Factor:
         """Pivot-ratio threshold of the low-precision safeguard (0 = CHOLMOD semantics)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 2, int(round(rho * 1e9))))
 
+    def set_lookahead(self, on: bool):
+        """Overlap each panel's far trailing update with the next panel (default on)."""
+        check("mxc_set_option", lib().mxc_set_option(self._num, 3, int(bool(on))))
+
     def set_graphs(self, on: bool):
         """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 1, int(bool(on))))

@@ -106,13 +106,19 @@ static inline int tile_bucket(int M, int N, int K) {
 }
 static constexpr int NBUCKET = 75;
 static constexpr int NKIND = 3;     // 0 descendant update, 1 trailing, 2 TRSM
-static inline int kind_of(int launch_kind) { return launch_kind == 0 ? 0 : launch_kind == 3 ? 1 : 2; }
+static inline int kind_of(int launch_kind) {
+    return launch_kind == 0 ? 0 : (launch_kind == 3 || launch_kind == 9) ? 1 : 2;
+}
+// L_TRAIL2: the part of a panel's trailing update right of the next panel, on
+// the second stream; L_FORK / L_JOIN: the event edges between the two streams.
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7,
-       L_DIAG_SMALL = 8 };
+       L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11 };
 
 struct StepBlocks {
     std::vector<DiagTask> diags;
     std::vector<UpdDesc> trsm, trail;
+    std::vector<UpdDesc> trail2;   // right of the next panel (second stream)
+    bool panel_end = false;        // trail holds the update of the next panel's columns
 };
 
 struct mxc_factor {
@@ -123,6 +129,9 @@ struct mxc_factor {
     int64_t npairs = 0, nupd_tiles = 0;
     double fl = 0, lnz = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;       // look-ahead: trailing updates right of the next panel
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool lookahead = true;
     size_t bytes = 0;
 
     // host copies
@@ -263,6 +272,7 @@ static int flatten_upload(mxc_factor *F) {
         return total;
     };
     for (size_t l = 0; l < F->blk_upd.size(); l++) {
+        bool level_forked = false;
         nupd += emit(L_UPD, F->blk_upd[l], true);
         for (const StepBlocks &sb : F->blk_steps[l]) {
             // blocks of at most 16 columns: one wave each; wider blocks: a workgroup each
@@ -274,8 +284,23 @@ static int flatten_upload(mxc_factor *F) {
                 if (dc > 0) F->factor_launches.push_back(Launch{small ? L_DIAG_SMALL : L_DIAG, db, dc, 0});
             }
             emit(L_TRSM, sb.trsm, false);
+            // At the end of panel P: wait for the second-stream update of panel
+            // P-1 (it also wrote the columns of panel P+1), then start panel P's
+            // own second-stream update (columns right of panel P+1; it needs only
+            // the finished panel P) and update panel P+1's columns here.  Panel
+            // P+1's steps then overlap the second-stream update.
+            // (a join only after a fork: the second stream must already be part of
+            // the work, also inside a graph capture)
+            if (sb.panel_end && level_forked) F->factor_launches.push_back(Launch{L_JOIN, 0, 0, 0});
+            if (!sb.trail2.empty()) {
+                F->factor_launches.push_back(Launch{L_FORK, 0, 0, 0});
+                emit(L_TRAIL2, sb.trail2, true);
+                level_forked = true;
+            }
             emit(L_TRAIL, sb.trail, true);
         }
+        // the next level reads this level's columns: join the second stream
+        if (level_forked) F->factor_launches.push_back(Launch{L_JOIN, 0, 0, 0});
     }
     F->nupd_tiles = nupd;
     if (descs.size() >= (size_t)INT32_MAX || tiles.size() >= (size_t)INT32_MAX) return MXC_TOO_LARGE;
@@ -514,19 +539,30 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
                 }
                 if (r0 == pend && pend < nscol) {
-                    UpdDesc u;
-                    memset(&u, 0, sizeof(u));
-                    u.x_off = psx + pend + pstart * nsrow;
-                    u.y_off = u.x_off;
-                    u.ldx = u.ldy = (int)nsrow;
-                    u.K = (int)(pend - pstart);
-                    u.M = (int)(nsrow - pend);
-                    u.N = (int)(nscol - pend);
-                    u.c_off = psx + pend + pend * nsrow;
-                    u.ldc = (int)nsrow;
-                    sb.trail.push_back(u);
-                    double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
-                    F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
+                    // update by the finished panel of everything right of it, split
+                    // at the end of the next panel: the next panel's columns (on the
+                    // critical path) and the rest, which overlaps the next panel's
+                    // steps on the second stream
+                    const int64_t split = std::min<int64_t>(nscol, pend + PANEL);
+                    for (int part = 0; part < 2; part++) {
+                        const int64_t c_lo = part == 0 ? pend : split;
+                        const int64_t c_hi = part == 0 ? split : nscol;
+                        if (c_lo >= c_hi) continue;
+                        UpdDesc u;
+                        memset(&u, 0, sizeof(u));
+                        u.x_off = psx + c_lo + pstart * nsrow;
+                        u.y_off = u.x_off;
+                        u.ldx = u.ldy = (int)nsrow;
+                        u.K = (int)(pend - pstart);
+                        u.M = (int)(nsrow - c_lo);
+                        u.N = (int)(c_hi - c_lo);
+                        u.c_off = psx + c_lo + c_lo * nsrow;
+                        u.ldc = (int)nsrow;
+                        (part == 0 ? sb.trail : sb.trail2).push_back(u);
+                        double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
+                        F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
+                    }
+                    sb.panel_end = true;
                 }
             }
         }
@@ -702,6 +738,9 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
         if (p) (void)hipFree(p);
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
     F->ev.clear();
+    if (F->ev_fork) (void)hipEventDestroy(F->ev_fork);
+    if (F->ev_join) (void)hipEventDestroy(F->ev_join);
+    if (F->stream2) (void)hipStreamDestroy(F->stream2);
     if (F->stream) (void)hipStreamDestroy(F->stream);
     delete F;
 }
@@ -724,9 +763,12 @@ extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const
         }
     hipError_t e = hipSetDevice(g_device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&F->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&F->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
         int rc = set_hip_err(e, "hipStreamCreate", __LINE__);
-        delete F;
+        mxc_factor_free(F);
         return rc;
     }
     int rc = precision == MXC_FP64 ? build_plan<double>(F, S, Ap, Ai) : build_plan<float>(F, S, Ap, Ai);
@@ -829,15 +871,16 @@ static inline unsigned grid_for(long long n, int bs, long long cap = 65536) {
 }
 
 template <typename T, int MODE>
-static void launch_tile(mxc_factor *F, const Launch &L, T *W) {
+static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = nullptr) {
+    if (!st) st = F->stream;
     if (L.tile == 32)
-        hipLaunchKernelGGL((k_tile<T, MODE, 32>), dim3((unsigned)L.count), dim3(64), 0, F->stream,
+        hipLaunchKernelGGL((k_tile<T, MODE, 32>), dim3((unsigned)L.count), dim3(64), 0, st,
                            F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
     else if (L.tile == 64)
-        hipLaunchKernelGGL((k_tile<T, MODE, 64>), dim3((unsigned)L.count), dim3(256), 0, F->stream,
+        hipLaunchKernelGGL((k_tile<T, MODE, 64>), dim3((unsigned)L.count), dim3(256), 0, st,
                            F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
     else
-        hipLaunchKernelGGL((k_tile<T, MODE, 128>), dim3((unsigned)L.count), dim3(256), 0, F->stream,
+        hipLaunchKernelGGL((k_tile<T, MODE, 128>), dim3((unsigned)L.count), dim3(256), 0, st,
                            F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
 }
 
@@ -861,7 +904,20 @@ static int enqueue_factor(mxc_factor *F) {
     hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(F->nnz_full, 256)), dim3(256), 0, sm,
                        F->d_Ax, F->d_src, (long long)F->nnz_full, F->d_vals);
     tock(F, 0, tt);
+    // two streams (look-ahead) except when launches are timed one by one
+    const bool two = F->lookahead && F->stream2 != nullptr && F->timing_mask == 0;
     for (const Launch &L : F->factor_launches) {
+        if (L.kind == L_FORK || L.kind == L_JOIN) {
+            if (!two) continue;
+            if (L.kind == L_FORK) {
+                HIPTRY(hipEventRecord(F->ev_fork, sm));
+                HIPTRY(hipStreamWaitEvent(F->stream2, F->ev_fork, 0));
+            } else {
+                HIPTRY(hipEventRecord(F->ev_join, F->stream2));
+                HIPTRY(hipStreamWaitEvent(sm, F->ev_join, 0));
+            }
+            continue;
+        }
         const int cls = L.kind == L_UPD ? 1 : (L.kind == L_DIAG || L.kind == L_DIAG_SMALL) ? 2
                       : L.kind == L_TRSM ? 3 : 4;
         tick(F, cls, &tt);
@@ -900,6 +956,10 @@ static int enqueue_factor(mxc_factor *F) {
                 break;
             case L_TRAIL:
                 launch_tile<T, MODE_TRAIL>(F, L, W);
+                F->counts[3]++;
+                break;
+            case L_TRAIL2:
+                launch_tile<T, MODE_TRAIL>(F, L, W, two ? F->stream2 : sm);
                 F->counts[3]++;
                 break;
         }
@@ -1564,6 +1624,10 @@ extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
         case MXC_OPT_PIVOT_RHO_E9:
             if (value < 0) return MXC_INVALID;
             F->rho = (double)value * 1e-9;
+            drop_factor_graph(F);
+            return MXC_OK;
+        case MXC_OPT_LOOKAHEAD:
+            F->lookahead = value != 0;
             drop_factor_graph(F);
             return MXC_OK;
         default:
