@@ -175,6 +175,12 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * panel of a large supernode right of the next panel on a second stream, so
  * that it overlaps the next panel's diagonal-block steps.  0: one stream. */
 #define MXC_OPT_LOOKAHEAD 3
+/* MXC_OPT_SPLIT16 (default 1, FP32 factors only): the descendant-update and
+ * trailing tiles 64 and 128 wide compute each product as hi*hi + hi*lo + lo*hi
+ * of FP16 halves on the FP16 MFMA, FP32 accumulation, after a per-row
+ * power-of-two scaling by the assembled diagonal (|L_ij| <= sqrt(a_ii)).
+ * 0: FP32 MFMA. */
+#define MXC_OPT_SPLIT16 4
 int mxc_set_option(mxc_factor *F, int option, int64_t value);
 
 /* As mxc_solve_refine for a block-diagonal batch of independent systems: rows

@@ -170,6 +170,10 @@ class Factor:
         """Overlap each panel's far trailing update with the next panel (default on)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 3, int(bool(on))))
 
+    def set_split16(self, on: bool):
+        """FP32 factors: split-FP16 products on the FP16 MFMA (default on)."""
+        check("mxc_set_option", lib().mxc_set_option(self._num, 4, int(bool(on))))
+
     def set_graphs(self, on: bool):
         """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 1, int(bool(on))))

@@ -261,6 +261,185 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
 }
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// FP32 factor, split-FP16 products (update and trailing blocks).  A staged
+// FP32 value v of global row g is scaled by 2^s_g, s_g = 13 - floor(log2
+// sqrt(a_gg)) with a_gg the assembled diagonal of A (|L_gk| <= sqrt(a_gg), so
+// the scaled values stay below 2^14), and split into hi = fp16(v 2^s) and
+// lo = fp16(v 2^s - hi).  C = Y X' accumulates hi*hi + hi*lo + lo*hi in FP32
+// on the 16x16x32 FP16 MFMA (lo*lo lies below FP32 rounding relative to the
+// row norms) and is unscaled by 2^-(s_i + s_j) in the epilogue, which is the
+// one of k_tile.  One LDS buffer of hi / lo rows, k contiguous, row stride 40
+// halves (the 16-byte reads of a 16-row group fall in distinct banks); the
+// next chunk is prefetched into registers.
+// ---------------------------------------------------------------------------
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ int row_scale_exp(const double *__restrict__ adiag, int g) {
+    const double a = adiag[g];
+    if (!(a > 0.0) || !(a < 1e300)) return 0;
+    return min(max(13 - ilogb(sqrt(a)), -100), 100);
+}
+
+template <int MODE, int TS>
+__global__ __launch_bounds__(256, 2) void k_tile_h(const UpdDesc *__restrict__ descs,
+                                                   const Tile *__restrict__ tiles,
+                                                   float *__restrict__ W,
+                                                   const int *__restrict__ Ls,
+                                                   const double *__restrict__ adiag) {
+    static_assert(TS == 64 || TS == 128, "split-FP16 tiles are 64 or 128 wide");
+    constexpr int NT = 256;
+    constexpr int WB = TS / 2;           // edge of a wave's block of C (2 x 2 waves)
+    constexpr int MB = WB / 16;
+    constexpr int BKH = 32;              // K per chunk: one MFMA depth
+    constexpr int RS = BKH + 8;          // LDS row stride (halves)
+    constexpr int KG = NT / TS;          // staging threads per row
+    constexpr int PER = BKH / KG;        // consecutive k per staging thread
+    __shared__ __attribute__((aligned(16))) _Float16 Xh[TS * RS];
+    __shared__ __attribute__((aligned(16))) _Float16 Xl[TS * RS];
+    __shared__ __attribute__((aligned(16))) _Float16 Yh[TS * RS];
+    __shared__ __attribute__((aligned(16))) _Float16 Yl[TS * RS];
+    __shared__ int relr[TS], relc[TS];
+    __shared__ int exr[TS], exc[TS];     // scale exponents of the Y rows (i) and X rows (j)
+
+    const Tile tl = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+    const UpdDesc d = descs[tl.desc];
+    const int i0 = tl.ti * TS, j0 = tl.tj * TS;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wc = wave >> 1;
+
+    for (int e = tid; e < 2 * TS; e += NT) {
+        if (e < TS) {
+            const int i = i0 + e;
+            int r = -1, ex = 0;
+            if (i < d.M) {
+                const int g = Ls[d.ls_off + i];
+                ex = row_scale_exp(adiag, g);
+                if (MODE == MODE_SCATTER) {
+                    if (g < d.base + d.t_nscol) {
+                        r = g - d.base;
+                    } else {
+                        int lo = d.t_psi + d.t_nscol, hi = d.t_psi + d.t_nsrow - 1;
+                        while (lo < hi) {
+                            int mid = (lo + hi) >> 1;
+                            if (Ls[mid] < g) lo = mid + 1; else hi = mid;
+                        }
+                        r = lo - d.t_psi;
+                    }
+                }
+            }
+            relr[e] = r;
+            exr[e] = ex;
+        } else {
+            const int j = j0 + e - TS;
+            int c = -1, ex = 0;
+            if (j < d.N) {
+                const int g = Ls[d.ls_off + j];
+                ex = row_scale_exp(adiag, g);
+                c = g - d.base;
+            }
+            relc[e - TS] = c;
+            exc[e - TS] = ex;
+        }
+    }
+    __syncthreads();
+
+    // staging: thread t holds row r = t % TS, k in [kq * PER, kq * PER + PER) of a chunk
+    const int r = tid % TS, kq = tid / TS;
+    const bool rowy = (i0 + r) < d.M, rowx = (j0 + r) < d.N;
+    const float sy = ldexpf(1.0f, exr[r]), sx = ldexpf(1.0f, exc[r]);
+    const float *yrow = W + d.y_off + (i0 + r);
+    const float *xrow = W + d.x_off + (j0 + r);
+    float ry[PER], rx[PER];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int gk = k0 + kq * PER + q;
+            const bool kin = gk < d.K;
+            ry[q] = (kin && rowy) ? yrow[(long long)gk * d.ldy] : 0.0f;
+            rx[q] = (kin && rowx) ? xrow[(long long)gk * d.ldx] : 0.0f;
+        }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int h = 0; h < PER; h += 8) {
+            half8_t yh, yl, xh, xl;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const float vy = ry[h + q] * sy, vx = rx[h + q] * sx;
+                const _Float16 ay = (_Float16)vy, ax = (_Float16)vx;
+                yh[q] = ay;
+                yl[q] = (_Float16)(vy - (float)ay);
+                xh[q] = ax;
+                xl[q] = (_Float16)(vx - (float)ax);
+            }
+            const int o = r * RS + kq * PER + h;
+            *reinterpret_cast<half8_t *>(&Yh[o]) = yh;
+            *reinterpret_cast<half8_t *>(&Yl[o]) = yl;
+            *reinterpret_cast<half8_t *>(&Xh[o]) = xh;
+            *reinterpret_cast<half8_t *>(&Xl[o]) = xl;
+        }
+    };
+
+    f32x4 acc[MB][MB];
+#pragma unroll
+    for (int a = 0; a < MB; a++)
+#pragma unroll
+        for (int b = 0; b < MB; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+
+    fetch(0);
+    const int ko = 8 * (lane >> 4);
+    for (int k0 = 0; k0 < d.K; k0 += BKH) {
+        __syncthreads();                         // the previous chunk's reads are done
+        stage();
+        __syncthreads();
+        if (k0 + BKH < d.K) fetch(k0 + BKH);     // next chunk in flight during the MFMAs
+        half8_t xh[MB], xl[MB], yh[MB], yl[MB];
+#pragma unroll
+        for (int m = 0; m < MB; m++) {
+            const int o = (wc * WB + m * 16 + (lane & 15)) * RS + ko;
+            xh[m] = *reinterpret_cast<const half8_t *>(&Xh[o]);
+            xl[m] = *reinterpret_cast<const half8_t *>(&Xl[o]);
+        }
+#pragma unroll
+        for (int n = 0; n < MB; n++) {
+            const int o = (wr * WB + n * 16 + (lane & 15)) * RS + ko;
+            yh[n] = *reinterpret_cast<const half8_t *>(&Yh[o]);
+            yl[n] = *reinterpret_cast<const half8_t *>(&Yl[o]);
+        }
+#pragma unroll
+        for (int m = 0; m < MB; m++)
+#pragma unroll
+            for (int n = 0; n < MB; n++) {
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh[m], yh[n], acc[m][n], 0, 0, 0);
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh[m], yl[n], acc[m][n], 0, 0, 0);
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl[m], yh[n], acc[m][n], 0, 0, 0);
+            }
+    }
+
+    // epilogue: acc[m][n][v] = C(i, j) 2^(s_i + s_j), i = i0 + WB*wr + 16n + lane%16,
+    //                                               j = j0 + WB*wc + 16m + 4 (lane/16) + v
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+#pragma unroll
+        for (int n = 0; n < MB; n++) {
+            const int il = wr * WB + n * 16 + (lane & 15);
+            const int i = i0 + il;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int jl = wc * WB + m * 16 + 4 * (lane >> 4) + v;
+                const int j = j0 + jl;
+                if (i >= d.M || j >= d.N || i < j) continue;
+                const float c = ldexpf(acc[m][n][v], -(exr[il] + exc[jl]));
+                if (MODE == MODE_SCATTER)
+                    atomic_sub(W + d.c_off + (long long)relc[jl] * d.ldc + relr[il], c);
+                else
+                    W[d.c_off + (long long)j * d.ldc + i] -= c;
+            }
+        }
+    }
+}
+
 // Diagonal block Cholesky + triangular inverse of a 64x64 block (a block
 // narrower than 64 is extended by the identity), one 256-thread workgroup,
 // block-recursive over 16x16 sub-blocks with the 16x16x4 MFMA:

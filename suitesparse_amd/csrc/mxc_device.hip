@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cholmod_mi355x.h"
@@ -132,6 +133,7 @@ struct mxc_factor {
     hipStream_t stream2 = nullptr;       // look-ahead: trailing updates right of the next panel
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool lookahead = true;
+    bool split16 = true;                 // FP32 factors: split-FP16 update/trailing tiles
     size_t bytes = 0;
 
     // host copies
@@ -533,6 +535,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.M = (int)(nsrow - r0);
                     u.N = (int)(pend - r0);
                     u.c_off = psx + r0 + r0 * nsrow;
+                    u.ls_off = (int)(S->Lpi[s] + r0);   // global rows (for per-row scales)
                     u.ldc = (int)nsrow;
                     sb.trail.push_back(u);
                     double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
@@ -557,6 +560,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                         u.M = (int)(nsrow - c_lo);
                         u.N = (int)(c_hi - c_lo);
                         u.c_off = psx + c_lo + c_lo * nsrow;
+                        u.ls_off = (int)(S->Lpi[s] + c_lo);
                         u.ldc = (int)nsrow;
                         (part == 0 ? sb.trail : sb.trail2).push_back(u);
                         double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
@@ -873,6 +877,18 @@ static inline unsigned grid_for(long long n, int bs, long long cap = 65536) {
 template <typename T, int MODE>
 static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = nullptr) {
     if (!st) st = F->stream;
+    if constexpr (std::is_same<T, float>::value && MODE != MODE_STORE) {
+        // FP32 factor: split-FP16 products on the FP16 MFMA for 64- and 128-wide tiles
+        if (F->split16 && L.tile >= 64) {
+            if (L.tile == 64)
+                hipLaunchKernelGGL((k_tile_h<MODE, 64>), dim3((unsigned)L.count), dim3(256), 0, st,
+                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, (const double *)F->d_adiag);
+            else
+                hipLaunchKernelGGL((k_tile_h<MODE, 128>), dim3((unsigned)L.count), dim3(256), 0, st,
+                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, (const double *)F->d_adiag);
+            return;
+        }
+    }
     if (L.tile == 32)
         hipLaunchKernelGGL((k_tile<T, MODE, 32>), dim3((unsigned)L.count), dim3(64), 0, st,
                            F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
@@ -1628,6 +1644,10 @@ extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
             return MXC_OK;
         case MXC_OPT_LOOKAHEAD:
             F->lookahead = value != 0;
+            drop_factor_graph(F);
+            return MXC_OK;
+        case MXC_OPT_SPLIT16:
+            F->split16 = value != 0;
             drop_factor_graph(F);
             return MXC_OK;
         default:

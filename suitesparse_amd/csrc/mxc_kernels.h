@@ -26,6 +26,8 @@ constexpr int LDS_PAD = 4;
 //   Ls[t_psi+t_nscol .. t_psi+t_nsrow); C column j lands on local column g-base.
 // MODE_TRAIL:   W[c_off + j*ldc + i] -= C(i,j) for i >= j (in-place trailing update)
 // MODE_STORE:   W[c_off + j*ldc + i]  = C(i,j)           (TRSM by an inverse block)
+// Block C(i, j) = sum_k Y[i][k] X[j][k], i < M, j < N (X rows are the first N
+// rows of the Y side); ls_off: Ls position of Y row 0 (its global row index).
 struct UpdDesc {
     long long x_off, y_off, c_off;
     int ldx, ldy, ldc;
