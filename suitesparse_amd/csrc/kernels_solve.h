@@ -37,35 +37,85 @@ __global__ void k_perm_scatter(const T *__restrict__ Y, const long long *__restr
     for (int r = 0; r < nrhs; r++) X[pk + r * n] = (double)Y[k + r * n];
 }
 
-// Value of v in lane addr / 4 (ds_bpermute, 32 bits at a time).
-__device__ __forceinline__ float lane_read(float v, int addr) {
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+// Cross-lane moves for the wave butterfly, all on the VALU (no LDS traffic):
+// v_permlane32_swap / v_permlane16_swap exchange the upper half of one operand
+// with the lower half of the other across 32- / 16-lane halves, and DPP row
+// operations move data inside a 16-lane row.  64-bit values move as two words.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
 }
-__device__ __forceinline__ double lane_read(double v, int addr) {
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
     const long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)u);
-    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(u >> 32));
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// a + b after swapping across 32-lane (W = 32) or 16-lane (W = 16) halves: in
+// a lane of the lower half the sum of its own a and its partner's a, in a lane
+// of the upper half the sum of its partner's b and its own b.
+template <int W>
+__device__ __forceinline__ void swap_words(unsigned &a, unsigned &b) {
+    if constexpr (W == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+        a = r[0];
+        b = r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+        a = r[0];
+        b = r[1];
+    }
+}
+template <int W>
+__device__ __forceinline__ float swap_sum(float a, float b) {
+    unsigned x = __float_as_uint(a), y = __float_as_uint(b);
+    swap_words<W>(x, y);
+    return __uint_as_float(x) + __uint_as_float(y);
+}
+template <int W>
+__device__ __forceinline__ double swap_sum(double a, double b) {
+    const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+    const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+    unsigned alo = (unsigned)ua, ahi = (unsigned)(ua >> 32);
+    unsigned blo = (unsigned)ub, bhi = (unsigned)(ub >> 32);
+    swap_words<W>(alo, blo);
+    swap_words<W>(ahi, bhi);
+    const double x = __longlong_as_double((long long)(((unsigned long long)ahi << 32) | alo));
+    const double y = __longlong_as_double((long long)(((unsigned long long)bhi << 32) | blo));
+    return x + y;
+}
+
 // One halving step: lanes with bit 4*HALF set keep the upper HALF values,
-// the others the lower HALF, and each adds its partner's copy of them.
+// the others the lower HALF, each summed with its partner's copy (a lane that
+// differs in that bit; after the last step the sums over each group of 4
+// lanes cover all 64).
 template <int HALF, typename T>
 __device__ __forceinline__ void fold_half(T (&v)[16], int lane) {
-    constexpr int MASK = 4 * HALF;
-    const bool up = (lane & MASK) != 0;
-    const int addr = (lane ^ MASK) << 2;
+    if constexpr (HALF == 8 || HALF == 4) {
 #pragma unroll
-    for (int i = 0; i < HALF; i++) {
-        const T keep = up ? v[i + HALF] : v[i];
-        const T send = up ? v[i] : v[i + HALF];
-        v[i] = keep + lane_read(send, addr);
+        for (int i = 0; i < HALF; i++) v[i] = swap_sum<4 * HALF>(v[i], v[i + HALF]);
+    } else {
+        constexpr int MASK = 4 * HALF;
+        const bool up = (lane & MASK) != 0;
+#pragma unroll
+        for (int i = 0; i < HALF; i++) {
+            const T keep = up ? v[i + HALF] : v[i];
+            const T send = up ? v[i] : v[i + HALF];
+            // partner across lane bit 3: row_ror:8 (lane ^ 8); across bit 2:
+            // row_half_mirror (lane i <-> 7 - i of each 8 lanes).  One uniform DPP
+            // move each: a per-lane choice between two moves is compiled into
+            // divergent branches, where DPP reads of disabled lanes return 0.
+            const T got = (HALF == 2) ? dpp_mov<0x128>(send) : dpp_mov<0x141>(send);
+            v[i] = keep + got;
+        }
     }
 }
 
 // Sum of 16 per-lane values over the 64 lanes of a wave, transposed: on
 // return lane l holds the total of index (l >> 2) & 15 (every lane of a
-// group of 4 holds the same total).  Halving butterfly: 17 lane exchanges.
+// group of 4 holds the same total).
 template <typename T>
 __device__ __forceinline__ T wave_sum16(T (&v)[16], int lane) {
     fold_half<8>(v, lane);
@@ -73,8 +123,8 @@ __device__ __forceinline__ T wave_sum16(T (&v)[16], int lane) {
     fold_half<2>(v, lane);
     fold_half<1>(v, lane);
     T s = v[0];
-    s += lane_read(s, (lane ^ 2) << 2);
-    s += lane_read(s, (lane ^ 1) << 2);
+    s += dpp_mov<0x4E>(s);      // quad_perm [2,3,0,1]: lane ^ 2
+    s += dpp_mov<0xB1>(s);      // quad_perm [1,0,3,2]: lane ^ 1
     return s;
 }
 
@@ -226,8 +276,13 @@ __global__ __launch_bounds__(256) void k_sweep_bwd(const SweepTask *__restrict__
     const int qcol = (lane >> 2) & 15;          // column of a wave_sum16 total
     T *xb = X + t.k1 + t.c0;
     if ((int)blockIdx.x < ndiag) {
+        // The two products with transposed operands (L[t+1, t]' and inv(L_tt)')
+        // go through a transposed LDS copy: lane c then owns output column c and
+        // sums its wave's 16 rows with plain LDS reads (no lane exchanges).
+        __shared__ T tt[NB][NB + 1];         // tt[c][k]: transposed 64 x 64 block
+        __shared__ T ptot[ZC][NB];           // partial sums of the rows below
         const int nb = t.nb, nbn = t.r0;
-        // lane j: row c0 + 64 + j (block t+1) / row j of the inverse;
+        // lane k: row c0 + 64 + k (block t+1) / row k of the inverse;
         // wave w: columns 16w .. 16w+15 of block t
         const T *Lb = W + t.lx + (long long)t.c0 * t.nsrow + t.c0 + NB + lane;
         T ln[16], iv[16];
@@ -237,52 +292,81 @@ __global__ __launch_bounds__(256) void k_sweep_bwd(const SweepTask *__restrict__
             ln[j] = (lane < nbn && k < nb) ? Lb[(long long)k * t.nsrow] : (T)0;
             iv[j] = (lane < nb && k < nb) ? W[t.winv + lane + (long long)k * nb] : (T)0;
         }
+        const long long pstride = (long long)nrhs * NB;
         for (int z0 = 0; z0 < nrhs; z0 += ZC) {
             const int zc = min(ZC, nrhs - z0);
             for (int e = tid; e < ZC * NB; e += 256) {
                 const int z = e >> 6, k = e & 63;
                 xs[z][k] = (z < zc && k < nbn) ? xb[(long long)(z0 + z) * n + NB + k] : (T)0;
             }
-            __syncthreads();
-            for (int z = 0; z < zc; z++) {
-                T v[16];
-                const T xj = xs[z][lane];
 #pragma unroll
-                for (int j = 0; j < 16; j++) v[j] = ln[j] * xj;
-                const T s = wave_sum16(v, lane);
-                if ((lane & 3) == 0) red[0][z][16 * w + qcol] = s;
+            for (int j = 0; j < 16; j++) tt[16 * w + j][lane] = ln[j];
+            // partial sums of the rows below: wave w takes slots p = w (mod 4), every
+            // right-hand side of the chunk, two slots in flight
+            {
+                T s0[ZC], s1[ZC];
+#pragma unroll
+                for (int z = 0; z < ZC; z++) s0[z] = s1[z] = 0;
+                const T *pp = part + ((long long)t.p0 * nrhs + z0) * NB + lane;
+                int p = w;
+                for (; p + 4 < t.np; p += 8) {
+#pragma unroll
+                    for (int z = 0; z < ZC; z++) {
+                        if (z < zc) {
+                            s0[z] += pp[(long long)p * pstride + z * NB];
+                            s1[z] += pp[(long long)(p + 4) * pstride + z * NB];
+                        }
+                    }
+                }
+                if (p < t.np) {
+#pragma unroll
+                    for (int z = 0; z < ZC; z++)
+                        if (z < zc) s0[z] += pp[(long long)p * pstride + z * NB];
+                }
+#pragma unroll
+                for (int z = 0; z < ZC; z++) red[w][z][lane] = s0[z] + s1[z];
             }
             __syncthreads();
+            // ptot = partials; red = neighbour term L[t+1, t]' x_{t+1}, wave w: rows 16w..
+            for (int z = w; z < ZC; z += 4)
+                ptot[z][lane] = (red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]);
+            __syncthreads();
+#pragma unroll
+            for (int z = 0; z < ZC; z++) {
+                T a = 0;
+                if (z < zc) {
+#pragma unroll
+                    for (int j = 0; j < 16; j++) a += tt[lane][16 * w + j] * xs[z][16 * w + j];
+                }
+                red[w][z][lane] = a;
+            }
+            __syncthreads();
+            // v = x_t - partials - neighbour term (into xs); the inverse goes into tt
             for (int z = w; z < ZC; z += 4) {
                 T v = 0;
-                if (z < zc && lane < nb) {
-                    const T *pp = part + ((long long)t.p0 * nrhs + z0 + z) * NB + lane;
-                    const long long stride = (long long)nrhs * NB;
-                    T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-                    int p = 0;
-                    for (; p + 4 <= t.np; p += 4) {
-                        a0 += pp[(p + 0) * stride];
-                        a1 += pp[(p + 1) * stride];
-                        a2 += pp[(p + 2) * stride];
-                        a3 += pp[(p + 3) * stride];
-                    }
-                    for (; p < t.np; p++) a0 += pp[p * stride];
-                    v = xb[(long long)(z0 + z) * n + lane] - red[0][z][lane] - ((a0 + a1) + (a2 + a3));
-                }
+                if (z < zc && lane < nb)
+                    v = xb[(long long)(z0 + z) * n + lane] - ptot[z][lane] -
+                        ((red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]));
                 xs[z][lane] = v;
             }
-            __syncthreads();
-            for (int z = 0; z < zc; z++) {
-                T v[16];
-                const T xk = xs[z][lane];
 #pragma unroll
-                for (int j = 0; j < 16; j++) v[j] = iv[j] * xk;
-                const T s = wave_sum16(v, lane);
-                if ((lane & 3) == 0) red[0][z][16 * w + qcol] = s;
+            for (int j = 0; j < 16; j++) tt[16 * w + j][lane] = iv[j];
+            __syncthreads();
+            // x_t = inv(L_tt)' v: column c sums inv[k][c] v[k] over wave w's rows k
+#pragma unroll
+            for (int z = 0; z < ZC; z++) {
+                T a = 0;
+                if (z < zc) {
+#pragma unroll
+                    for (int j = 0; j < 16; j++) a += tt[lane][16 * w + j] * xs[z][16 * w + j];
+                }
+                red[w][z][lane] = a;
             }
             __syncthreads();
             for (int z = w; z < zc; z += 4)
-                if (lane < nb) xb[(long long)(z0 + z) * n + lane] = red[0][z][lane];
+                if (lane < nb)
+                    xb[(long long)(z0 + z) * n + lane] =
+                        (red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]);
             __syncthreads();
         }
         return;
