@@ -176,7 +176,7 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * that it overlaps the next panel's diagonal-block steps.  0: one stream. */
 #define MXC_OPT_LOOKAHEAD 3
 /* MXC_OPT_SPLIT16 (default 1, FP32 factors only): the descendant-update and
- * trailing tiles 64 and 128 wide compute each product as hi*hi + hi*lo + lo*hi
+ * trailing tiles compute each product as hi*hi + hi*lo + lo*hi
  * of FP16 halves on the FP16 MFMA, FP32 accumulation, after a per-row
  * power-of-two scaling by the assembled diagonal (|L_ij| <= sqrt(a_ii)).
  * 0: FP32 MFMA. */

@@ -878,9 +878,12 @@ template <typename T, int MODE>
 static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = nullptr) {
     if (!st) st = F->stream;
     if constexpr (std::is_same<T, float>::value && MODE != MODE_STORE) {
-        // FP32 factor: split-FP16 products on the FP16 MFMA for 64- and 128-wide tiles
-        if (F->split16 && L.tile >= 64) {
-            if (L.tile == 64)
+        // FP32 factor: split-FP16 products on the FP16 MFMA
+        if (F->split16) {
+            if (L.tile == 32)
+                hipLaunchKernelGGL((k_tile_h<MODE, 32>), dim3((unsigned)L.count), dim3(64), 0, st,
+                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, (const double *)F->d_adiag);
+            else if (L.tile == 64)
                 hipLaunchKernelGGL((k_tile_h<MODE, 64>), dim3((unsigned)L.count), dim3(256), 0, st,
                                    F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, (const double *)F->d_adiag);
             else
