@@ -181,6 +181,12 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * power-of-two scaling by the assembled diagonal (|L_ij| <= sqrt(a_ii)).
  * 0: FP32 MFMA. */
 #define MXC_OPT_SPLIT16 4
+/* MXC_OPT_SOLVE_CHAIN (default 1): the solve sweeps take one launch per etree
+ * level; inside it each 64-column block of a supernode waits, through a
+ * per-supernode counter in device memory, for the blocks it depends on.  A
+ * wait is bounded: a sweep whose wait gives up returns MXC_GPU_PROBLEM.
+ * 0: one launch per 64-column step of a level. */
+#define MXC_OPT_SOLVE_CHAIN 5
 int mxc_set_option(mxc_factor *F, int option, int64_t value);
 
 /* As mxc_solve_refine for a block-diagonal batch of independent systems: rows

@@ -174,6 +174,11 @@ class Factor:
         """FP32 factors: split-FP16 products on the FP16 MFMA (default on)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 4, int(bool(on))))
 
+    def set_solve_chain(self, on: bool):
+        """Solve sweeps: one chained launch per etree level (default on); off: one
+        launch per 64-column step."""
+        check("mxc_set_option", lib().mxc_set_option(self._num, 5, int(bool(on))))
+
     def set_graphs(self, on: bool):
         """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 1, int(bool(on))))

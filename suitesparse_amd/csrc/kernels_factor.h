@@ -135,11 +135,30 @@ __device__ __forceinline__ int xcd_tile(int b, int g) {
     return (x + 8 * (k / C)) * C + (k % C);
 }
 
+// Split-FP16 form of the FP32 factor (see k_tile_h): the scale exponent of
+// global row g, s_g = 13 - floor(log2 sqrt(a_gg)), and the packed pair
+// (hi | lo << 16) of a value v of that row, hi = fp16(v 2^s), lo = fp16(v 2^s - hi).
+__device__ __forceinline__ int row_scale_exp(const double *__restrict__ adiag, int g) {
+    const double a = adiag[g];
+    if (!(a > 0.0) || !(a < 1e300)) return 0;
+    return min(max(13 - ilogb(sqrt(a)), -100), 100);
+}
+
+__device__ __forceinline__ unsigned split_pack(float v, int e) {
+    const float sv = v * ldexpf(1.0f, e);
+    const _Float16 hi = (_Float16)sv;
+    const _Float16 lo = (_Float16)(sv - (float)hi);
+    return (unsigned)__builtin_bit_cast(unsigned short, hi) |
+           ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
+}
+
 template <typename T, int MODE, int TS>
 __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile(const UpdDesc *__restrict__ descs,
                                                               const Tile *__restrict__ tiles,
                                                               T *__restrict__ W,
-                                                              const int *__restrict__ Ls) {
+                                                              const int *__restrict__ Ls,
+                                                              unsigned *__restrict__ Lh,
+                                                              const double *__restrict__ adiag) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     constexpr int NT = TS == 32 ? 64 : 256;     // threads
     constexpr int WB = TS == 32 ? 32 : TS / 2;  // edge of a wave's block of C
@@ -232,6 +251,19 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
     }
     if (MODE == MODE_STORE) __syncthreads();   // in-place TRSM: every read of the panel is done
 
+    // FP32 TRSM with Lh: the panel is also stored in split-FP16 form, each row
+    // scaled by its own exponent, for the update and trailing tiles that read it
+    int exs[MB];
+#pragma unroll
+    for (int n = 0; n < MB; n++) exs[n] = 0;
+    if (MODE == MODE_STORE && std::is_same<T, float>::value && Lh) {
+#pragma unroll
+        for (int n = 0; n < MB; n++) {
+            const int i = i0 + wr * WB + n * 16 + (lane & 15);
+            if (i < d.M) exs[n] = row_scale_exp(adiag, Ls[d.ls_off + i]);
+        }
+    }
+
     // epilogue: acc[m][n][v] = C(i, j), i = i0 + WB*wr + 16n + lane%16,
     //                                   j = j0 + WB*wc + 16m + row(lane, v)
 #pragma unroll
@@ -254,6 +286,8 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
                     W[d.c_off + (long long)j * d.ldc + i] -= c;
                 } else {
                     W[d.c_off + (long long)j * d.ldc + i] = c;
+                    if (std::is_same<T, float>::value && Lh)
+                        Lh[d.c_off + (long long)j * d.ldc + i] = split_pack((float)c, exs[n]);
                 }
             }
         }
@@ -262,29 +296,26 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
 
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
-// FP32 factor, split-FP16 products (update and trailing blocks).  A staged
-// FP32 value v of global row g is scaled by 2^s_g, s_g = 13 - floor(log2
+// FP32 factor, split-FP16 products (update and trailing blocks).  An FP32
+// value v of L in global row g is scaled by 2^s_g, s_g = 13 - floor(log2
 // sqrt(a_gg)) with a_gg the assembled diagonal of A (|L_gk| <= sqrt(a_gg), so
 // the scaled values stay below 2^14), and split into hi = fp16(v 2^s) and
-// lo = fp16(v 2^s - hi).  C = Y X' accumulates hi*hi + hi*lo + lo*hi in FP32
-// on the 16x16x32 FP16 MFMA (lo*lo lies below FP32 rounding relative to the
-// row norms) and is unscaled by 2^-(s_i + s_j) in the epilogue, which is the
-// one of k_tile.  One LDS buffer of hi / lo rows, k contiguous, row stride 40
-// halves (the 16-byte reads of a 16-row group fall in distinct banks); the
-// next chunk is prefetched into registers.
+// lo = fp16(v 2^s - hi).  The TRSM that finalizes a value writes the pair
+// once into Lh (same layout as L), so the tiles here stage the halves as they
+// are.  C = Y X' accumulates hi*hi + hi*lo + lo*hi in FP32 on the 16x16x32
+// FP16 MFMA (lo*lo lies below FP32 rounding relative to the row norms) and is
+// unscaled by 2^-(s_i + s_j) in the epilogue, which is the one of k_tile.
+// One LDS buffer of hi / lo rows, k contiguous, row stride 40 halves (the
+// 16-byte reads of a 16-row group fall in distinct banks); the next chunk is
+// prefetched into registers.
 // ---------------------------------------------------------------------------
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ int row_scale_exp(const double *__restrict__ adiag, int g) {
-    const double a = adiag[g];
-    if (!(a > 0.0) || !(a < 1e300)) return 0;
-    return min(max(13 - ilogb(sqrt(a)), -100), 100);
-}
 
 template <int MODE, int TS>
 __global__ __launch_bounds__(TS == 32 ? 64 : 256, 2) void k_tile_h(const UpdDesc *__restrict__ descs,
                                                                   const Tile *__restrict__ tiles,
                                                                   float *__restrict__ W,
+                                                                  const unsigned *__restrict__ Lh,
                                                                   const int *__restrict__ Ls,
                                                                   const double *__restrict__ adiag) {
     static_assert(TS == 32 || TS == 64 || TS == 128, "split-FP16 tiles are 32, 64 or 128 wide");
@@ -347,31 +378,29 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, 2) void k_tile_h(const UpdDesc
     // staging: thread t holds row r = t % TS, k in [kq * PER, kq * PER + PER) of a chunk
     const int r = tid % TS, kq = tid / TS;
     const bool rowy = (i0 + r) < d.M, rowx = (j0 + r) < d.N;
-    const float sy = ldexpf(1.0f, exr[r]), sx = ldexpf(1.0f, exc[r]);
-    const float *yrow = W + d.y_off + (i0 + r);
-    const float *xrow = W + d.x_off + (j0 + r);
-    float ry[PER], rx[PER];
+    const unsigned *yrow = Lh + d.y_off + (i0 + r);
+    const unsigned *xrow = Lh + d.x_off + (j0 + r);
+    unsigned ry[PER], rx[PER];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int gk = k0 + kq * PER + q;
             const bool kin = gk < d.K;
-            ry[q] = (kin && rowy) ? yrow[(long long)gk * d.ldy] : 0.0f;
-            rx[q] = (kin && rowx) ? xrow[(long long)gk * d.ldx] : 0.0f;
+            ry[q] = (kin && rowy) ? yrow[(long long)gk * d.ldy] : 0u;
+            rx[q] = (kin && rowx) ? xrow[(long long)gk * d.ldx] : 0u;
         }
     };
+    auto half_of = [](unsigned u) { return __builtin_bit_cast(_Float16, (unsigned short)(u & 0xffffu)); };
     auto stage = [&]() {
 #pragma unroll
         for (int h = 0; h < PER; h += 8) {
             half8_t yh, yl, xh, xl;
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                const float vy = ry[h + q] * sy, vx = rx[h + q] * sx;
-                const _Float16 ay = (_Float16)vy, ax = (_Float16)vx;
-                yh[q] = ay;
-                yl[q] = (_Float16)(vy - (float)ay);
-                xh[q] = ax;
-                xl[q] = (_Float16)(vx - (float)ax);
+                yh[q] = half_of(ry[h + q]);
+                yl[q] = half_of(ry[h + q] >> 16);
+                xh[q] = half_of(rx[h + q]);
+                xl[q] = half_of(rx[h + q] >> 16);
             }
             const int o = r * RS + kq * PER + h;
             *reinterpret_cast<half8_t *>(&Yh[o]) = yh;

@@ -556,6 +556,267 @@ __global__ __launch_bounds__(256) void k_sweep_small_bwd(const SweepTask *__rest
 }
 
 // ---------------------------------------------------------------------------
+// Chained sweeps: one launch per level for the supernodes that are not small.
+// Workgroup (s, t) computes block t of supernode s.  It accumulates the
+// products with the blocks it depends on as they become final and applies the
+// inverse of the diagonal block at the end.  A per-supernode counter of
+// finished blocks passes blocks between the workgroups of a launch: block
+// values and counters are read and written with device-scope atomic loads and
+// stores (performed at the device coherence point, no cache maintenance), and
+// a producer waits for its block's stores to complete (s_waitcnt) before the
+// workgroup barrier that precedes the counter store; a consumer issues its
+// block loads only after its counter load returned.  The launch order puts
+// every block after the blocks it depends on, so a workgroup waits only for
+// workgroups of lower index: the lowest unfinished workgroup never waits, and
+// workgroups start in index order, so every wait ends.  A wait is still
+// bounded (CHAIN_SPIN polls); past that it raises *abort, every wait of the
+// launch returns, and the host reports a device error instead of hanging.
+//
+// Inside a workgroup, lane l owns output row (forward) or column (backward)
+// l of the 64-wide block and wave w the slice 16w .. 16w+15 of the summed
+// dimension; the 16 x ZC values of a slice are loaded once per wave (one per
+// lane) and broadcast with readlane, and the four waves' partial sums meet in
+// LDS once per pass.
+// ---------------------------------------------------------------------------
+constexpr int CHAIN_SPIN = 1 << 22;
+
+template <typename T>
+__device__ __forceinline__ T ld_dev(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_dev(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-uniform wait until *cnt >= need.  Returns the counter value seen.
+__device__ __forceinline__ int chain_wait(const int *cnt, int need, int *abort) {
+    int c = __builtin_amdgcn_readfirstlane(ld_dev(cnt));
+    for (int it = 1; c < need; it++) {
+        __builtin_amdgcn_s_sleep(1);
+        if ((it & 63) == 0) {
+            const int ab = __builtin_amdgcn_readfirstlane(ld_dev(abort));
+            if (ab != 0 || it >= CHAIN_SPIN) {
+                if (ab == 0) st_dev(abort, 1);
+                return need;
+            }
+        }
+        c = __builtin_amdgcn_readfirstlane(ld_dev(cnt));
+    }
+    asm volatile("" ::: "memory");   // block loads stay after the counter load
+    return c;
+}
+
+// Publish block count v of a supernode once every thread's block stores are done.
+__device__ __forceinline__ void chain_publish(int *cnt, int v) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) st_dev(cnt, v);
+}
+
+__global__ void k_chain_reset(int *__restrict__ cnt, int m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) cnt[i] = 0;
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_get(T v, int l) {
+    if constexpr (sizeof(T) == 8) {
+        const long long u = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane((int)u, l);
+        const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    } else {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+    }
+}
+
+// acc[z] += sum_j l[j] q(j + 16 z), q(m) = lane m's value of q (wave-uniform)
+template <typename T, int ZC>
+__device__ __forceinline__ void fma_bcast(T (&acc)[ZC], const T (&l)[16], T q) {
+#pragma unroll
+    for (int z = 0; z < ZC; z++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc[z] += l[j] * lane_get(q, j + 16 * z);
+}
+
+// Forward.  Task fields: c0 / nb = block t (diag workgroup, pad[0] = 0) or
+// rows [r0, r1) below the supernode's columns (pad[0] = 1); p0 = counter of
+// the supernode, np = its number of blocks.
+//   diag:  x_t = inv(L_tt) (x_t - sum_{b < t} L[t, b] x_b), then publish t + 1
+//   below: x[g(r)] -= sum_b L[r, b] x_b  (ancestor rows, atomics)
+template <typename T, int ZC>
+__global__ __launch_bounds__(256) void k_chain_fwd(const SweepTask *__restrict__ tasks,
+                                                   const T *__restrict__ W,
+                                                   const int *__restrict__ Ls, T *__restrict__ X,
+                                                   long long n, int nrhs, int *__restrict__ cnt,
+                                                   int *__restrict__ abort) {
+    __shared__ T red[4][ZC][NB];
+    __shared__ T vs[ZC][NB];
+    const SweepTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const bool below = t.pad[0] != 0;
+    const int row = (below ? t.r0 : t.c0) + lane;  // the lane's row of the supernode
+    const bool rok = row < (below ? t.r1 : t.c0 + t.nb);
+    const int nblk = below ? t.np : t.c0 / NB;     // blocks summed
+    int *mycnt = cnt + t.p0;
+    const T *Lr = W + t.lx + row + (long long)(16 * w) * t.nsrow;
+    // the inverse (diag): row = lane, columns 16w + j; loaded before any wait
+    T iv[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+        iv[j] = (!below && lane < t.nb && 16 * w + j < t.nb)
+                    ? W[t.winv + lane + (long long)(16 * w + j) * t.nb] : (T)0;
+    auto load_blk = [&](T (&l)[16], int b) {
+        const int c = b * NB + 16 * w;             // first column of the wave's slice
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            l[j] = (rok && c + j < t.nscol) ? Lr[(long long)(b * NB + j) * t.nsrow] : (T)0;
+    };
+    const int qz = lane >> 4, qj = lane & 15;      // the value lane q loads: column qj, rhs qz
+    int seen = 0;
+    for (int z0 = 0; z0 < nrhs; z0 += ZC) {
+        const int zc = min(ZC, nrhs - z0);
+        T acc[ZC];
+#pragma unroll
+        for (int z = 0; z < ZC; z++) acc[z] = 0;
+        T lc[16], ln[16];
+        if (nblk > 0) load_blk(lc, 0);
+        for (int b = 0; b < nblk; b++) {
+            if (b + 1 < nblk) load_blk(ln, b + 1);
+            if (seen <= b) seen = chain_wait(mycnt, b + 1, abort);
+            const int c = b * NB + 16 * w + qj;
+            const T q = (qz < zc && c < t.nscol) ? ld_dev(X + t.k1 + c + (long long)(z0 + qz) * n) : (T)0;
+            fma_bcast<T, ZC>(acc, lc, q);
+#pragma unroll
+            for (int j = 0; j < 16; j++) lc[j] = ln[j];
+        }
+#pragma unroll
+        for (int z = 0; z < ZC; z++) red[w][z][lane] = acc[z];
+        __syncthreads();
+        if (below) {
+            for (int z = w; z < zc; z += 4)
+                if (rok) {
+                    const T sum = (red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]);
+                    atomic_add(X + Ls[t.psi + row] + (long long)(z0 + z) * n, -sum);
+                }
+            __syncthreads();                        // red is rewritten by the next pass
+            continue;
+        }
+        for (int z = w; z < ZC; z += 4) {
+            T v = 0;
+            if (rok && z < zc)
+                v = X[t.k1 + row + (long long)(z0 + z) * n] -
+                    ((red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]));
+            vs[z][lane] = v;
+        }
+        __syncthreads();
+        // x_t = inv(L_tt) v: row = lane, wave w sums the columns 16w .. 16w+15
+#pragma unroll
+        for (int z = 0; z < ZC; z++) {
+            T a = 0;
+#pragma unroll
+            for (int j = 0; j < 16; j++) a += iv[j] * vs[z][16 * w + j];
+            red[w][z][lane] = a;
+        }
+        __syncthreads();
+        for (int z = w; z < zc; z += 4)
+            if (rok)
+                st_dev(X + t.k1 + row + (long long)(z0 + z) * n,
+                       (red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]));
+        __syncthreads();
+    }
+    if (!below) chain_publish(mycnt, t.c0 / NB + 1);
+}
+
+// Backward.  Task fields: c0 / nb = block t, p0 = counter of the supernode,
+// np = its number of blocks.
+//   x_t = inv(L_tt)' (x_t - L[below, t]' x[g(below)] - sum_{b > t} L[b, t]' x_b),
+// then publish np - t (blocks finished, counted from the last).
+template <typename T, int ZC>
+__global__ __launch_bounds__(256) void k_chain_bwd(const SweepTask *__restrict__ tasks,
+                                                   const T *__restrict__ W,
+                                                   const int *__restrict__ Ls, T *__restrict__ X,
+                                                   long long n, int nrhs, int *__restrict__ cnt,
+                                                   int *__restrict__ abort) {
+    __shared__ T red[4][ZC][NB];
+    __shared__ T vs[ZC][NB];
+    const SweepTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nblk = t.np, tb = t.c0 / NB;
+    const int col = t.c0 + lane;                   // the lane's column of the supernode
+    const bool cok = lane < t.nb;
+    int *mycnt = cnt + t.p0;
+    const T *Lc = W + t.lx + (long long)col * t.nsrow + 16 * w;
+    // the inverse: column = lane, rows 16w + j (inv[k][c] at winv + k + c nb)
+    T iv[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+        iv[j] = (cok && 16 * w + j < t.nb) ? W[t.winv + 16 * w + j + (long long)lane * t.nb] : (T)0;
+    // rows r + 16w .. r + 16w + 15 of the lane's column (zero at or past rend)
+    auto load_rows = [&](T (&l)[16], int r, int rend) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) l[j] = (cok && r + 16 * w + j < rend) ? Lc[r + j] : (T)0;
+    };
+    const int qz = lane >> 4, qj = lane & 15;      // the value lane q loads: row qj, rhs qz
+    int seen = 0;
+    for (int z0 = 0; z0 < nrhs; z0 += ZC) {
+        const int zc = min(ZC, nrhs - z0);
+        T acc[ZC];
+#pragma unroll
+        for (int z = 0; z < ZC; z++) acc[z] = 0;
+        T lc[16], ln[16];
+        // rows below the supernode's columns: ancestor rows, final before the launch
+        if (t.nscol < t.nsrow) load_rows(lc, t.nscol, t.nsrow);
+        for (int r = t.nscol; r < t.nsrow; r += NB) {
+            if (r + NB < t.nsrow) load_rows(ln, r + NB, t.nsrow);
+            const int rq = r + 16 * w + qj;
+            const T q = (qz < zc && rq < t.nsrow) ? X[Ls[t.psi + rq] + (long long)(z0 + qz) * n] : (T)0;
+            fma_bcast<T, ZC>(acc, lc, q);
+#pragma unroll
+            for (int j = 0; j < 16; j++) lc[j] = ln[j];
+        }
+        // blocks nblk-1 .. tb+1 of the supernode's own rows, as they are finished
+        if (tb + 1 < nblk) load_rows(lc, (nblk - 1) * NB, t.nscol);
+        for (int b = nblk - 1; b > tb; b--) {
+            if (b - 1 > tb) load_rows(ln, (b - 1) * NB, t.nscol);
+            if (seen < nblk - b) seen = chain_wait(mycnt, nblk - b, abort);
+            const int rq = b * NB + 16 * w + qj;
+            const T q = (qz < zc && rq < t.nscol) ? ld_dev(X + t.k1 + rq + (long long)(z0 + qz) * n) : (T)0;
+            fma_bcast<T, ZC>(acc, lc, q);
+#pragma unroll
+            for (int j = 0; j < 16; j++) lc[j] = ln[j];
+        }
+#pragma unroll
+        for (int z = 0; z < ZC; z++) red[w][z][lane] = acc[z];
+        __syncthreads();
+        for (int z = w; z < ZC; z += 4) {
+            T v = 0;
+            if (cok && z < zc)
+                v = X[t.k1 + col + (long long)(z0 + z) * n] -
+                    ((red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]));
+            vs[z][lane] = v;
+        }
+        __syncthreads();
+        // x_t = inv(L_tt)' v: column = lane, wave w sums the rows 16w .. 16w+15
+#pragma unroll
+        for (int z = 0; z < ZC; z++) {
+            T a = 0;
+#pragma unroll
+            for (int j = 0; j < 16; j++) a += iv[j] * vs[z][16 * w + j];
+            red[w][z][lane] = a;
+        }
+        __syncthreads();
+        for (int z = w; z < zc; z += 4)
+            if (cok)
+                st_dev(X + t.k1 + col + (long long)(z0 + z) * n,
+                       (red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]));
+        __syncthreads();
+    }
+    chain_publish(mycnt, nblk - tb);
+}
+
+// ---------------------------------------------------------------------------
 // FP64 residual R = B - A X with A held as full CSR (both triangles), values
 // gathered from tril(A) by src.
 // ---------------------------------------------------------------------------

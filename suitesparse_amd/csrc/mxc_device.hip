@@ -111,9 +111,10 @@ static inline int kind_of(int launch_kind) {
     return launch_kind == 0 ? 0 : (launch_kind == 3 || launch_kind == 9) ? 1 : 2;
 }
 // L_TRAIL2: the part of a panel's trailing update right of the next panel, on
-// the second stream; L_FORK / L_JOIN: the event edges between the two streams.
+// the second stream; L_FORK / L_JOIN: the event edges between the two streams;
+// L_FCHAIN / L_BCHAIN: a level's chained sweep launch.
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7,
-       L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11 };
+       L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11, L_FCHAIN = 12, L_BCHAIN = 13 };
 
 struct StepBlocks {
     std::vector<DiagTask> diags;
@@ -134,6 +135,7 @@ struct mxc_factor {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool lookahead = true;
     bool split16 = true;                 // FP32 factors: split-FP16 update/trailing tiles
+    bool chain = true;                   // solves: one chained launch per level
     size_t bytes = 0;
 
     // host copies
@@ -142,6 +144,7 @@ struct mxc_factor {
 
     // device
     void *W = nullptr;                  // T[wsize]
+    unsigned *d_Lh = nullptr;           // FP32 factors: split-FP16 pairs of L (xsize)
     int *d_Ls = nullptr;
     long long *d_pos = nullptr, *d_dpos = nullptr, *d_perm = nullptr;
     UpdDesc *d_desc = nullptr;
@@ -157,12 +160,16 @@ struct mxc_factor {
     double *d_beta = nullptr;
     void *d_work = nullptr;             // T[n * work_nrhs] solve vector
     void *d_part = nullptr;             // T[nslots * 64 * work_nrhs] backward partial sums
+    int *d_cnt = nullptr;               // chained sweeps: finished blocks per supernode (fwd, bwd)
+    int *d_abort = nullptr;             // chained sweeps: a wait gave up
+    int *h_abort = nullptr;             // pinned host copy of *d_abort
     int64_t nslots = 0;
     double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr;   // FP64 n * work_nrhs
     int64_t work_nrhs = 0;
 
     std::vector<Launch> factor_launches;   // per level: UPD, then (DIAG, TRSM, TRAIL)*
     std::vector<Launch> fwd_launches, bwd_launches;
+    std::vector<Launch> fwd_chain, bwd_chain;   // chained sweeps: per level small + chain
     bool values_loaded = false;
     int64_t minor = 0;                     // L->minor of the last factorization
     int64_t nperturbed = 0;                // pivots raised by the low-precision safeguard
@@ -517,6 +524,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.N = (int)nb;
                     u.c_off = u.y_off;
                     u.ldc = (int)nsrow;
+                    u.ls_off = (int)(S->Lpi[s] + r0);   // global rows (split-FP16 copy)
                     sb.trsm.push_back(u);
                     F->flops_trsm += (double)u.M * nb * nb;
                 }
@@ -674,6 +682,60 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     }
     F->nslots = nslot;
 
+    // ---- chained sweeps: per level, the small supernodes' launch, then one
+    // launch whose workgroups are the blocks of the other supernodes in an
+    // order where every block follows the blocks it depends on (forward: by
+    // block index, then the rows below each supernode; backward: by distance
+    // from the supernode's last block).  p0 = the supernode's counter.
+    F->fwd_chain.clear();
+    F->bwd_chain.clear();
+    for (int l = 0; l < nlev; l++) {
+        push_small(l, F->fwd_chain, L_FSMALL);
+        int64_t maxsteps = 0;
+        for (int64_t s : by_level[l])
+            if (!is_small(s)) maxsteps = std::max(maxsteps, nsteps(s));
+        long long b = (long long)st.size();
+        for (int64_t t = 0; t < maxsteps; t++)
+            for (int64_t s : by_level[l])
+                if (!is_small(s) && t < nsteps(s)) {
+                    SweepTask x = make_task(s, t);
+                    x.p0 = (int)s;
+                    x.np = (int)nsteps(s);
+                    st.push_back(x);
+                }
+        for (int64_t s : by_level[l]) {
+            if (is_small(s)) continue;
+            int64_t nscol = S->Super[s + 1] - S->Super[s];
+            int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
+            for (int64_t r = nscol; r < nsrow; r += NB) {
+                SweepTask x = make_task(s, 0);
+                x.r0 = (int)r;
+                x.r1 = (int)std::min<int64_t>(nsrow, r + NB);
+                x.p0 = (int)s;
+                x.np = (int)nsteps(s);
+                x.pad[0] = 1;
+                st.push_back(x);
+            }
+        }
+        if ((long long)st.size() > b) F->fwd_chain.push_back(Launch{L_FCHAIN, b, (long long)st.size() - b, 0});
+    }
+    for (int l = nlev - 1; l >= 0; l--) {
+        push_small(l, F->bwd_chain, L_BSMALL);
+        int64_t maxsteps = 0;
+        for (int64_t s : by_level[l])
+            if (!is_small(s)) maxsteps = std::max(maxsteps, nsteps(s));
+        long long b = (long long)st.size();
+        for (int64_t k = 0; k < maxsteps; k++)
+            for (int64_t s : by_level[l])
+                if (!is_small(s) && k < nsteps(s)) {
+                    SweepTask x = make_task(s, nsteps(s) - 1 - k);
+                    x.p0 = (int)(ns + s);
+                    x.np = (int)nsteps(s);
+                    st.push_back(x);
+                }
+        if ((long long)st.size() > b) F->bwd_chain.push_back(Launch{L_BCHAIN, b, (long long)st.size() - b, 0});
+    }
+
     // ---- device upload
     std::vector<int> Ls32(S->ssize);
     for (int64_t k = 0; k < S->ssize; k++) Ls32[k] = (int)S->Ls[k];
@@ -681,6 +743,8 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     hipStream_t sm = F->stream;
     int rc;
     if ((rc = dalloc(F, (T **)&F->W, (size_t)F->wsize))) return rc;
+    if (std::is_same<T, float>::value)
+        if ((rc = dalloc(F, &F->d_Lh, (size_t)F->xsize))) return rc;
     if ((rc = dupload(F, &F->d_Ls, Ls32, sm))) return rc;
     if ((rc = dupload(F, &F->d_pos, pos, sm))) return rc;
     if ((rc = dupload(F, &F->d_dpos, dpos, sm))) return rc;
@@ -699,6 +763,11 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     F->Ai_host.assign(Ai, Ai + nnz);
     if ((rc = dalloc(F, &F->d_norm, 64))) return rc;
     if ((rc = dalloc(F, &F->d_beta, 1))) return rc;
+    if ((rc = dalloc(F, &F->d_cnt, (size_t)(2 * ns)))) return rc;
+    if ((rc = dalloc(F, &F->d_abort, 1))) return rc;
+    HIPTRY(hipMemsetAsync(F->d_abort, 0, sizeof(int), sm));
+    HIPTRY(hipHostMalloc((void **)&F->h_abort, sizeof(int), hipHostMallocDefault));
+    *F->h_abort = 0;
     HIPTRY(hipStreamSynchronize(sm));
     return MXC_OK;
 }
@@ -735,11 +804,13 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
     drop_sweep_graphs(F);
     drop_factor_graph(F);
     if (F->fallback) mxc_factor_free(F->fallback);
-    void *ptrs[] = {F->W, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
+    void *ptrs[] = {F->W, F->d_Lh, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
                     F->d_diag, F->d_stask, F->d_fail, F->d_adiag, F->d_rowptr, F->d_src, F->d_col,
-                    F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_part, F->d_b, F->d_x, F->d_r};
+                    F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_part, F->d_b, F->d_x, F->d_r,
+                    F->d_cnt, F->d_abort};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (F->h_abort) (void)hipHostFree(F->h_abort);
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
     F->ev.clear();
     if (F->ev_fork) (void)hipEventDestroy(F->ev_fork);
@@ -877,30 +948,31 @@ static inline unsigned grid_for(long long n, int bs, long long cap = 65536) {
 template <typename T, int MODE>
 static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = nullptr) {
     if (!st) st = F->stream;
+    const bool h16 = std::is_same<T, float>::value && F->split16 && F->d_Lh;
+    const double *adiag = F->d_adiag;
     if constexpr (std::is_same<T, float>::value && MODE != MODE_STORE) {
-        // FP32 factor: split-FP16 products on the FP16 MFMA
-        if (F->split16) {
-            if (L.tile == 32)
-                hipLaunchKernelGGL((k_tile_h<MODE, 32>), dim3((unsigned)L.count), dim3(64), 0, st,
-                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, (const double *)F->d_adiag);
-            else if (L.tile == 64)
+        // FP32 factor: split-FP16 products on the FP16 MFMA, operands from the
+        // pairs the TRSM wrote (32-wide tiles are latency-bound: FP32 MFMA)
+        if (h16 && L.tile != 32) {
+            if (L.tile == 64)
                 hipLaunchKernelGGL((k_tile_h<MODE, 64>), dim3((unsigned)L.count), dim3(256), 0, st,
-                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, (const double *)F->d_adiag);
+                                   F->d_desc, F->d_tiles + L.begin, W, (const unsigned *)F->d_Lh, F->d_Ls, adiag);
             else
                 hipLaunchKernelGGL((k_tile_h<MODE, 128>), dim3((unsigned)L.count), dim3(256), 0, st,
-                                   F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, (const double *)F->d_adiag);
+                                   F->d_desc, F->d_tiles + L.begin, W, (const unsigned *)F->d_Lh, F->d_Ls, adiag);
             return;
         }
     }
+    unsigned *Lh = (MODE == MODE_STORE && h16) ? F->d_Lh : nullptr;
     if (L.tile == 32)
         hipLaunchKernelGGL((k_tile<T, MODE, 32>), dim3((unsigned)L.count), dim3(64), 0, st,
-                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, Lh, adiag);
     else if (L.tile == 64)
         hipLaunchKernelGGL((k_tile<T, MODE, 64>), dim3((unsigned)L.count), dim3(256), 0, st,
-                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, Lh, adiag);
     else
         hipLaunchKernelGGL((k_tile<T, MODE, 128>), dim3((unsigned)L.count), dim3(256), 0, st,
-                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls);
+                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, Lh, adiag);
 }
 
 // Enqueue one whole factorization (values already in d_Ax, beta in d_beta).
@@ -1160,6 +1232,49 @@ extern "C" int mxc_factor_get_Lx(const mxc_factor *F, double *Lx) {
 // ---------------------------------------------------------------------------
 // solves
 // ---------------------------------------------------------------------------
+// A level's chained sweep launch; right-hand sides per pass 1, 2 or 4.
+template <typename T, bool FWD>
+static void launch_chain(mxc_factor *F, const Launch &L, T *Xw, int nr) {
+    hipStream_t sm = F->stream;
+    const T *W = (const T *)F->W;
+    const SweepTask *tk = F->d_stask + L.begin;
+    const dim3 g((unsigned)L.count), b(256);
+    const long long n = (long long)F->n;
+    if (FWD) {
+        if (nr == 1)
+            hipLaunchKernelGGL((k_chain_fwd<T, 1>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
+        else if (nr == 2)
+            hipLaunchKernelGGL((k_chain_fwd<T, 2>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
+        else
+            hipLaunchKernelGGL((k_chain_fwd<T, 4>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
+    } else {
+        if (nr == 1)
+            hipLaunchKernelGGL((k_chain_bwd<T, 1>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
+        else if (nr == 2)
+            hipLaunchKernelGGL((k_chain_bwd<T, 2>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
+        else
+            hipLaunchKernelGGL((k_chain_bwd<T, 4>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
+    }
+}
+
+// Chained sweeps: fetch the abort flag behind the stream's work (read after
+// the caller's synchronization by chain_status).
+static int chain_fetch(mxc_factor *F) {
+    if (F->chain) HIPTRY(hipMemcpyAsync(F->h_abort, F->d_abort, sizeof(int), hipMemcpyDeviceToHost, F->stream));
+    return MXC_OK;
+}
+
+// After a synchronization: a chained sweep whose wait gave up is a device
+// error (the flag is cleared for the next call).
+static int chain_status(mxc_factor *F) {
+    if (!*F->h_abort) return MXC_OK;
+    *F->h_abort = 0;
+    HIPTRY(hipMemsetAsync(F->d_abort, 0, sizeof(int), F->stream));
+    HIPTRY(hipStreamSynchronize(F->stream));
+    snprintf(g_err, sizeof(g_err), "chained sweep: a wait for a block did not end");
+    return MXC_GPU_PROBLEM;
+}
+
 template <typename T>
 static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
     hipStream_t sm = F->stream;
@@ -1167,12 +1282,18 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     size_t tt = 0;
     int n = (int)F->n;
     int nr = (int)nrhs;
+    const bool chain = F->chain;
+    const int ns = (int)F->nsuper;
     if (fwd) {
         tick(F, 5, &tt);
-        for (const Launch &L : F->fwd_launches) {
+        if (chain && ns > 0)
+            hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(ns, 256, 1LL << 30)), dim3(256), 0, sm, F->d_cnt, ns);
+        for (const Launch &L : chain ? F->fwd_chain : F->fwd_launches) {
             if (L.kind == L_FSMALL)
                 hipLaunchKernelGGL(k_sweep_small_fwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,
                                    sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xw, (long long)n, nr);
+            else if (L.kind == L_FCHAIN)
+                launch_chain<T, true>(F, L, Xw, nr);
             else
                 hipLaunchKernelGGL(k_sweep_fwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
                                    F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (long long)n, nr);
@@ -1182,10 +1303,15 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     }
     if (bwd) {
         tick(F, 6, &tt);
-        for (const Launch &L : F->bwd_launches) {
+        if (chain && ns > 0)
+            hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(ns, 256, 1LL << 30)), dim3(256), 0, sm,
+                               F->d_cnt + ns, ns);
+        for (const Launch &L : chain ? F->bwd_chain : F->bwd_launches) {
             if (L.kind == L_BSMALL)
                 hipLaunchKernelGGL(k_sweep_small_bwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,
                                    sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xw, (long long)n, nr);
+            else if (L.kind == L_BCHAIN)
+                launch_chain<T, false>(F, L, Xw, nr);
             else
                 hipLaunchKernelGGL(k_sweep_bwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
                                    F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (T *)F->d_part,
@@ -1273,8 +1399,9 @@ extern "C" int mxc_solve(mxc_factor *F, const double *B, int64_t nrhs, double *X
     if (rc) return rc;
     if (location != MXC_DEVICE)
         HIPTRY(hipMemcpyAsync(X, dX, F->n * nrhs * sizeof(double), hipMemcpyDeviceToHost, F->stream));
+    if ((rc = chain_fetch(F))) return rc;
     HIPTRY(hipStreamSynchronize(F->stream));
-    return MXC_OK;
+    return chain_status(F);
 }
 
 static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, bool fwd) {
@@ -1291,6 +1418,7 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
     double *dX = location == MXC_DEVICE ? X : F->d_x;
     if (F->precision == MXC_FP64) {
         rc = run_sweep<double>(F, dX, nrhs, fwd, !fwd);
+        if (rc == MXC_OK) rc = chain_fetch(F);
     } else {
         float *Xw = (float *)F->d_work;
         std::vector<float> tmp;
@@ -1300,6 +1428,7 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
         tmp.assign(h.begin(), h.end());
         HIPTRY(hipMemcpyAsync(Xw, tmp.data(), len * sizeof(float), hipMemcpyHostToDevice, sm));
         rc = run_sweep<float>(F, Xw, nrhs, fwd, !fwd);
+        if (rc == MXC_OK) rc = chain_fetch(F);
         if (rc) return rc;
         HIPTRY(hipMemcpyAsync(tmp.data(), Xw, len * sizeof(float), hipMemcpyDeviceToHost, sm));
         HIPTRY(hipStreamSynchronize(sm));
@@ -1310,7 +1439,7 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
     if (location != MXC_DEVICE)
         HIPTRY(hipMemcpyAsync(X, dX, len * sizeof(double), hipMemcpyDeviceToHost, sm));
     HIPTRY(hipStreamSynchronize(sm));
-    return MXC_OK;
+    return chain_status(F);
 }
 
 extern "C" int mxc_super_lsolve(mxc_factor *F, double *X, int64_t nrhs, int location) {
@@ -1404,12 +1533,14 @@ static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *
     unsigned g = grid_for(F->n, 256, 1LL << 30);
     hipLaunchKernelGGL(k_perm_gather<T>, dim3(g), dim3(256), 0, sm, dB, F->d_perm, (long long)F->n, (int)nrhs, Xw);
     if ((rc = run_sweep<T>(F, Xw, nrhs, true, true))) return rc;
+    if ((rc = chain_fetch(F))) return rc;
     hipLaunchKernelGGL(k_perm_scatter<T>, dim3(g), dim3(256), 0, sm, Xw, F->d_perm, (long long)F->n, (int)nrhs, dX);
     int it = 0;
     double rel = 0, rel0 = -1;
     for (;;) {
         launch_residual(F, dB, dX, F->d_r, nrhs);
         if ((rc = colnorms(F, F->d_r, nrhs, nseg, d_seg, d_out, rn))) return rc;
+        if ((rc = chain_status(F))) return rc;   // colnorms synchronized the stream
         rel = 0;
         for (size_t q = 0; q < rn.size(); q++) {
             double v = bn[q] > 0 ? rn[q] / bn[q] : rn[q];
@@ -1420,6 +1551,7 @@ static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *
         // d = A \ r ; x += d
         hipLaunchKernelGGL(k_perm_gather<T>, dim3(g), dim3(256), 0, sm, F->d_r, F->d_perm, (long long)F->n, (int)nrhs, Xw);
         if ((rc = run_sweep<T>(F, Xw, nrhs, true, true))) return rc;
+        if ((rc = chain_fetch(F))) return rc;
         hipLaunchKernelGGL(k_perm_scatter<T>, dim3(g), dim3(256), 0, sm, Xw, F->d_perm, (long long)F->n, (int)nrhs, F->d_r);
         hipLaunchKernelGGL(k_axpy_update<double>, dim3(grid_for((long long)len, 256)), dim3(256), 0, sm, dX,
                            (const double *)F->d_r, (long long)len);
@@ -1652,6 +1784,10 @@ extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
         case MXC_OPT_SPLIT16:
             F->split16 = value != 0;
             drop_factor_graph(F);
+            return MXC_OK;
+        case MXC_OPT_SOLVE_CHAIN:
+            F->chain = value != 0;
+            drop_sweep_graphs(F);
             return MXC_OK;
         default:
             return MXC_INVALID;
