@@ -729,30 +729,37 @@ __global__ __launch_bounds__(256) void k_chain_fwd(const SweepTask *__restrict__
     if (!below) chain_publish(mycnt, t.c0 / NB + 1);
 }
 
-// Backward.  Task fields: c0 / nb = block t, p0 = counter of the supernode,
-// np = its number of blocks.
-//   x_t = inv(L_tt)' (x_t - L[below, t]' x[g(below)] - sum_{b > t} L[b, t]' x_b),
-// then publish np - t (blocks finished, counted from the last).
+// Backward.  Below workgroups (pad[0] = 1) come first in a launch: rows
+// [r0, r1) below the supernode's columns times block t = c0 / NB,
+//   part[np] = L[r0:r1, t]' x[g(r0:r1)]   (one slot; ancestor rows, final)
+// then counter p0 += 1.  Diag workgroups (pad[0] = 0): p0 = counter of the
+// supernode's blocks, np = its number of blocks, slots [q0, q0 + nq) of block
+// t complete once counter bc reaches bt:
+//   x_t = inv(L_tt)' (x_t - sum of slots - sum_{b > t} L[b, t]' x_b),
+// then publish np - t (blocks finished, counted from the last).  Slots are
+// written, not accumulated, and summed in a fixed order: the sweep is
+// deterministic.
 template <typename T, int ZC>
 __global__ __launch_bounds__(256) void k_chain_bwd(const SweepTask *__restrict__ tasks,
                                                    const T *__restrict__ W,
                                                    const int *__restrict__ Ls, T *__restrict__ X,
-                                                   long long n, int nrhs, int *__restrict__ cnt,
-                                                   int *__restrict__ abort) {
+                                                   T *__restrict__ part, long long n, int nrhs,
+                                                   int *__restrict__ cnt, int *__restrict__ abort) {
     __shared__ T red[4][ZC][NB];
     __shared__ T vs[ZC][NB];
     const SweepTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const bool below = t.pad[0] != 0;
     const int nblk = t.np, tb = t.c0 / NB;
     const int col = t.c0 + lane;                   // the lane's column of the supernode
     const bool cok = lane < t.nb;
-    int *mycnt = cnt + t.p0;
     const T *Lc = W + t.lx + (long long)col * t.nsrow + 16 * w;
-    // the inverse: column = lane, rows 16w + j (inv[k][c] at winv + k + c nb)
+    // the inverse (diag): column = lane, rows 16w + j (inv[k][c] at winv + k + c nb)
     T iv[16];
 #pragma unroll
     for (int j = 0; j < 16; j++)
-        iv[j] = (cok && 16 * w + j < t.nb) ? W[t.winv + 16 * w + j + (long long)lane * t.nb] : (T)0;
+        iv[j] = (!below && cok && 16 * w + j < t.nb) ? W[t.winv + 16 * w + j + (long long)lane * t.nb]
+                                                     : (T)0;
     // rows r + 16w .. r + 16w + 15 of the lane's column (zero at or past rend)
     auto load_rows = [&](T (&l)[16], int r, int rend) {
 #pragma unroll
@@ -760,27 +767,49 @@ __global__ __launch_bounds__(256) void k_chain_bwd(const SweepTask *__restrict__
     };
     const int qz = lane >> 4, qj = lane & 15;      // the value lane q loads: row qj, rhs qz
     int seen = 0;
+    bool slots_ready = false;
     for (int z0 = 0; z0 < nrhs; z0 += ZC) {
         const int zc = min(ZC, nrhs - z0);
         T acc[ZC];
 #pragma unroll
         for (int z = 0; z < ZC; z++) acc[z] = 0;
         T lc[16], ln[16];
-        // rows below the supernode's columns: ancestor rows, final before the launch
-        if (t.nscol < t.nsrow) load_rows(lc, t.nscol, t.nsrow);
-        for (int r = t.nscol; r < t.nsrow; r += NB) {
-            if (r + NB < t.nsrow) load_rows(ln, r + NB, t.nsrow);
-            const int rq = r + 16 * w + qj;
-            const T q = (qz < zc && rq < t.nsrow) ? X[Ls[t.psi + rq] + (long long)(z0 + qz) * n] : (T)0;
-            fma_bcast<T, ZC>(acc, lc, q);
+        if (below) {
+            // rows [r0, r1): ancestor rows, final before the launch
+            load_rows(lc, t.r0, t.r1);
+            for (int r = t.r0; r < t.r1; r += NB) {
+                if (r + NB < t.r1) load_rows(ln, r + NB, t.r1);
+                const int rq = r + 16 * w + qj;
+                const T q = (qz < zc && rq < t.r1) ? X[Ls[t.psi + rq] + (long long)(z0 + qz) * n] : (T)0;
+                fma_bcast<T, ZC>(acc, lc, q);
 #pragma unroll
-            for (int j = 0; j < 16; j++) lc[j] = ln[j];
+                for (int j = 0; j < 16; j++) lc[j] = ln[j];
+            }
+#pragma unroll
+            for (int z = 0; z < ZC; z++) red[w][z][lane] = acc[z];
+            __syncthreads();
+            for (int z = w; z < zc; z += 4)
+                st_dev(part + ((long long)t.np * nrhs + z0 + z) * NB + lane,
+                       (red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]));
+            __syncthreads();
+            continue;
+        }
+        // the slots of the rows below block t: wave w takes slots w, w + 4, ...
+        if (t.nq > 0) {
+            if (!slots_ready) {
+                chain_wait(cnt + t.bc, t.bt, abort);
+                slots_ready = true;
+            }
+            for (int k = w; k < t.nq; k += 4)
+#pragma unroll
+                for (int z = 0; z < ZC; z++)
+                    if (z < zc) acc[z] += ld_dev(part + ((long long)(t.q0 + k) * nrhs + z0 + z) * NB + lane);
         }
         // blocks nblk-1 .. tb+1 of the supernode's own rows, as they are finished
         if (tb + 1 < nblk) load_rows(lc, (nblk - 1) * NB, t.nscol);
         for (int b = nblk - 1; b > tb; b--) {
             if (b - 1 > tb) load_rows(ln, (b - 1) * NB, t.nscol);
-            if (seen < nblk - b) seen = chain_wait(mycnt, nblk - b, abort);
+            if (seen < nblk - b) seen = chain_wait(cnt + t.p0, nblk - b, abort);
             const int rq = b * NB + 16 * w + qj;
             const T q = (qz < zc && rq < t.nscol) ? ld_dev(X + t.k1 + rq + (long long)(z0 + qz) * n) : (T)0;
             fma_bcast<T, ZC>(acc, lc, q);
@@ -813,7 +842,13 @@ __global__ __launch_bounds__(256) void k_chain_bwd(const SweepTask *__restrict__
                        (red[0][z][lane] + red[1][z][lane]) + (red[2][z][lane] + red[3][z][lane]));
         __syncthreads();
     }
-    chain_publish(mycnt, nblk - tb);
+    if (below) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_fetch_add(cnt + t.p0, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        chain_publish(cnt + t.p0, nblk - tb);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -680,7 +680,6 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                 F->bwd_launches.push_back(Launch{L_BSTEP, b, (long long)st.size() - b, ndiag});
         }
     }
-    F->nslots = nslot;
 
     // ---- chained sweeps: per level, the small supernodes' launch, then one
     // launch whose workgroups are the blocks of the other supernodes in an
@@ -719,22 +718,54 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
         }
         if ((long long)st.size() > b) F->fwd_chain.push_back(Launch{L_FCHAIN, b, (long long)st.size() - b, 0});
     }
+    // backward: the below workgroups of a level first (rows below each block,
+    // CHAIN_ROWS at a time, one slot each; slots numbered per level), then the
+    // blocks.  Counters: [ns, 2 ns) blocks, [2 ns, 3 ns) below workgroups done.
+    const int CHAIN_ROWS = 256;
     for (int l = nlev - 1; l >= 0; l--) {
         push_small(l, F->bwd_chain, L_BSMALL);
         int64_t maxsteps = 0;
         for (int64_t s : by_level[l])
             if (!is_small(s)) maxsteps = std::max(maxsteps, nsteps(s));
         long long b = (long long)st.size();
-        for (int64_t k = 0; k < maxsteps; k++)
-            for (int64_t s : by_level[l])
-                if (!is_small(s) && k < nsteps(s)) {
-                    SweepTask x = make_task(s, nsteps(s) - 1 - k);
-                    x.p0 = (int)(ns + s);
-                    x.np = (int)nsteps(s);
+        int slot = 0;
+        std::vector<int> first_slot(by_level[l].size(), 0), nchunk(by_level[l].size(), 0);
+        for (size_t q = 0; q < by_level[l].size(); q++) {
+            const int64_t s = by_level[l][q];
+            if (is_small(s)) continue;
+            int64_t nscol = S->Super[s + 1] - S->Super[s];
+            int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
+            nchunk[q] = (int)ceil_div(nsrow - nscol, CHAIN_ROWS);
+            first_slot[q] = slot;
+            for (int64_t t = 0; t < nsteps(s); t++)
+                for (int64_t r = nscol; r < nsrow; r += CHAIN_ROWS) {
+                    SweepTask x = make_task(s, t);
+                    x.r0 = (int)r;
+                    x.r1 = (int)std::min<int64_t>(nsrow, r + CHAIN_ROWS);
+                    x.p0 = (int)(2 * ns + s);
+                    x.np = slot++;
+                    x.pad[0] = 1;
                     st.push_back(x);
                 }
+        }
+        nslot = std::max<int>(nslot, slot);
+        for (int64_t k = 0; k < maxsteps; k++)
+            for (size_t q = 0; q < by_level[l].size(); q++) {
+                const int64_t s = by_level[l][q];
+                if (is_small(s) || k >= nsteps(s)) continue;
+                const int64_t t = nsteps(s) - 1 - k;
+                SweepTask x = make_task(s, t);
+                x.p0 = (int)(ns + s);
+                x.np = (int)nsteps(s);
+                x.q0 = first_slot[q] + (int)t * nchunk[q];
+                x.nq = nchunk[q];
+                x.bc = (int)(2 * ns + s);
+                x.bt = (int)nsteps(s) * nchunk[q];
+                st.push_back(x);
+            }
         if ((long long)st.size() > b) F->bwd_chain.push_back(Launch{L_BCHAIN, b, (long long)st.size() - b, 0});
     }
+    F->nslots = nslot;
 
     // ---- device upload
     std::vector<int> Ls32(S->ssize);
@@ -763,7 +794,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     F->Ai_host.assign(Ai, Ai + nnz);
     if ((rc = dalloc(F, &F->d_norm, 64))) return rc;
     if ((rc = dalloc(F, &F->d_beta, 1))) return rc;
-    if ((rc = dalloc(F, &F->d_cnt, (size_t)(2 * ns)))) return rc;
+    if ((rc = dalloc(F, &F->d_cnt, (size_t)(3 * ns)))) return rc;
     if ((rc = dalloc(F, &F->d_abort, 1))) return rc;
     HIPTRY(hipMemsetAsync(F->d_abort, 0, sizeof(int), sm));
     HIPTRY(hipHostMalloc((void **)&F->h_abort, sizeof(int), hipHostMallocDefault));
@@ -1249,11 +1280,14 @@ static void launch_chain(mxc_factor *F, const Launch &L, T *Xw, int nr) {
             hipLaunchKernelGGL((k_chain_fwd<T, 4>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
     } else {
         if (nr == 1)
-            hipLaunchKernelGGL((k_chain_bwd<T, 1>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
+            hipLaunchKernelGGL((k_chain_bwd<T, 1>), g, b, 0, sm, tk, W, F->d_Ls, Xw, (T *)F->d_part, n, nr,
+                               F->d_cnt, F->d_abort);
         else if (nr == 2)
-            hipLaunchKernelGGL((k_chain_bwd<T, 2>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
+            hipLaunchKernelGGL((k_chain_bwd<T, 2>), g, b, 0, sm, tk, W, F->d_Ls, Xw, (T *)F->d_part, n, nr,
+                               F->d_cnt, F->d_abort);
         else
-            hipLaunchKernelGGL((k_chain_bwd<T, 4>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
+            hipLaunchKernelGGL((k_chain_bwd<T, 4>), g, b, 0, sm, tk, W, F->d_Ls, Xw, (T *)F->d_part, n, nr,
+                               F->d_cnt, F->d_abort);
     }
 }
 
@@ -1304,8 +1338,8 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     if (bwd) {
         tick(F, 6, &tt);
         if (chain && ns > 0)
-            hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(ns, 256, 1LL << 30)), dim3(256), 0, sm,
-                               F->d_cnt + ns, ns);
+            hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(2 * ns, 256, 1LL << 30)), dim3(256), 0, sm,
+                               F->d_cnt + ns, 2 * ns);
         for (const Launch &L : chain ? F->bwd_chain : F->bwd_launches) {
             if (L.kind == L_BSMALL)
                 hipLaunchKernelGGL(k_sweep_small_bwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,

@@ -69,6 +69,9 @@ struct SweepTask {
     int r0, r1;
     int p0, np;
     int pad[2];
+    // chained backward sweep: the slots [q0, q0 + nq) of the rows below block t,
+    // complete once counter bc reaches bt
+    int q0, nq, bc, bt;
 };
 
 enum { MODE_SCATTER = 0, MODE_TRAIL = 1, MODE_STORE = 2 };
