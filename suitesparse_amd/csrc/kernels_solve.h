@@ -579,6 +579,7 @@ __global__ __launch_bounds__(256) void k_sweep_small_bwd(const SweepTask *__rest
 // LDS once per pass.
 // ---------------------------------------------------------------------------
 constexpr int CHAIN_SPIN = 1 << 22;
+constexpr int CHAIN_RHS = 4;   // columns per launch (the host splits wider right-hand sides)
 
 template <typename T>
 __device__ __forceinline__ T ld_dev(const T *p) {

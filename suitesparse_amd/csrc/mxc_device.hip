@@ -162,7 +162,6 @@ struct mxc_factor {
     void *d_part = nullptr;             // T[nslots * 64 * work_nrhs] backward partial sums
     int *d_cnt = nullptr;               // chained sweeps: finished blocks per supernode (fwd, bwd)
     int *d_abort = nullptr;             // chained sweeps: a wait gave up
-    int *h_abort = nullptr;             // pinned host copy of *d_abort
     int64_t nslots = 0;
     double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr;   // FP64 n * work_nrhs
     int64_t work_nrhs = 0;
@@ -797,8 +796,6 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     if ((rc = dalloc(F, &F->d_cnt, (size_t)(3 * ns)))) return rc;
     if ((rc = dalloc(F, &F->d_abort, 1))) return rc;
     HIPTRY(hipMemsetAsync(F->d_abort, 0, sizeof(int), sm));
-    HIPTRY(hipHostMalloc((void **)&F->h_abort, sizeof(int), hipHostMallocDefault));
-    *F->h_abort = 0;
     HIPTRY(hipStreamSynchronize(sm));
     return MXC_OK;
 }
@@ -841,7 +838,7 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
                     F->d_cnt, F->d_abort};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    if (F->h_abort) (void)hipHostFree(F->h_abort);
+
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
     F->ev.clear();
     if (F->ev_fork) (void)hipEventDestroy(F->ev_fork);
@@ -1263,7 +1260,7 @@ extern "C" int mxc_factor_get_Lx(const mxc_factor *F, double *Lx) {
 // ---------------------------------------------------------------------------
 // solves
 // ---------------------------------------------------------------------------
-// A level's chained sweep launch; right-hand sides per pass 1, 2 or 4.
+// A level's chained sweep launch over nr <= CHAIN_RHS columns (one pass).
 template <typename T, bool FWD>
 static void launch_chain(mxc_factor *F, const Launch &L, T *Xw, int nr) {
     hipStream_t sm = F->stream;
@@ -1291,18 +1288,14 @@ static void launch_chain(mxc_factor *F, const Launch &L, T *Xw, int nr) {
     }
 }
 
-// Chained sweeps: fetch the abort flag behind the stream's work (read after
-// the caller's synchronization by chain_status).
-static int chain_fetch(mxc_factor *F) {
-    if (F->chain) HIPTRY(hipMemcpyAsync(F->h_abort, F->d_abort, sizeof(int), hipMemcpyDeviceToHost, F->stream));
-    return MXC_OK;
-}
-
-// After a synchronization: a chained sweep whose wait gave up is a device
-// error (the flag is cleared for the next call).
+// After the caller's synchronization of the factor's stream: a chained sweep
+// whose wait gave up is a device error (the flag is cleared for the next call).
 static int chain_status(mxc_factor *F) {
-    if (!*F->h_abort) return MXC_OK;
-    *F->h_abort = 0;
+    if (!F->chain) return MXC_OK;
+    int ab = 0;
+    HIPTRY(hipMemcpyAsync(&ab, F->d_abort, sizeof(int), hipMemcpyDeviceToHost, F->stream));
+    HIPTRY(hipStreamSynchronize(F->stream));
+    if (!ab) return MXC_OK;
     HIPTRY(hipMemsetAsync(F->d_abort, 0, sizeof(int), F->stream));
     HIPTRY(hipStreamSynchronize(F->stream));
     snprintf(g_err, sizeof(g_err), "chained sweep: a wait for a block did not end");
@@ -1316,18 +1309,46 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     size_t tt = 0;
     int n = (int)F->n;
     int nr = (int)nrhs;
-    const bool chain = F->chain;
     const int ns = (int)F->nsuper;
+    if (F->chain) {
+        // chained sweeps: one pass per launch, so the columns go CHAIN_RHS at a
+        // time (a block is published once all of its columns are final)
+        for (int dir = 0; dir < 2; dir++) {
+            if (!(dir == 0 ? fwd : bwd)) continue;
+            tick(F, dir == 0 ? 5 : 6, &tt);
+            for (int z0 = 0; z0 < nr; z0 += CHAIN_RHS) {
+                const int zn = std::min(CHAIN_RHS, nr - z0);
+                T *Xz = Xw + (size_t)z0 * (size_t)n;
+                if (ns > 0)
+                    hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(3 * (long long)ns, 256, 1LL << 30)), dim3(256),
+                                       0, sm, F->d_cnt, 3 * ns);
+                for (const Launch &L : dir == 0 ? F->fwd_chain : F->bwd_chain) {
+                    if (L.kind == L_FSMALL)
+                        hipLaunchKernelGGL(k_sweep_small_fwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256),
+                                           0, sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xz,
+                                           (long long)n, zn);
+                    else if (L.kind == L_BSMALL)
+                        hipLaunchKernelGGL(k_sweep_small_bwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256),
+                                           0, sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xz,
+                                           (long long)n, zn);
+                    else if (dir == 0)
+                        launch_chain<T, true>(F, L, Xz, zn);
+                    else
+                        launch_chain<T, false>(F, L, Xz, zn);
+                    F->counts[4]++;
+                }
+            }
+            tock(F, dir == 0 ? 5 : 6, tt);
+        }
+        HIPTRY(hipGetLastError());
+        return MXC_OK;
+    }
     if (fwd) {
         tick(F, 5, &tt);
-        if (chain && ns > 0)
-            hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(ns, 256, 1LL << 30)), dim3(256), 0, sm, F->d_cnt, ns);
-        for (const Launch &L : chain ? F->fwd_chain : F->fwd_launches) {
+        for (const Launch &L : F->fwd_launches) {
             if (L.kind == L_FSMALL)
                 hipLaunchKernelGGL(k_sweep_small_fwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,
                                    sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xw, (long long)n, nr);
-            else if (L.kind == L_FCHAIN)
-                launch_chain<T, true>(F, L, Xw, nr);
             else
                 hipLaunchKernelGGL(k_sweep_fwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
                                    F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (long long)n, nr);
@@ -1337,15 +1358,10 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     }
     if (bwd) {
         tick(F, 6, &tt);
-        if (chain && ns > 0)
-            hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(2 * ns, 256, 1LL << 30)), dim3(256), 0, sm,
-                               F->d_cnt + ns, 2 * ns);
-        for (const Launch &L : chain ? F->bwd_chain : F->bwd_launches) {
+        for (const Launch &L : F->bwd_launches) {
             if (L.kind == L_BSMALL)
                 hipLaunchKernelGGL(k_sweep_small_bwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,
                                    sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xw, (long long)n, nr);
-            else if (L.kind == L_BCHAIN)
-                launch_chain<T, false>(F, L, Xw, nr);
             else
                 hipLaunchKernelGGL(k_sweep_bwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
                                    F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (T *)F->d_part,
@@ -1433,7 +1449,6 @@ extern "C" int mxc_solve(mxc_factor *F, const double *B, int64_t nrhs, double *X
     if (rc) return rc;
     if (location != MXC_DEVICE)
         HIPTRY(hipMemcpyAsync(X, dX, F->n * nrhs * sizeof(double), hipMemcpyDeviceToHost, F->stream));
-    if ((rc = chain_fetch(F))) return rc;
     HIPTRY(hipStreamSynchronize(F->stream));
     return chain_status(F);
 }
@@ -1452,7 +1467,6 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
     double *dX = location == MXC_DEVICE ? X : F->d_x;
     if (F->precision == MXC_FP64) {
         rc = run_sweep<double>(F, dX, nrhs, fwd, !fwd);
-        if (rc == MXC_OK) rc = chain_fetch(F);
     } else {
         float *Xw = (float *)F->d_work;
         std::vector<float> tmp;
@@ -1462,7 +1476,6 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
         tmp.assign(h.begin(), h.end());
         HIPTRY(hipMemcpyAsync(Xw, tmp.data(), len * sizeof(float), hipMemcpyHostToDevice, sm));
         rc = run_sweep<float>(F, Xw, nrhs, fwd, !fwd);
-        if (rc == MXC_OK) rc = chain_fetch(F);
         if (rc) return rc;
         HIPTRY(hipMemcpyAsync(tmp.data(), Xw, len * sizeof(float), hipMemcpyDeviceToHost, sm));
         HIPTRY(hipStreamSynchronize(sm));
@@ -1567,7 +1580,6 @@ static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *
     unsigned g = grid_for(F->n, 256, 1LL << 30);
     hipLaunchKernelGGL(k_perm_gather<T>, dim3(g), dim3(256), 0, sm, dB, F->d_perm, (long long)F->n, (int)nrhs, Xw);
     if ((rc = run_sweep<T>(F, Xw, nrhs, true, true))) return rc;
-    if ((rc = chain_fetch(F))) return rc;
     hipLaunchKernelGGL(k_perm_scatter<T>, dim3(g), dim3(256), 0, sm, Xw, F->d_perm, (long long)F->n, (int)nrhs, dX);
     int it = 0;
     double rel = 0, rel0 = -1;
@@ -1585,8 +1597,7 @@ static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *
         // d = A \ r ; x += d
         hipLaunchKernelGGL(k_perm_gather<T>, dim3(g), dim3(256), 0, sm, F->d_r, F->d_perm, (long long)F->n, (int)nrhs, Xw);
         if ((rc = run_sweep<T>(F, Xw, nrhs, true, true))) return rc;
-        if ((rc = chain_fetch(F))) return rc;
-        hipLaunchKernelGGL(k_perm_scatter<T>, dim3(g), dim3(256), 0, sm, Xw, F->d_perm, (long long)F->n, (int)nrhs, F->d_r);
+            hipLaunchKernelGGL(k_perm_scatter<T>, dim3(g), dim3(256), 0, sm, Xw, F->d_perm, (long long)F->n, (int)nrhs, F->d_r);
         hipLaunchKernelGGL(k_axpy_update<double>, dim3(grid_for((long long)len, 256)), dim3(256), 0, sm, dX,
                            (const double *)F->d_r, (long long)len);
         it++;
