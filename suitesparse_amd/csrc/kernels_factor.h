@@ -152,21 +152,56 @@ __device__ __forceinline__ unsigned split_pack(float v, int e) {
            ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
 }
 
-template <typename T, int MODE, int TS>
+template <typename T, bool SAFE>
+__device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W, T *S, T *X, int *sfail,
+                                            int *__restrict__ fail, const double *__restrict__ adiag,
+                                            double rho, int *__restrict__ npert);
+
+// Arguments of the diagonal-block tail of a trailing launch (PD): after its
+// epilogue, the workgroup of tile (0, 0) of a block with pd >= 0 has the final
+// values of the next step's diagonal block and factors it (potrf_block).
+struct PdArgs {
+    const DiagTask *diags;
+    int *fail;
+    double rho;
+    int *npert;
+};
+
+// The tail itself: S and X of potrf_block carved from the tile's LDS buffer.
+template <typename T>
+__device__ __forceinline__ void pd_tail(const PdArgs &pda, int pd, T *__restrict__ W, unsigned char *smem,
+                                        const double *__restrict__ adiag) {
+    __shared__ int sfail;
+    __syncthreads();                        // the tile's stores and LDS reads are done
+    T *S = reinterpret_cast<T *>(smem);
+    if (pda.rho > 0)
+        potrf_block<T, true>(pda.diags[pd], W, S, S + NB * DS, &sfail, pda.fail, adiag, pda.rho, pda.npert);
+    else
+        potrf_block<T, false>(pda.diags[pd], W, S, S + NB * DS, &sfail, pda.fail, adiag, pda.rho, pda.npert);
+}
+
+template <typename T, int MODE, int TS, bool PD = false>
 __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile(const UpdDesc *__restrict__ descs,
                                                               const Tile *__restrict__ tiles,
                                                               T *__restrict__ W,
                                                               const int *__restrict__ Ls,
                                                               unsigned *__restrict__ Lh,
-                                                              const double *__restrict__ adiag) {
+                                                              const double *__restrict__ adiag,
+                                                              PdArgs pda = PdArgs{}) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     constexpr int NT = TS == 32 ? 64 : 256;     // threads
     constexpr int WB = TS == 32 ? 32 : TS / 2;  // edge of a wave's block of C
     constexpr int MB = WB / 16;                 // 16x16 MFMA blocks per wave edge
     constexpr int KG = NT / TS;                 // k rows staged per pass
     constexpr int PER = (BK * TS) / NT;         // elements of X (and of Y) each thread stages
-    __shared__ T Xs[2][BK][TS + LDS_PAD];
-    __shared__ T Ys[2][BK][TS + LDS_PAD];
+    static_assert(!PD || (MODE == MODE_TRAIL && TS >= 64), "diagonal tails: trailing tiles of 64 or 128");
+    // one LDS buffer: the staged X / Y chunks, reused by the diagonal-block tail
+    constexpr size_t XY_BYTES = sizeof(T) * 2 * BK * (TS + LDS_PAD);
+    constexpr size_t PD_BYTES = PD ? sizeof(T) * 2 * NB * DS : 0;
+    constexpr size_t SMEM = 2 * XY_BYTES > PD_BYTES ? 2 * XY_BYTES : PD_BYTES;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
+    T (*Xs)[BK][TS + LDS_PAD] = reinterpret_cast<T (*)[BK][TS + LDS_PAD]>(smem);
+    T (*Ys)[BK][TS + LDS_PAD] = reinterpret_cast<T (*)[BK][TS + LDS_PAD]>(smem + XY_BYTES);
     __shared__ int relr[TS];
     __shared__ int relc[TS];
 
@@ -292,6 +327,9 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
             }
         }
     }
+    if constexpr (PD) {
+        if (d.pd >= 0 && tl.ti == 0 && tl.tj == 0) pd_tail<T>(pda, d.pd, W, smem, adiag);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -311,13 +349,14 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
 // ---------------------------------------------------------------------------
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 
-template <int MODE, int TS>
+template <int MODE, int TS, bool PD = false>
 __global__ __launch_bounds__(TS == 32 ? 64 : 256, 2) void k_tile_h(const UpdDesc *__restrict__ descs,
                                                                   const Tile *__restrict__ tiles,
                                                                   float *__restrict__ W,
                                                                   const unsigned *__restrict__ Lh,
                                                                   const int *__restrict__ Ls,
-                                                                  const double *__restrict__ adiag) {
+                                                                  const double *__restrict__ adiag,
+                                                                  PdArgs pda = PdArgs{}) {
     static_assert(TS == 32 || TS == 64 || TS == 128, "split-FP16 tiles are 32, 64 or 128 wide");
     constexpr int NT = TS == 32 ? 64 : 256;  // one wave for a 32-wide tile, else 2 x 2 waves
     constexpr int WB = TS == 32 ? 32 : TS / 2;   // edge of a wave's block of C
@@ -326,10 +365,15 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, 2) void k_tile_h(const UpdDesc
     constexpr int RS = BKH + 8;          // LDS row stride (halves)
     constexpr int KG = NT / TS;          // staging threads per row
     constexpr int PER = BKH / KG;        // consecutive k per staging thread
-    __shared__ __attribute__((aligned(16))) _Float16 Xh[TS * RS];
-    __shared__ __attribute__((aligned(16))) _Float16 Xl[TS * RS];
-    __shared__ __attribute__((aligned(16))) _Float16 Yh[TS * RS];
-    __shared__ __attribute__((aligned(16))) _Float16 Yl[TS * RS];
+    static_assert(!PD || (MODE == MODE_TRAIL && TS >= 64), "diagonal tails: trailing tiles of 64 or 128");
+    // one LDS buffer: the staged halves, reused by the diagonal-block tail
+    constexpr size_t H_BYTES = 4 * sizeof(_Float16) * TS * RS;
+    constexpr size_t PD_BYTES = PD ? sizeof(float) * 2 * NB * DS : 0;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[H_BYTES > PD_BYTES ? H_BYTES : PD_BYTES];
+    _Float16 *Xh = reinterpret_cast<_Float16 *>(smem);
+    _Float16 *Xl = Xh + TS * RS;
+    _Float16 *Yh = Xl + TS * RS;
+    _Float16 *Yl = Yh + TS * RS;
     __shared__ int relr[TS], relc[TS];
     __shared__ int exr[TS], exc[TS];     // scale exponents of the Y rows (i) and X rows (j)
 
@@ -467,6 +511,9 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, 2) void k_tile_h(const UpdDesc
             }
         }
     }
+    if constexpr (PD) {
+        if (d.pd >= 0 && tl.ti == 0 && tl.tj == 0) pd_tail<float>(pda, d.pd, W, smem, adiag);
+    }
 }
 
 // Diagonal block Cholesky + triangular inverse of a 64x64 block (a block
@@ -481,7 +528,6 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, 2) void k_tile_h(const UpdDesc
 // the diagonal) goes to the block's inverse slot (ld nb).  A non-positive
 // pivot at column j records col0 + j in *fail (atomic min).
 // ---------------------------------------------------------------------------
-constexpr int DS = NB + 1;     // LDS row stride of the 64x64 block
 
 // 1/sqrt(x) for x > 0: the hardware estimate refined by two Newton steps
 // y <- y + y (1 - x y^2) / 2 (each roughly doubles the correct bits).
@@ -615,16 +661,15 @@ __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, i
 // rho * a_jj and counted in *npert, and the FP64 refinement (or the FP64
 // fallback factor) repairs the perturbation.  !SAFE: exact semantics, a pivot
 // <= 0 stops.
+//
+// potrf_block: the whole workgroup (256 threads) factors diagonal block t in
+// LDS (S: the block, X: its inverse, both NB x DS row-major; *sfail: a shared
+// int) and writes L_tt and the inverse back to W.
 template <typename T, bool SAFE>
-__global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__ tasks,
-                                                    T *__restrict__ W, int *__restrict__ fail,
-                                                    const double *__restrict__ adiag, double rho,
-                                                    int *__restrict__ npert) {
+__device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W, T *S, T *X, int *sfail,
+                                            int *__restrict__ fail, const double *__restrict__ adiag,
+                                            double rho, int *__restrict__ npert) {
     typedef typename MfmaOp<T>::acc_t acc_t;
-    __shared__ T S[NB * DS];       // the block, row-major
-    __shared__ T X[NB * DS];       // its inverse
-    __shared__ int sfail;
-    const DiagTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nb = t.nb;
     const int np = (nb + 15) >> 4;          // 16-column panels that hold real columns
@@ -644,7 +689,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
             X[i * DS + c] = 0;
         }
     }
-    if (tid == 0) sfail = -1;
+    if (tid == 0) *sfail = -1;
     __syncthreads();
 
     for (int p = 0; p < np; p++) {
@@ -652,7 +697,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
         // (1) POTRF + inverse of the 16x16 diagonal block, one wave
         if (wave == 0) {
             const int badj = potrf16_wave<T, SAFE>(S, X, DS, R, nb, lane, adiag, t.col0, rho, npert);
-            if (lane == 0 && badj < 16 && sfail < 0) sfail = R + badj;
+            if (lane == 0 && badj < 16 && *sfail < 0) *sfail = R + badj;
         }
         __syncthreads();
         // (2) panel TRSM: rows below, 16-row tiles t = p+1..np-1, one wave each
@@ -704,7 +749,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
         }
         __syncthreads();
     }
-    if (tid == 0 && sfail >= 0) atomicMin(fail, t.col0 + sfail);
+    if (tid == 0 && *sfail >= 0) atomicMin(fail, t.col0 + *sfail);
     {
         const int i = tid & (NB - 1), c0 = tid >> 6;
 #pragma unroll
@@ -716,6 +761,17 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
             }
         }
     }
+}
+
+template <typename T, bool SAFE>
+__global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__ tasks,
+                                                    T *__restrict__ W, int *__restrict__ fail,
+                                                    const double *__restrict__ adiag, double rho,
+                                                    int *__restrict__ npert) {
+    __shared__ T S[NB * DS];       // the block, row-major
+    __shared__ T X[NB * DS];       // its inverse
+    __shared__ int sfail;
+    potrf_block<T, SAFE>(tasks[blockIdx.x], W, S, X, &sfail, fail, adiag, rho, npert);
 }
 
 

@@ -108,19 +108,22 @@ static inline int tile_bucket(int M, int N, int K) {
 static constexpr int NBUCKET = 75;
 static constexpr int NKIND = 3;     // 0 descendant update, 1 trailing, 2 TRSM
 static inline int kind_of(int launch_kind) {
-    return launch_kind == 0 ? 0 : (launch_kind == 3 || launch_kind == 9) ? 1 : 2;
+    return launch_kind == 0 ? 0 : (launch_kind == 3 || launch_kind == 9 || launch_kind == 14) ? 1 : 2;
 }
 // L_TRAIL2: the part of a panel's trailing update right of the next panel, on
 // the second stream; L_FORK / L_JOIN: the event edges between the two streams;
-// L_FCHAIN / L_BCHAIN: a level's chained sweep launch.
+// L_FCHAIN / L_BCHAIN: a level's chained sweep launch; L_TRAILD: a trailing
+// launch whose (0, 0) tiles also factor the next step's diagonal blocks.
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7,
-       L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11, L_FCHAIN = 12, L_BCHAIN = 13 };
+       L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11, L_FCHAIN = 12, L_BCHAIN = 13,
+       L_TRAILD = 14 };
 
 struct StepBlocks {
     std::vector<DiagTask> diags;
     std::vector<UpdDesc> trsm, trail;
     std::vector<UpdDesc> trail2;   // right of the next panel (second stream)
     bool panel_end = false;        // trail holds the update of the next panel's columns
+    std::vector<int64_t> diag_s, trail_s;   // supernode of each diag task / trail block
 };
 
 struct mxc_factor {
@@ -136,6 +139,7 @@ struct mxc_factor {
     bool lookahead = true;
     bool split16 = true;                 // FP32 factors: split-FP16 update/trailing tiles
     bool chain = true;                   // solves: one chained launch per level
+    bool diag_tails = true;              // trailing launches factor the next diagonal blocks
     size_t bytes = 0;
 
     // host copies
@@ -252,20 +256,52 @@ static void drop_factor_graph(mxc_factor *F) {
 
 // Turn the host block lists into device descriptors, tiles and the launch
 // list, one launch per tile class (from F->tune) per phase.
+// FP64 trailing launches of at most this many tiles carry the next step's
+// diagonal blocks as tails: the FP64 tail kernels run at half the occupancy of
+// the plain ones (2 waves / SIMD, two resident workgroups per CU).  The FP32
+// tail kernels keep most of theirs and take tails in every trailing launch
+// (96^3 27-point FP32 factor: 72.3 ms unbounded, 73.0 ms at 512 tiles).
+constexpr long long PD_MAX_TILES_FP64 = 512;
+
 static int flatten_upload(mxc_factor *F) {
     std::vector<UpdDesc> descs;
     std::vector<Tile> tiles;
     std::vector<DiagTask> diags;
     F->factor_launches.clear();
     int64_t nupd = 0;
-    auto emit = [&](int kind, const std::vector<UpdDesc> &blocks, bool lower_only) {
+    // follow[i] (trailing blocks): the next step's diagonal task of block i's
+    // supernode, or null.  A block tiled at 64 or 128 gets it as its tail (pd);
+    // fused[i] records that, so the next step's diagonal launch skips it.
+    auto emit = [&](int kind, const std::vector<UpdDesc> &blocks, bool lower_only,
+                    const std::vector<const DiagTask *> *follow = nullptr, std::vector<char> *fused = nullptr) {
         long long total = 0;
         for (int ts : {32, 64, 128}) {
             long long b = (long long)tiles.size();
-            for (const UpdDesc &u : blocks) {
+            // tails only in launches that fit on the device at once (a tail's
+            // kernel runs at lower occupancy)
+            bool allow = follow != nullptr && ts >= 64;
+            if (allow) {
+                long long cnt = 0;
+                for (const UpdDesc &u : blocks)
+                    if (F->tune[kind_of(kind)][tile_bucket(u.M, u.N, u.K)] == ts) {
+                        const long long mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
+                        cnt += lower_only ? mt * nt - nt * (nt - 1) / 2 : mt * nt;
+                    }
+                allow = F->precision != MXC_FP64 || cnt <= PD_MAX_TILES_FP64;
+            }
+            bool tails = false;
+            for (size_t bi = 0; bi < blocks.size(); bi++) {
+                const UpdDesc &u = blocks[bi];
                 if (F->tune[kind_of(kind)][tile_bucket(u.M, u.N, u.K)] != ts) continue;
                 int di = (int)descs.size();
                 descs.push_back(u);
+                descs.back().pd = -1;
+                if (allow && (*follow)[bi]) {
+                    descs.back().pd = (int)diags.size();
+                    diags.push_back(*(*follow)[bi]);
+                    (*fused)[bi] = 1;
+                    tails = true;
+                }
                 int64_t mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
                 for (int64_t ti = 0; ti < mt; ti++)
                     for (int64_t tj = 0; tj < nt; tj++) {
@@ -274,7 +310,7 @@ static int flatten_upload(mxc_factor *F) {
                     }
             }
             long long c = (long long)tiles.size() - b;
-            if (c > 0) F->factor_launches.push_back(Launch{kind, b, c, ts});
+            if (c > 0) F->factor_launches.push_back(Launch{tails ? (int)L_TRAILD : kind, b, c, ts});
             total += c;
         }
         return total;
@@ -282,12 +318,18 @@ static int flatten_upload(mxc_factor *F) {
     for (size_t l = 0; l < F->blk_upd.size(); l++) {
         bool level_forked = false;
         nupd += emit(L_UPD, F->blk_upd[l], true);
-        for (const StepBlocks &sb : F->blk_steps[l]) {
+        std::vector<int64_t> fused_s;              // supernodes whose diagonal block of this step is
+                                                   // factored by the previous step's trailing launch
+        for (size_t t = 0; t < F->blk_steps[l].size(); t++) {
+            const StepBlocks &sb = F->blk_steps[l][t];
             // blocks of at most 16 columns: one wave each; wider blocks: a workgroup each
             for (int small = 1; small >= 0; small--) {
                 long long db = (long long)diags.size();
-                for (const DiagTask &d : sb.diags)
+                for (size_t q = 0; q < sb.diags.size(); q++) {
+                    const DiagTask &d = sb.diags[q];
+                    if (std::binary_search(fused_s.begin(), fused_s.end(), sb.diag_s[q])) continue;
                     if ((d.nb <= 16) == (small == 1)) diags.push_back(d);
+                }
                 long long dc = (long long)diags.size() - db;
                 if (dc > 0) F->factor_launches.push_back(Launch{small ? L_DIAG_SMALL : L_DIAG, db, dc, 0});
             }
@@ -305,7 +347,22 @@ static int flatten_upload(mxc_factor *F) {
                 emit(L_TRAIL2, sb.trail2, true);
                 level_forked = true;
             }
-            emit(L_TRAIL, sb.trail, true);
+            // the next step's diagonal blocks, one per trailing block
+            std::vector<const DiagTask *> follow(sb.trail.size(), nullptr);
+            std::vector<char> fused(sb.trail.size(), 0);
+            if (F->diag_tails && t + 1 < F->blk_steps[l].size()) {
+                // both lists ascend by supernode
+                const StepBlocks &nx = F->blk_steps[l][t + 1];
+                size_t r = 0;
+                for (size_t q = 0; q < sb.trail.size(); q++) {
+                    while (r < nx.diags.size() && nx.diag_s[r] < sb.trail_s[q]) r++;
+                    if (r < nx.diags.size() && nx.diag_s[r] == sb.trail_s[q]) follow[q] = &nx.diags[r];
+                }
+            }
+            emit(L_TRAIL, sb.trail, true, &follow, &fused);
+            fused_s.clear();
+            for (size_t q = 0; q < sb.trail.size(); q++)
+                if (fused[q]) fused_s.push_back(sb.trail_s[q]);
         }
         // the next level reads this level's columns: join the second stream
         if (level_forked) F->factor_launches.push_back(Launch{L_JOIN, 0, 0, 0});
@@ -509,6 +566,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                 dt.col0 = (int)(S->Super[s] + c0);
                 dt.pad = 0;
                 sb.diags.push_back(dt);
+                sb.diag_s.push_back(s);
                 F->flops_potrf += (double)nb * nb * nb / 3.0;
                 int64_t r0 = c0 + nb;
                 if (nsrow > r0) {
@@ -545,6 +603,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.ls_off = (int)(S->Lpi[s] + r0);   // global rows (for per-row scales)
                     u.ldc = (int)nsrow;
                     sb.trail.push_back(u);
+                    sb.trail_s.push_back(s);
                     double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
                     F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
                 }
@@ -570,6 +629,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                         u.ls_off = (int)(S->Lpi[s] + c_lo);
                         u.ldc = (int)nsrow;
                         (part == 0 ? sb.trail : sb.trail2).push_back(u);
+                        if (part == 0) sb.trail_s.push_back(s);
                         double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
                         F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
                     }
@@ -973,34 +1033,40 @@ static inline unsigned grid_for(long long n, int bs, long long cap = 65536) {
     return (unsigned)g;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool PD = false>
 static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = nullptr) {
     if (!st) st = F->stream;
     const bool h16 = std::is_same<T, float>::value && F->split16 && F->d_Lh;
     const double *adiag = F->d_adiag;
+    const PdArgs pda{F->d_diag, F->d_fail, F->rho, F->d_fail + 1};
+    const dim3 g((unsigned)L.count);
     if constexpr (std::is_same<T, float>::value && MODE != MODE_STORE) {
         // FP32 factor: split-FP16 products on the FP16 MFMA, operands from the
         // pairs the TRSM wrote (32-wide tiles are latency-bound: FP32 MFMA)
         if (h16 && L.tile != 32) {
             if (L.tile == 64)
-                hipLaunchKernelGGL((k_tile_h<MODE, 64>), dim3((unsigned)L.count), dim3(256), 0, st,
-                                   F->d_desc, F->d_tiles + L.begin, W, (const unsigned *)F->d_Lh, F->d_Ls, adiag);
+                hipLaunchKernelGGL((k_tile_h<MODE, 64, PD>), g, dim3(256), 0, st, F->d_desc, F->d_tiles + L.begin,
+                                   W, (const unsigned *)F->d_Lh, F->d_Ls, adiag, pda);
             else
-                hipLaunchKernelGGL((k_tile_h<MODE, 128>), dim3((unsigned)L.count), dim3(256), 0, st,
-                                   F->d_desc, F->d_tiles + L.begin, W, (const unsigned *)F->d_Lh, F->d_Ls, adiag);
+                hipLaunchKernelGGL((k_tile_h<MODE, 128, PD>), g, dim3(256), 0, st, F->d_desc, F->d_tiles + L.begin,
+                                   W, (const unsigned *)F->d_Lh, F->d_Ls, adiag, pda);
             return;
         }
     }
     unsigned *Lh = (MODE == MODE_STORE && h16) ? F->d_Lh : nullptr;
-    if (L.tile == 32)
-        hipLaunchKernelGGL((k_tile<T, MODE, 32>), dim3((unsigned)L.count), dim3(64), 0, st,
-                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, Lh, adiag);
-    else if (L.tile == 64)
-        hipLaunchKernelGGL((k_tile<T, MODE, 64>), dim3((unsigned)L.count), dim3(256), 0, st,
-                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, Lh, adiag);
+    if constexpr (!PD) {
+        if (L.tile == 32) {
+            hipLaunchKernelGGL((k_tile<T, MODE, 32>), g, dim3(64), 0, st, F->d_desc, F->d_tiles + L.begin, W,
+                               F->d_Ls, Lh, adiag, pda);
+            return;
+        }
+    }
+    if (L.tile == 64)
+        hipLaunchKernelGGL((k_tile<T, MODE, 64, PD>), g, dim3(256), 0, st, F->d_desc, F->d_tiles + L.begin, W,
+                           F->d_Ls, Lh, adiag, pda);
     else
-        hipLaunchKernelGGL((k_tile<T, MODE, 128>), dim3((unsigned)L.count), dim3(256), 0, st,
-                           F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, Lh, adiag);
+        hipLaunchKernelGGL((k_tile<T, MODE, 128, PD>), g, dim3(256), 0, st, F->d_desc, F->d_tiles + L.begin, W,
+                           F->d_Ls, Lh, adiag, pda);
 }
 
 // Enqueue one whole factorization (values already in d_Ax, beta in d_beta).
@@ -1075,6 +1141,10 @@ static int enqueue_factor(mxc_factor *F) {
                 break;
             case L_TRAIL:
                 launch_tile<T, MODE_TRAIL>(F, L, W);
+                F->counts[3]++;
+                break;
+            case L_TRAILD:
+                launch_tile<T, MODE_TRAIL, true>(F, L, W);
                 F->counts[3]++;
                 break;
             case L_TRAIL2:

@@ -14,6 +14,7 @@ namespace mxc {
 constexpr int TILE = 64;     // C tile edge (rows and columns)
 constexpr int BK = 16;       // K chunk staged through LDS
 constexpr int NB = 64;       // diagonal block width of the blocked POTRF
+constexpr int DS = NB + 1;   // LDS row stride of a 64x64 diagonal block
 constexpr int LDS_PAD = 4;
 
 // One dense update C = Yp * Xp' (C is M x N) applied to a target.
@@ -34,7 +35,7 @@ struct UpdDesc {
     int K, M, N;
     int ls_off, base;
     int t_psi, t_nscol, t_nsrow;
-    int pad;
+    int pd;            // trailing blocks: diagonal task factored after tile (0, 0), or -1
 };
 
 struct Tile {

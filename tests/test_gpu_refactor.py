@@ -62,6 +62,42 @@ def test_not_posdef_matches_reference(gpu, ref_lib):
             assert err <= 1e-12, err
 
 
+def test_not_posdef_inside_a_supernode(gpu, ref_lib):
+    """A dense 320-column supernode (five 64-column steps, natural order) whose
+    leading minor turns indefinite in step 0, 2 or 4.  From step 1 on, a step's
+    diagonal block is factored at the end of the previous step's trailing
+    launch (the workgroup of the block's tile); the breakdown it finds must give
+    the reference's status, L->minor and factor (zeros after the failing
+    column), in FP64 and, with the FP32 pivot safeguard off, in FP32."""
+    import scipy.sparse as sp
+    n = 320
+    rng = np.random.default_rng(21)
+    M = rng.standard_normal((n, n))
+    A = M @ M.T + n * np.eye(n)
+    for bad in (30, 150, 290):
+        A2 = A.copy()
+        A2[bad, bad] = -1.0
+        T = sp.csc_matrix(np.tril(A2))
+        As = problems.SymMatrix(n=n, p=T.indptr.astype(np.int64), i=T.indices.astype(np.int64),
+                                x=T.data.astype(np.float64))
+        R = ref_lib.RefFactor(As, ref_lib.NATURAL)
+        sym = R.symbolic()
+        assert R.factorize() == 1
+        Lref = R.Lx()
+        print(f"case bad={bad} ref minor={R.minor}", flush=True)
+        for prec in ("fp64", "fp32"):
+            cm = cholmod.Common(precision=prec)
+            L = cholmod.analyze_p(As, sym["Perm"], cm)
+            L._numeric(cm)
+            if prec == "fp32":
+                L.set_pivot_rho(0.0)
+            assert cholmod.factorize(As, L, cm)
+            assert cm.status == cholmod.CHOLMOD_NOT_POSDEF
+            assert L.minor == R.minor
+            err = np.abs(L.x - Lref).max() / np.abs(Lref).max()
+            assert err <= (1e-12 if prec == "fp64" else 1e-4), err
+
+
 def test_device_resident_inputs(gpu, ref_lib):
     import torch
     A = problems.stencil3d_27pt(10, seed=8)
