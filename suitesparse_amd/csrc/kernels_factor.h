@@ -195,8 +195,11 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
     constexpr int KG = NT / TS;                 // k rows staged per pass
     constexpr int PER = (BK * TS) / NT;         // elements of X (and of Y) each thread stages
     static_assert(!PD || (MODE == MODE_TRAIL && TS >= 64), "diagonal tails: trailing tiles of 64 or 128");
-    // one LDS buffer: the staged X / Y chunks, reused by the diagonal-block tail
-    constexpr size_t XY_BYTES = sizeof(T) * 2 * BK * (TS + LDS_PAD);
+    // one LDS buffer: the staged X / Y chunks, reused by the diagonal-block tail.
+    // A one-wave tile (32) stages into a single buffer: its barriers cost
+    // little, and half the LDS doubles the resident tiles per CU.
+    constexpr int NBUF = TS == 32 ? 1 : 2;
+    constexpr size_t XY_BYTES = sizeof(T) * NBUF * BK * (TS + LDS_PAD);
     constexpr size_t PD_BYTES = PD ? sizeof(T) * 2 * NB * DS : 0;
     constexpr size_t SMEM = 2 * XY_BYTES > PD_BYTES ? 2 * XY_BYTES : PD_BYTES;
     __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
@@ -263,6 +266,7 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
     fetch(0);
     int buf = 0;
     for (int k0 = 0; k0 < d.K; k0 += BK) {
+        if (NBUF == 1 && k0 > 0) __syncthreads();   // the previous chunk's reads are done
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             Ys[buf][kq + KG * q][r] = ry[q];
@@ -282,7 +286,7 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
 #pragma unroll
                 for (int n = 0; n < MB; n++) acc[m][n] = MfmaOp<T>::mma(xa[m], yb[n], acc[m][n]);
         }
-        buf ^= 1;
+        if constexpr (NBUF == 2) buf ^= 1;
     }
     if (MODE == MODE_STORE) __syncthreads();   // in-place TRSM: every read of the panel is done
 
