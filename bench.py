@@ -483,7 +483,7 @@ def serving_section(args, out):
     arrivals = serving.poisson_arrivals(n, args.serving_rate)
     srv = serving.Server(slots=args.serving_slots, hbm_budget=int(args.serving_budget_gib * (1 << 30)),
                          precision="fp32", tol=1e-10, keep_results=False)
-    st = serving.run_stream(srv, reqs, arrivals)
+    st = serving.run_stream(srv, reqs, arrivals, progress=1000)
     st.update({"workload": f"serving_{n}req_poisson{args.serving_rate:g}_2d32..512_3d16..64_amd_fp32ir",
                "rate_per_s": args.serving_rate, "slots": args.serving_slots,
                "hbm_budget_gib": args.serving_budget_gib})

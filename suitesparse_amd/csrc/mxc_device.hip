@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <climits>
 #include <cmath>
 #include <map>
@@ -56,6 +57,13 @@ static int set_hip_err(hipError_t e, const char *what, int line) {
 extern "C" const char *mxc_last_error(void) { return g_err; }
 
 static int g_device = 0;
+
+// Graph capture and instantiation, factor creation and release, and work-buffer
+// (re)allocation are serialized across host threads (each is rare: once per
+// plan, per right-hand-side count or per growth), so that no thread captures
+// while another allocates, frees or creates streams.  Recursive: releasing a
+// factor also releases its FP64 fallback.
+static std::recursive_mutex g_rt_mutex;
 
 extern "C" int mxc_device_init(int device) {
     int count = 0;
@@ -870,6 +878,7 @@ static void drop_sweep_graphs(mxc_factor *F) {
 
 static int ensure_work(mxc_factor *F, int64_t nrhs) {
     if (F->work_nrhs >= nrhs) return MXC_OK;
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
     drop_sweep_graphs(F);
     size_t esz = F->precision == MXC_FP64 ? sizeof(double) : sizeof(float);
     if (F->d_work) { (void)hipFree(F->d_work); F->d_work = nullptr; }
@@ -889,6 +898,7 @@ static int ensure_work(mxc_factor *F, int64_t nrhs) {
 
 extern "C" void mxc_factor_free(mxc_factor *F) {
     if (!F) return;
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
     drop_sweep_graphs(F);
     drop_factor_graph(F);
     if (F->fallback) mxc_factor_free(F->fallback);
@@ -915,6 +925,7 @@ extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const
         return MXC_INVALID;
     if (precision != MXC_FP64 && precision != MXC_FP32) return MXC_INVALID;
     *out = nullptr;
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
     mxc_factor *F = new (std::nothrow) mxc_factor();
     if (!F) return MXC_OUT_OF_MEMORY;
     F->precision = precision;
@@ -1166,6 +1177,7 @@ static int run_factor(mxc_factor *F) {
     hipStream_t sm = F->stream;
     if (!F->use_graphs || F->timing_mask != 0) return enqueue_factor<T>(F);
     if (!F->x_factor) {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
         HIPTRY(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_factor<T>(F);
         hipGraph_t g = nullptr;
@@ -1454,6 +1466,7 @@ static int run_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
     int64_t key = nrhs * 4 + (fwd ? 2 : 0) + (bwd ? 1 : 0);
     auto it = F->g_sweep.find(key);
     if (it == F->g_sweep.end()) {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
         HIPTRY(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_sweep<T>(F, Xw, nrhs, fwd, bwd);
         hipGraph_t g = nullptr;
@@ -1634,9 +1647,15 @@ static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *
     double *d_out = F->d_norm;
     struct Free {
         void *a, *b;
-        ~Free() { if (a) (void)hipFree(a); if (b) (void)hipFree(b); }
+        ~Free() {
+            if (!a && !b) return;
+            std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+        }
     } fr{nullptr, nullptr};
     if (nseg > 0) {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
         HIPTRY(hipMalloc((void **)&d_seg, sizeof(long long) * (nseg + 1)));
         fr.a = d_seg;
         HIPTRY(hipMalloc((void **)&d_out, sizeof(double) * nseg * nrhs));

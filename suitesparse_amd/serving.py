@@ -342,10 +342,11 @@ def materialize(req: Request, bank: PatternBank, seed: int = 4):
 
 
 def run_stream(server: Server, reqs, arrivals, bank: PatternBank | None = None,
-               lookahead: int = 64, producers: int = 8) -> dict:
+               lookahead: int = 64, producers: int = 8, progress: int = 0) -> dict:
     """Feed requests at their arrival times; return latency / throughput stats.
     Requests built by request_specs are materialized ahead of their arrival
-    by a pool of producer threads, in arrival order, at most `lookahead` ahead."""
+    by a pool of producer threads, in arrival order, at most `lookahead` ahead.
+    progress > 0 prints a line each time that many more requests were submitted."""
     import concurrent.futures
     import queue
     ready: "queue.Queue" = queue.Queue(maxsize=lookahead)
@@ -375,6 +376,9 @@ def run_stream(server: Server, reqs, arrivals, bank: PatternBank | None = None,
         elif delay < -0.005:
             late += 1                     # producer behind the arrival clock
         server.submit(req)
+        if progress and (req.rid + 1) % progress == 0:
+            print(f"[serving] {req.rid + 1} submitted, {len(server.completed)} done at t = "
+                  f"{server.now():.1f} s", flush=True)
     prod.join()
     server.drain()
     done = sorted(server.completed, key=lambda r: r.rid)
