@@ -696,15 +696,20 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
     if (tid == 0) *sfail = -1;
     __syncthreads();
 
+    // Right-looking over 16-column panels, one panel of look-ahead: after the
+    // TRSM of panel p, wave 0 updates only the next diagonal block and factors
+    // it at once, while waves 1..3 apply panel p's other updates and form block
+    // row p of the inverse, whose inputs are final by then:
+    //   L_ii, X_ii = potrf16_wave(S_ii);   L_ip = S_ip X_pp';
+    //   S_ij -= L_ip L_jp' (p < j <= i);   X_pj = -X_pp sum_{k=j}^{p-1} L_pk X_kj (j < p).
+    if (wave == 0) {
+        const int badj = potrf16_wave<T, SAFE>(S, X, DS, 0, nb, lane, adiag, t.col0, rho, npert);
+        if (lane == 0 && badj < 16 && *sfail < 0) *sfail = badj;
+    }
+    __syncthreads();
     for (int p = 0; p < np; p++) {
         const int R = 16 * p;
-        // (1) POTRF + inverse of the 16x16 diagonal block, one wave
-        if (wave == 0) {
-            const int badj = potrf16_wave<T, SAFE>(S, X, DS, R, nb, lane, adiag, t.col0, rho, npert);
-            if (lane == 0 && badj < 16 && *sfail < 0) *sfail = R + badj;
-        }
-        __syncthreads();
-        // (2) panel TRSM: rows below, 16-row tiles t = p+1..np-1, one wave each
+        // (1) panel TRSM: rows below, 16-row tiles p+1..np-1, one wave each
         if (wave < np - 1 - p) {
             const int rt = 16 * (p + 1 + wave);
             acc_t acc = acc_t{0, 0, 0, 0};
@@ -715,12 +720,27 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
                 S[(rt + (lane & 15)) * DS + R + MfmaOp<T>::row(lane, v)] = acc[v];
         }
         __syncthreads();
-        // (3) trailing update of the lower block triangle right of the panel
-        {
+        if (wave == 0) {
+            // (2a) the next diagonal block: its update by panel p, then its POTRF
+            if (p + 1 < np) {
+                const int Q = R + 16;
+                acc_t acc = acc_t{0, 0, 0, 0};
+                mma_nt16<T>(S, Q, R, S, Q, R, acc, lane);
+#pragma unroll
+                for (int v = 0; v < 4; v++) S[(Q + (lane & 15)) * DS + Q + MfmaOp<T>::row(lane, v)] -= acc[v];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                const int badj = potrf16_wave<T, SAFE>(S, X, DS, Q, nb, lane, adiag, t.col0, rho, npert);
+                if (lane == 0 && badj < 16 && *sfail < 0) *sfail = Q + badj;
+            }
+        } else {
+            // (2b) waves 1..3, round robin: panel p's other updates, then block row p of the inverse
             int q = 0;
             for (int i = p + 1; i < np; i++)
-                for (int j = p + 1; j <= i; j++, q++) {
-                    if ((q & 3) != wave) continue;
+                for (int j = p + 1; j <= i; j++) {
+                    if (i == p + 1 && j == p + 1) continue;
+                    if ((q++ % 3) != wave - 1) continue;
                     acc_t acc = acc_t{0, 0, 0, 0};
                     // C(m = row in block j, n = row in block i) = sum_k L_jp(m,k) L_ip(n,k)
                     mma_nt16<T>(S, 16 * j, R, S, 16 * i, R, acc, lane);
@@ -730,26 +750,21 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
                         *dst -= acc[v];
                     }
                 }
-        }
-        __syncthreads();
-    }
-    // off-diagonal blocks of the inverse, block row by block row
-    for (int i = 1; i < np; i++) {
-        if (wave < i) {
-            const int j = wave;
-            acc_t tacc = acc_t{0, 0, 0, 0};
-            for (int k = j; k < i; k++) mma_nn16<T>(S, 16 * i, 16 * k, X, 16 * k, 16 * j, tacc, lane);
-            // stage T = sum_k L_ik X_kj in X's (still zero) block (i, j), then X_ij = -X_ii T
-            store16<T>(X, 16 * i, 16 * j, tacc, lane, (T)1, false);
-        }
-        __syncthreads();
-        if (wave < i) {
-            const int j = wave;
-            acc_t acc = acc_t{0, 0, 0, 0};
-            mma_nn16<T>(X, 16 * i, 16 * i, X, 16 * i, 16 * j, acc, lane);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            store16<T>(X, 16 * i, 16 * j, acc, lane, (T)-1, false);
+            for (int j = 0; j < p; j++) {
+                if ((q++ % 3) != wave - 1) continue;
+                acc_t tacc = acc_t{0, 0, 0, 0};
+                for (int k = j; k < p; k++) mma_nn16<T>(S, R, 16 * k, X, 16 * k, 16 * j, tacc, lane);
+                // stage T = sum_k L_pk X_kj in X's (still zero) block (p, j), then X_pj = -X_pp T
+                store16<T>(X, R, 16 * j, tacc, lane, (T)1, false);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                acc_t acc = acc_t{0, 0, 0, 0};
+                mma_nn16<T>(X, R, R, X, R, 16 * j, acc, lane);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                store16<T>(X, R, 16 * j, acc, lane, (T)-1, false);
+            }
         }
         __syncthreads();
     }
