@@ -132,6 +132,7 @@ struct StepBlocks {
     std::vector<UpdDesc> trail2;   // right of the next panel (second stream)
     bool panel_end = false;        // trail holds the update of the next panel's columns
     std::vector<int64_t> diag_s, trail_s;   // supernode of each diag task / trail block
+    std::vector<int64_t> trail2_s;          // supernode of each trail2 block
 };
 
 struct mxc_factor {
@@ -148,6 +149,7 @@ struct mxc_factor {
     bool split16 = true;                 // FP32 factors: split-FP16 update/trailing tiles
     bool chain = true;                   // solves: one chained launch per level
     bool diag_tails = true;              // trailing launches factor the next diagonal blocks
+    bool merge_t2 = true;                // FP64: fold second-stream updates into later trailing launches
     size_t bytes = 0;
 
     // host copies
@@ -271,6 +273,25 @@ static void drop_factor_graph(mxc_factor *F) {
 // (96^3 27-point FP32 factor: 72.3 ms unbounded, 73.0 ms at 512 tiles).
 constexpr long long PD_MAX_TILES_FP64 = 512;
 
+// Tiles of edge ts over block u (its lower part when lower_only).
+static long long lower_tiles(const UpdDesc &u, int ts, bool lower_only) {
+    const long long mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
+    return lower_only ? mt * nt - nt * (nt - 1) / 2 : mt * nt;
+}
+
+// Columns [a, b) of trailing block u (relative to its first column): the
+// lower trapezoid of those columns, rows from a down.
+static UpdDesc column_slice(const UpdDesc &u, int a, int b) {
+    UpdDesc v = u;
+    v.x_off += a;
+    v.y_off += a;
+    v.c_off += a + (long long)a * u.ldc;
+    v.M = u.M - a;
+    v.N = b - a;
+    v.ls_off = u.ls_off + a;
+    return v;
+}
+
 static int flatten_upload(mxc_factor *F) {
     std::vector<UpdDesc> descs;
     std::vector<Tile> tiles;
@@ -280,8 +301,15 @@ static int flatten_upload(mxc_factor *F) {
     // follow[i] (trailing blocks): the next step's diagonal task of block i's
     // supernode, or null.  A block tiled at 64 or 128 gets it as its tail (pd);
     // fused[i] records that, so the next step's diagonal launch skips it.
+    // force[i] > 0 sets block i's tile edge (merged second-stream slices take the
+    // edge of their supernode's own trailing block, so they share its launch).
     auto emit = [&](int kind, const std::vector<UpdDesc> &blocks, bool lower_only,
-                    const std::vector<const DiagTask *> *follow = nullptr, std::vector<char> *fused = nullptr) {
+                    const std::vector<const DiagTask *> *follow = nullptr, std::vector<char> *fused = nullptr,
+                    const std::vector<int> *force = nullptr) {
+        auto edge = [&](size_t bi) {
+            const UpdDesc &u = blocks[bi];
+            return (force && (*force)[bi] > 0) ? (*force)[bi] : F->tune[kind_of(kind)][tile_bucket(u.M, u.N, u.K)];
+        };
         long long total = 0;
         for (int ts : {32, 64, 128}) {
             long long b = (long long)tiles.size();
@@ -290,21 +318,18 @@ static int flatten_upload(mxc_factor *F) {
             bool allow = follow != nullptr && ts >= 64;
             if (allow) {
                 long long cnt = 0;
-                for (const UpdDesc &u : blocks)
-                    if (F->tune[kind_of(kind)][tile_bucket(u.M, u.N, u.K)] == ts) {
-                        const long long mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
-                        cnt += lower_only ? mt * nt - nt * (nt - 1) / 2 : mt * nt;
-                    }
+                for (size_t bi = 0; bi < blocks.size(); bi++)
+                    if (edge(bi) == ts) cnt += lower_tiles(blocks[bi], ts, lower_only);
                 allow = F->precision != MXC_FP64 || cnt <= PD_MAX_TILES_FP64;
             }
             bool tails = false;
             for (size_t bi = 0; bi < blocks.size(); bi++) {
                 const UpdDesc &u = blocks[bi];
-                if (F->tune[kind_of(kind)][tile_bucket(u.M, u.N, u.K)] != ts) continue;
+                if (edge(bi) != ts) continue;
                 int di = (int)descs.size();
                 descs.push_back(u);
                 descs.back().pd = -1;
-                if (allow && (*follow)[bi]) {
+                if (allow && bi < follow->size() && (*follow)[bi]) {
                     descs.back().pd = (int)diags.size();
                     diags.push_back(*(*follow)[bi]);
                     (*fused)[bi] = 1;
@@ -328,7 +353,13 @@ static int flatten_upload(mxc_factor *F) {
         nupd += emit(L_UPD, F->blk_upd[l], true);
         std::vector<int64_t> fused_s;              // supernodes whose diagonal block of this step is
                                                    // factored by the previous step's trailing launch
-        for (size_t t = 0; t < F->blk_steps[l].size(); t++) {
+        const size_t nst = F->blk_steps[l].size();
+        // second-stream updates merged into later trailing launches: per step, the
+        // column slices and their supernodes
+        std::vector<std::vector<UpdDesc>> merged(nst);
+        std::vector<std::vector<int64_t>> merged_s(nst);
+        auto edge_of = [&](const UpdDesc &u) { return F->tune[kind_of(L_TRAIL)][tile_bucket(u.M, u.N, u.K)]; };
+        for (size_t t = 0; t < nst; t++) {
             const StepBlocks &sb = F->blk_steps[l][t];
             // blocks of at most 16 columns: one wave each; wider blocks: a workgroup each
             for (int small = 1; small >= 0; small--) {
@@ -351,9 +382,53 @@ static int flatten_upload(mxc_factor *F) {
             // the work, also inside a graph capture)
             if (sb.panel_end && level_forked) F->factor_launches.push_back(Launch{L_JOIN, 0, 0, 0});
             if (!sb.trail2.empty()) {
-                F->factor_launches.push_back(Launch{L_FORK, 0, 0, 0});
-                emit(L_TRAIL2, sb.trail2, true);
-                level_forked = true;
+                // FP64: fold the update right of the next panel into the next panel's
+                // three trailing launches, one column slice each, where it runs beside
+                // their diagonal tails (under graph replay the second stream's branch
+                // ran after the main one).  Each slice takes its supernode's own
+                // trailing edge; every merged launch must stay within the tail limit.
+                bool merge = F->merge_t2 && F->precision == MXC_FP64 && t + 3 < nst;
+                std::vector<std::vector<UpdDesc>> add(3);
+                std::vector<std::vector<int64_t>> add_s(3);
+                for (size_t q = 0; merge && q < sb.trail2.size(); q++) {
+                    const UpdDesc &u = sb.trail2[q];
+                    const double M = u.M, tot = (double)u.N * M - (double)u.N * u.N / 2.0;
+                    int cut[4] = {0, 0, 0, u.N};
+                    for (int k = 1; k <= 2; k++) {      // area-balanced cuts on 64-column boundaries
+                        int a = cut[k - 1];
+                        while (a < u.N && ((double)a * M - (double)a * a / 2.0) < tot * k / 3.0) a += NB;
+                        cut[k] = std::min(a, u.N);
+                    }
+                    for (int k = 0; k < 3; k++) {
+                        if (cut[k + 1] <= cut[k]) continue;
+                        add[k].push_back(column_slice(u, cut[k], cut[k + 1]));
+                        add_s[k].push_back(sb.trail2_s[q]);
+                    }
+                }
+                for (int k = 0; merge && k < 3; k++) {
+                    const StepBlocks &tg = F->blk_steps[l][t + 1 + k];
+                    long long cnt[3] = {0, 0, 0};
+                    auto slot = [](int ts) { return ts == 32 ? 0 : ts == 64 ? 1 : 2; };
+                    for (const UpdDesc &u : tg.trail) cnt[slot(edge_of(u))] += lower_tiles(u, edge_of(u), true);
+                    for (size_t q = 0; q < add[k].size(); q++) {
+                        int e = 0;
+                        for (size_t r = 0; r < tg.trail.size(); r++)
+                            if (tg.trail_s[r] == add_s[k][q]) e = edge_of(tg.trail[r]);
+                        if (e == 0) { merge = false; break; }
+                        cnt[slot(e)] += lower_tiles(add[k][q], e, true);
+                    }
+                    for (int c = 0; merge && c < 3; c++) merge = cnt[c] <= PD_MAX_TILES_FP64;
+                }
+                if (merge) {
+                    for (int k = 0; k < 3; k++) {
+                        merged[t + 1 + k].insert(merged[t + 1 + k].end(), add[k].begin(), add[k].end());
+                        merged_s[t + 1 + k].insert(merged_s[t + 1 + k].end(), add_s[k].begin(), add_s[k].end());
+                    }
+                } else {
+                    F->factor_launches.push_back(Launch{L_FORK, 0, 0, 0});
+                    emit(L_TRAIL2, sb.trail2, true);
+                    level_forked = true;
+                }
             }
             // the next step's diagonal blocks, one per trailing block
             std::vector<const DiagTask *> follow(sb.trail.size(), nullptr);
@@ -367,7 +442,22 @@ static int flatten_upload(mxc_factor *F) {
                     if (r < nx.diags.size() && nx.diag_s[r] == sb.trail_s[q]) follow[q] = &nx.diags[r];
                 }
             }
-            emit(L_TRAIL, sb.trail, true, &follow, &fused);
+            if (merged[t].empty()) {
+                emit(L_TRAIL, sb.trail, true, &follow, &fused);
+            } else {
+                // this step's trailing blocks, then the merged slices at their
+                // supernodes' edges
+                std::vector<UpdDesc> blocks(sb.trail);
+                std::vector<int> force(sb.trail.size(), 0);
+                for (size_t q = 0; q < merged[t].size(); q++) {
+                    int e = 0;
+                    for (size_t r = 0; r < sb.trail.size(); r++)
+                        if (sb.trail_s[r] == merged_s[t][q]) e = edge_of(sb.trail[r]);
+                    blocks.push_back(merged[t][q]);
+                    force.push_back(e);
+                }
+                emit(L_TRAIL, blocks, true, &follow, &fused, &force);
+            }
             fused_s.clear();
             for (size_t q = 0; q < sb.trail.size(); q++)
                 if (fused[q]) fused_s.push_back(sb.trail_s[q]);
@@ -637,7 +727,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                         u.ls_off = (int)(S->Lpi[s] + c_lo);
                         u.ldc = (int)nsrow;
                         (part == 0 ? sb.trail : sb.trail2).push_back(u);
-                        if (part == 0) sb.trail_s.push_back(s);
+                        (part == 0 ? sb.trail_s : sb.trail2_s).push_back(s);
                         double n1 = (double)u.N, n3 = (double)(u.M - u.N), kk = (double)u.K;
                         F->flops_trail += kk * n1 * (n1 + 1) + 2.0 * n3 * n1 * kk;
                     }

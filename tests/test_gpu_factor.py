@@ -23,6 +23,7 @@ def _cases():
         ("st27_12_metis", lambda: problems.stencil3d_27pt(12, seed=5), 2),
         ("grid2d_40_amd", lambda: problems.grid2d_lognormal(40, rng), 1),
         ("dense_150", lambda: _dense(150, 3), 3),
+        ("dense_600", lambda: _dense(600, 6), 3),   # 3 panels: merged second-stream update
         ("tiny_1", lambda: _dense(1, 4), 3),
         ("tiny_3", lambda: _dense(3, 4), 3),
     ]
