@@ -101,7 +101,7 @@ def traffic_summary(fetch_dir: str, write_dir: str, calib_mib: int, cmd: str) ->
     lines = ["# HBM traffic of the descendant-update kernel, 64^3 7-point diffusion, FP64", "",
              "Two counter passes over the same command (repository root, one MI355X):", "", "```"]
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
-        lines.append(f"timeout -s KILL 180 rocprofv3 --pmc {counter} -d {d} -o run -- {cmd}")
+        lines.append(f"timeout -s KILL 180 rocprofv3 --pmc {counter} --output-format csv -d {d} -o run -- {cmd}")
     lines += ["```", "",
               f"Calibration: one device-to-device copy of {calib_mib} MiB ({CALIB_KERNEL}) in the same "
               "process; scale = known KiB / counter KiB.", "",
