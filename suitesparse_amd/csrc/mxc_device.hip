@@ -174,7 +174,9 @@ struct mxc_factor {
     double *d_beta = nullptr;
     void *d_work = nullptr;             // T[n * work_nrhs] solve vector
     void *d_part = nullptr;             // T[nslots * 64 * work_nrhs] backward partial sums
-    int *d_cnt = nullptr;               // chained sweeps: finished blocks per supernode (fwd, bwd)
+    int *d_cnt = nullptr;               // chained sweeps: finished blocks per supernode (fwd, bwd), and
+                                        // finished below workgroups per block (bwd)
+    int64_t nbcnt = 0;                  // blocks of the supernodes that are not small
     int *d_abort = nullptr;             // chained sweeps: a wait gave up
     int64_t nslots = 0;
     double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr;   // FP64 n * work_nrhs
@@ -876,9 +878,19 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
         if ((long long)st.size() > b) F->fwd_chain.push_back(Launch{L_FCHAIN, b, (long long)st.size() - b, 0});
     }
     // backward: the below workgroups of a level first (rows below each block,
-    // CHAIN_ROWS at a time, one slot each; slots numbered per level), then the
-    // blocks.  Counters: [ns, 2 ns) blocks, [2 ns, 3 ns) below workgroups done.
+    // CHAIN_ROWS at a time, one slot each; slots numbered per level; the last
+    // block's first, as the chain needs them), then the blocks.  Counters:
+    // [ns, 2 ns) blocks finished per supernode; [3 ns + bc_off[s] + t) below
+    // workgroups done for block t of s.
     const int CHAIN_ROWS = 256;
+    std::vector<int64_t> bc_off(ns, 0);
+    int64_t nbc = 0;
+    for (int64_t s = 0; s < ns; s++)
+        if (!is_small(s)) {
+            bc_off[s] = nbc;
+            nbc += nsteps(s);
+        }
+    F->nbcnt = nbc;
     for (int l = nlev - 1; l >= 0; l--) {
         push_small(l, F->bwd_chain, L_BSMALL);
         int64_t maxsteps = 0;
@@ -894,18 +906,28 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
             int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
             nchunk[q] = (int)ceil_div(nsrow - nscol, CHAIN_ROWS);
             first_slot[q] = slot;
-            for (int64_t t = 0; t < nsteps(s); t++)
-                for (int64_t r = nscol; r < nsrow; r += CHAIN_ROWS) {
+            slot += (int)nsteps(s) * nchunk[q];     // slot of (t, chunk): first + t * nchunk + chunk
+        }
+        // the below workgroups, last block first, then the diag workgroups (below
+        // workgroups placed between the blocks keep slots waiting: slower)
+        for (int64_t k = 0; k < maxsteps; k++)
+            for (size_t q = 0; q < by_level[l].size(); q++) {
+                const int64_t s = by_level[l][q];
+                if (is_small(s) || k >= nsteps(s)) continue;
+                const int64_t t = nsteps(s) - 1 - k;
+                int64_t nscol = S->Super[s + 1] - S->Super[s];
+                int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
+                int c = 0;
+                for (int64_t r = nscol; r < nsrow; r += CHAIN_ROWS, c++) {
                     SweepTask x = make_task(s, t);
                     x.r0 = (int)r;
                     x.r1 = (int)std::min<int64_t>(nsrow, r + CHAIN_ROWS);
-                    x.p0 = (int)(2 * ns + s);
-                    x.np = slot++;
+                    x.p0 = (int)(3 * ns + bc_off[s] + t);
+                    x.np = first_slot[q] + (int)t * nchunk[q] + c;
                     x.pad[0] = 1;
                     st.push_back(x);
                 }
-        }
-        nslot = std::max<int>(nslot, slot);
+            }
         for (int64_t k = 0; k < maxsteps; k++)
             for (size_t q = 0; q < by_level[l].size(); q++) {
                 const int64_t s = by_level[l][q];
@@ -916,8 +938,8 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                 x.np = (int)nsteps(s);
                 x.q0 = first_slot[q] + (int)t * nchunk[q];
                 x.nq = nchunk[q];
-                x.bc = (int)(2 * ns + s);
-                x.bt = (int)nsteps(s) * nchunk[q];
+                x.bc = (int)(3 * ns + bc_off[s] + t);
+                x.bt = nchunk[q];
                 st.push_back(x);
             }
         if ((long long)st.size() > b) F->bwd_chain.push_back(Launch{L_BCHAIN, b, (long long)st.size() - b, 0});
@@ -951,7 +973,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     F->Ai_host.assign(Ai, Ai + nnz);
     if ((rc = dalloc(F, &F->d_norm, 64))) return rc;
     if ((rc = dalloc(F, &F->d_beta, 1))) return rc;
-    if ((rc = dalloc(F, &F->d_cnt, (size_t)(3 * ns)))) return rc;
+    if ((rc = dalloc(F, &F->d_cnt, (size_t)(3 * ns + F->nbcnt)))) return rc;
     if ((rc = dalloc(F, &F->d_abort, 1))) return rc;
     HIPTRY(hipMemsetAsync(F->d_abort, 0, sizeof(int), sm));
     HIPTRY(hipStreamSynchronize(sm));
@@ -1491,9 +1513,10 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
             for (int z0 = 0; z0 < nr; z0 += CHAIN_RHS) {
                 const int zn = std::min(CHAIN_RHS, nr - z0);
                 T *Xz = Xw + (size_t)z0 * (size_t)n;
+                const int ncnt = 3 * ns + (int)F->nbcnt;
                 if (ns > 0)
-                    hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(3 * (long long)ns, 256, 1LL << 30)), dim3(256),
-                                       0, sm, F->d_cnt, 3 * ns);
+                    hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(ncnt, 256, 1LL << 30)), dim3(256),
+                                       0, sm, F->d_cnt, ncnt);
                 for (const Launch &L : dir == 0 ? F->fwd_chain : F->bwd_chain) {
                     if (L.kind == L_FSMALL)
                         hipLaunchKernelGGL(k_sweep_small_fwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256),
