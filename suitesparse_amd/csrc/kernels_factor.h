@@ -196,9 +196,11 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
     constexpr int PER = (BK * TS) / NT;         // elements of X (and of Y) each thread stages
     static_assert(!PD || (MODE == MODE_TRAIL && TS >= 64), "diagonal tails: trailing tiles of 64 or 128");
     // one LDS buffer: the staged X / Y chunks, reused by the diagonal-block tail.
-    // A one-wave tile (32) stages into a single buffer: its barriers cost
-    // little, and half the LDS doubles the resident tiles per CU.
-    constexpr int NBUF = TS == 32 ? 1 : 2;
+    // Tiles of 32 and 64 stage into a single buffer: the extra barrier per
+    // chunk costs less than the resident tiles that half the LDS adds (64³ FP64
+    // update kernel 28.1 -> 30.2 TFLOP/s); a 128 tile is register-bound at two
+    // waves per SIMD either way and keeps two buffers.
+    constexpr int NBUF = TS == 128 ? 2 : 1;
     constexpr size_t XY_BYTES = sizeof(T) * NBUF * BK * (TS + LDS_PAD);
     constexpr size_t PD_BYTES = PD ? sizeof(T) * 2 * NB * DS : 0;
     constexpr size_t SMEM = 2 * XY_BYTES > PD_BYTES ? 2 * XY_BYTES : PD_BYTES;
