@@ -176,10 +176,11 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * that it overlaps the next panel's diagonal-block steps.  0: one stream. */
 #define MXC_OPT_LOOKAHEAD 3
 /* MXC_OPT_SPLIT16 (default 1, FP32 factors only): the descendant-update and
- * trailing tiles compute each product as hi*hi + hi*lo + lo*hi
- * of FP16 halves on the FP16 MFMA, FP32 accumulation, after a per-row
- * power-of-two scaling by the assembled diagonal (|L_ij| <= sqrt(a_ii)).
- * 0: FP32 MFMA. */
+ * trailing tiles of 64 and 128 columns compute each product as
+ * hi*hi + hi*lo + lo*hi of FP16 halves on the FP16 MFMA, FP32 accumulation,
+ * after a per-row power-of-two scaling by the assembled diagonal
+ * (|L_ij| <= sqrt(a_ii)); the TRSM stores each value's pair once.  32-column
+ * tiles, and all tiles with 0, use the FP32 MFMA. */
 #define MXC_OPT_SPLIT16 4
 /* MXC_OPT_SOLVE_CHAIN (default 1): the solve sweeps take one launch per etree
  * level; inside it each 64-column block of a supernode waits, through a

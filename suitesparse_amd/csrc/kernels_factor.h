@@ -356,16 +356,16 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 
 template <int MODE, int TS, bool PD = false>
-__global__ __launch_bounds__(TS == 32 ? 64 : 256, 2) void k_tile_h(const UpdDesc *__restrict__ descs,
+__global__ __launch_bounds__(256, 2) void k_tile_h(const UpdDesc *__restrict__ descs,
                                                                   const Tile *__restrict__ tiles,
                                                                   float *__restrict__ W,
                                                                   const unsigned *__restrict__ Lh,
                                                                   const int *__restrict__ Ls,
                                                                   const double *__restrict__ adiag,
                                                                   PdArgs pda = PdArgs{}) {
-    static_assert(TS == 32 || TS == 64 || TS == 128, "split-FP16 tiles are 32, 64 or 128 wide");
-    constexpr int NT = TS == 32 ? 64 : 256;  // one wave for a 32-wide tile, else 2 x 2 waves
-    constexpr int WB = TS == 32 ? 32 : TS / 2;   // edge of a wave's block of C
+    static_assert(TS == 64 || TS == 128, "split-FP16 tiles are 64 or 128 wide");
+    constexpr int NT = 256;              // 2 x 2 waves
+    constexpr int WB = TS / 2;           // edge of a wave's block of C
     constexpr int MB = WB / 16;
     constexpr int BKH = 32;              // K per chunk: one MFMA depth
     constexpr int RS = BKH + 8;          // LDS row stride (halves)
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, 2) void k_tile_h(const UpdDesc
     const UpdDesc d = descs[tl.desc];
     const int i0 = tl.ti * TS, j0 = tl.tj * TS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = TS == 32 ? 0 : (wave & 1), wc = TS == 32 ? 0 : (wave >> 1);
+    const int wr = wave & 1, wc = wave >> 1;
 
     for (int e = tid; e < 2 * TS; e += NT) {
         if (e < TS) {
