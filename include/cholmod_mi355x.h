@@ -171,7 +171,7 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * FP64 fallback factor, corrects the perturbation.  rho = 0 keeps CHOLMOD's
  * semantics: a pivot <= 0 stops with MXC_NOT_POSDEF. */
 #define MXC_OPT_PIVOT_RHO_E9 2
-/* MXC_OPT_LOOKAHEAD (default 1): run the update of each finished 256-column
+/* MXC_OPT_LOOKAHEAD (default 1): run the update of each finished 384-column
  * panel of a large supernode right of the next panel on a second stream, so
  * that it overlaps the next panel's diagonal-block steps.  0: one stream. */
 #define MXC_OPT_LOOKAHEAD 3

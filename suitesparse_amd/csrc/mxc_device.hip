@@ -246,7 +246,7 @@ static int dupload(mxc_factor *F, X **p, const std::vector<X> &h, hipStream_t s)
 static inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Outer panel width of the in-supernode blocked Cholesky (a multiple of NB).
-static constexpr int64_t PANEL = 256;
+static constexpr int64_t PANEL = 384;
 
 // local row of global row g inside supernode s (rows Ls[psi..psi+nsrow) sorted,
 // first nscol of them = k1..k2-1)
