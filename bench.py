@@ -46,6 +46,22 @@ def relres(A, B, X) -> float:
     return float((np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)).max())
 
 
+def backward_error(A, B, X) -> float:
+    """|Ax - b| / (|A| |x| + |b|) in infinity norms (largest absolute row sum),
+    the residual cholmod_dl_demo.c:614-617 reports; |A| of the full symmetric A."""
+    B2 = np.asarray(B).reshape(A.n, -1)
+    X2 = np.asarray(X).reshape(A.n, -1)
+    R = B2 - A.matvec(X2)
+    col = np.repeat(np.arange(A.n), np.diff(A.p))
+    ax = np.abs(A.x)
+    off = A.i != col
+    anorm = (np.bincount(A.i, ax, A.n) + np.bincount(col[off], ax[off], A.n)).max()
+
+    def inf(M):
+        return float(np.abs(M).sum(axis=1).max())
+    return float(inf(R) / (anorm * inf(X2) + inf(B2)))
+
+
 TUNING = ROOT / "suitesparse_amd" / "tuning"
 PROFILES = ROOT / "profiles"
 TRAFFIC_FILE = PROFILES / "traffic_update_diffusion3d_64_fp64.md"
@@ -343,7 +359,7 @@ def main():
                        "p99": round(float(np.percentile(lat, 99)), 3)},
         "t_factor_ms": round(g["t_factor"] * 1e3, 3), "t_solve_ms": round(g["t_solve"] * 1e3, 3),
         "factor_gflops": round(fl / g["t_factor"] / 1e9, 2), "tiles": g["tiles"],
-        "relres": rr, "fwd_err": fe, "roofline": roof,
+        "relres": rr, "fwd_err": fe, "backward_error": backward_error(A, B, g["X"]), "roofline": roof,
         "breakdown_ms": {k: (round(v, 3) if k != "launches" else v)
                          for k, v in (g["breakdown"] or {}).items()},
     }
@@ -393,7 +409,8 @@ def mixed_section(args, out):
                          "t_solve_ms": round(r["t_solve"] * 1e3, 2),
                          "factor_gflops": round(r["L"].fl / r["t_factor"] / 1e9, 1),
                          "factor_solve_gflops": round((r["L"].fl + 4 * r["L"].lnz * B3.shape[1]) / tts / 1e9, 1),
-                         "relres": rr3, "refine": r["refine"], "tiles": r["tiles"]}
+                         "relres": rr3, "backward_error": backward_error(A3, B3, r["X"]),
+                         "refine": r["refine"], "tiles": r["tiles"]}
             log(f"[bench]   {mix[prec]}")
             del r
             import gc

@@ -29,6 +29,9 @@
  *                          CHOLMOD/Cholesky/cholmod_solve.c:656-724
  *   mxc_super_lsolve       cholmod_l_super_lsolve  cholmod.h:4047
  *   mxc_super_ltsolve      cholmod_l_super_ltsolve cholmod.h:4068
+ *   mxc_solve_bset         cholmod_l_solve2 cholmod.h:2845 with a sparse right-hand side
+ *                          (Bset / Xset, CHOLMOD/Cholesky/cholmod_solve.c:290-640)
+ *   mxc_lsolve_pattern     cholmod_l_lsolve_pattern (CHOLMOD/Cholesky/cholmod_rowfac.c:383)
  *   mxc_residual           cholmod_l_sdmult(A, 0, -1, 1, X, B) used for r = b - A x
  *                          (CHOLMOD/MatrixOps/cholmod_sdmult.c:62)
  *   mxc_solve_refine       the iterative-refinement loop of CHOLMOD/Demo/cholmod_dl_demo.c:627-663,
@@ -139,6 +142,34 @@ int mxc_solve(mxc_factor *F, const double *B, int64_t nrhs, double *X, int locat
  * (cholmod_l_super_lsolve / ltsolve). */
 int mxc_super_lsolve(mxc_factor *F, double *X, int64_t nrhs, int location);
 int mxc_super_ltsolve(mxc_factor *F, double *X, int64_t nrhs, int location);
+
+/* System codes of cholmod_l_solve / solve2 (cholmod.h:2805-2813). */
+#define MXC_SYS_A 0
+#define MXC_SYS_LDLt 1
+#define MXC_SYS_LD 2
+#define MXC_SYS_DLt 3
+#define MXC_SYS_L 4
+#define MXC_SYS_Lt 5
+#define MXC_SYS_D 6
+
+/* cholmod_l_solve2 (cholmod.h:2845) with Bset: one right-hand side b (dense,
+ * n entries, host) of which only b[Bset[0..blen)] is read.  sys is
+ * MXC_SYS_A .. MXC_SYS_D (an LL' factor: D = I; CHOLMOD ignores Bset for
+ * P and P').  On return Xset[0..*xlen) is the pattern the reference returns
+ * in Xset, in its order (the columns of L reached from Bset, permuted back
+ * for MXC_SYS_A), and X (host, n entries) holds the solution at those rows;
+ * its other entries are not written.  Xset must hold n entries.  L stays
+ * supernodal (the reference converts it to simplicial; the pattern and
+ * values returned are the same). */
+int mxc_solve_bset(mxc_factor *F, int sys, const double *B, const int64_t *Bset, int64_t blen,
+                   double *X, int64_t *Xset, int64_t *xlen);
+
+/* The pattern alone (host only, no device): Xset[0..*xlen) as mxc_solve_bset
+ * returns it for these sys and Bset -- cholmod_l_lsolve_pattern
+ * (CHOLMOD/Cholesky/cholmod_rowfac.c:383-398) on the simplicial form of L,
+ * permuted as cholmod_l_solve2 permutes it.  Xset must hold n entries. */
+int mxc_lsolve_pattern(const mxc_symbolic *S, int sys, const int64_t *Bset, int64_t blen, int64_t *Xset,
+                       int64_t *xlen);
 
 /* Reciprocal condition estimate of the last factorization, as
  * cholmod_l_rcond (CHOLMOD/Cholesky/cholmod_rcond.c:51-112) computes it for an

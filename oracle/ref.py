@@ -49,6 +49,8 @@ def lib():
         L.ref_get_Lx.argtypes = [C.c_void_p, _f64p]
         L.ref_solve.argtypes = [C.c_void_p, _f64p, C.c_int64, _f64p, C.POINTER(C.c_double)]
         L.ref_residual.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int64, _f64p]
+        L.ref_solve2_bset.argtypes = [C.c_void_p, C.c_int, _f64p, _i64p, C.c_int64, _f64p, _i64p,
+                                      C.POINTER(C.c_int64)]
         L.ref_free.argtypes = [C.c_void_p]
         L.ref_free.restype = None
         L.ref_read_sparse.argtypes = [C.c_char_p] + [C.c_void_p] * 4
@@ -144,6 +146,20 @@ class RefFactor:
             raise RuntimeError(f"reference solve failed ({rc})")
         self.t_solve = t.value
         return x.reshape(nrhs, self.n).T.copy()
+
+    def solve2_bset(self, sys: int, b: np.ndarray, bset) -> tuple[np.ndarray, np.ndarray]:
+        """cholmod_l_solve2 with Bset: (X, Xset); X is NaN off Xset.  Leaves the
+        factor simplicial (the reference converts it), so call it last."""
+        bb = np.ascontiguousarray(np.asarray(b, dtype=np.float64).reshape(-1))
+        bs = np.ascontiguousarray(bset, dtype=np.int64).reshape(-1)
+        x = np.full(self.n, np.nan)
+        xs = np.zeros(self.n, dtype=np.int64)
+        ln = C.c_int64(0)
+        rc = lib().ref_solve2_bset(self._h, int(sys), bb, bs if bs.size else np.zeros(1, np.int64),
+                                   bs.size, x, xs, C.byref(ln))
+        if rc != 0:
+            raise RuntimeError(f"reference solve2 failed ({rc})")
+        return x, xs[:ln.value].copy()
 
     def residual(self, B: np.ndarray, X: np.ndarray) -> np.ndarray:
         B2 = np.atleast_2d(np.asarray(B, dtype=np.float64).T).T

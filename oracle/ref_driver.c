@@ -10,6 +10,7 @@
 //   cholmod_l_analyze / cholmod_l_analyze_p  cholmod.h:2714, :2734
 //   cholmod_l_factorize                      cholmod.h:2775
 //   cholmod_l_solve                          cholmod.h:2823
+//   cholmod_l_solve2 (sparse b, Bset)        cholmod.h:2845
 //   cholmod_l_sdmult (residual)              cholmod.h (MatrixOps)
 //   cholmod_l_read_sparse2 (file reader)     CHOLMOD/Check/cholmod_read.c
 // Input matrices are the lower triangle of a symmetric matrix in CSC form
@@ -197,6 +198,47 @@ int ref_solve(void *vh, const double *B, int64_t nrhs, double *X, double *t_solv
     }
     cholmod_l_free_dense(&Xd, cm);
     return 0;
+}
+
+// cholmod_l_solve2 (cholmod.h:2845) with a sparse right-hand side: b dense
+// (n entries), Bset = rows bset[0..blen).  xset[0..*xlen) receives Xset in the
+// reference's order and x the solution at those rows (other entries are not
+// written).  The reference converts the factor to simplicial here, so this is
+// the last call to make on a handle.
+int ref_solve2_bset(void *vh, int sys, const double *b, const int64_t *bset, int64_t blen,
+                    double *x, int64_t *xset, int64_t *xlen) {
+    ref_handle *h = (ref_handle *)vh;
+    cholmod_common *cm = &h->cm;
+    if (h->dtype == CHOLMOD_SINGLE) return -1;
+    int64_t n = (int64_t)h->L->n;
+    cholmod_dense *Bd = cholmod_l_allocate_dense(n, 1, n, CHOLMOD_REAL, cm);
+    cholmod_sparse *Bs = cholmod_l_allocate_sparse(n, 1, blen > 0 ? blen : 1, 0, 1, 0, CHOLMOD_PATTERN, cm);
+    if (!Bd || !Bs) return -2;
+    memcpy(Bd->x, b, sizeof(double) * n);
+    int64_t *bp = (int64_t *)Bs->p, *bi = (int64_t *)Bs->i;
+    bp[0] = 0;
+    bp[1] = blen;
+    if (blen > 0) memcpy(bi, bset, sizeof(int64_t) * blen);
+    cholmod_dense *X = NULL, *Y = NULL, *E = NULL;
+    cholmod_sparse *Xs = NULL;
+    int ok = cholmod_l_solve2(sys, h->L, Bd, Bs, &X, &Xs, &Y, &E, cm);
+    int rc = ok ? 0 : (cm->status != 0 ? cm->status : -99);
+    if (ok) {
+        const int64_t *xp = (const int64_t *)Xs->p, *xi = (const int64_t *)Xs->i;
+        const double *xv = (const double *)X->x;
+        *xlen = xp[1];
+        for (int64_t p = 0; p < xp[1]; p++) {
+            xset[p] = xi[p];
+            x[xi[p]] = xv[xi[p]];
+        }
+    }
+    cholmod_l_free_dense(&Bd, cm);
+    cholmod_l_free_sparse(&Bs, cm);
+    cholmod_l_free_dense(&X, cm);
+    cholmod_l_free_dense(&Y, cm);
+    cholmod_l_free_dense(&E, cm);
+    cholmod_l_free_sparse(&Xs, cm);
+    return rc;
 }
 
 // R = B - A*X using the reference's cholmod_l_sdmult on the double-valued A.

@@ -44,6 +44,8 @@ _PROTOS = {
     "mxc_solve": (i32, [vp, vp, i64, vp, i32]),
     "mxc_super_lsolve": (i32, [vp, vp, i64, i32]),
     "mxc_super_ltsolve": (i32, [vp, vp, i64, i32]),
+    "mxc_solve_bset": (i32, [vp, i32, vp, vp, i64, vp, vp, vp]),
+    "mxc_lsolve_pattern": (i32, [vp, i32, vp, i64, vp, vp]),
     "mxc_residual": (i32, [vp, vp, vp, i64, vp, i32]),
     "mxc_solve_refine": (i32, [vp, vp, i64, vp, i32, f64, i32, vp]),
     "mxc_solve_refine_segments": (i32, [vp, vp, i64, vp, i32, f64, i32, i64, vp, vp]),

@@ -330,9 +330,7 @@ def solve(sys: int, L: Factor, B, cm: Common):
         import torch
         Bc = B if B.dim() == 2 else B.reshape(-1, 1)
         Bc = Bc.t().contiguous().t() if not Bc.t().is_contiguous() else Bc   # column-major
-        nrhs = Bc.shape[1]
         X = torch.empty_like(Bc.t().contiguous()).t()
-        bptr, xptr, loc = Bc, X, _lib.DEVICE
         if Bc.shape[0] != L.n:
             cm.status = CHOLMOD_INVALID
             return None
@@ -343,10 +341,21 @@ def solve(sys: int, L: Factor, B, cm: Common):
         if B2.shape[0] != L.n:
             cm.status = CHOLMOD_INVALID
             return None
-        nrhs = B2.shape[1]
         Bc = np.asfortranarray(B2)
         X = np.zeros_like(Bc, order="F")
-        bptr, xptr, loc = Bc, X, _lib.HOST
+    if not _solve_into(sys, L, Bc, X, cm):
+        return None
+    if not dev and np.ndim(B) == 1:
+        return X[:, 0].copy()
+    return X
+
+
+def _solve_into(sys: int, L: Factor, Bc, X, cm: Common) -> bool:
+    """The body of solve / solve2: Bc and X column-major n x nrhs, both host
+    arrays or both device tensors."""
+    dev = _is_device(Bc)
+    nrhs = Bc.shape[1]
+    bptr, xptr, loc = Bc, X, (_lib.DEVICE if dev else _lib.HOST)
     h = L._num
     if sys in (CHOLMOD_A, CHOLMOD_LDLt):
         st = lib().mxc_solve(h, _ptr(bptr), nrhs, _ptr(xptr), loc)
@@ -380,13 +389,90 @@ def solve(sys: int, L: Factor, B, cm: Common):
                 X[perm] = Bc
     else:
         cm.status = CHOLMOD_INVALID
-        return None
+        return False
     if st != 0:
         cm.status = st
-        return None
-    if not dev and np.ndim(B) == 1:
-        return X[:, 0].copy()
-    return X
+        return False
+    return True
+
+
+def solve2(sys: int, L: Factor, B, cm: Common, Bset=None, X=None):
+    """cholmod_l_solve2 (cholmod.h:2845; CHOLMOD/Cholesky/cholmod_solve.c:130-700).
+
+    Without Bset: as solve(), except that X is reused when it is an array (or
+    device tensor) of the right shape, float64 and column-major -- the
+    reference's ensure_dense -- so a sequence of solves allocates no new X.
+    The reference's workspaces Y and E live in the device factor and are
+    reused by every call.  Returns (X, None).
+
+    With Bset (row indices; B a single column, of which only B[Bset] is read):
+    the sparse right-hand-side solve, on host arrays.  Returns (X, Xset): Xset
+    holds the rows where X is defined, in the order the reference returns
+    them; X's other entries are not written (a reused X keeps its values, a
+    new X holds zeros).  Bset is ignored for CHOLMOD_P / CHOLMOD_Pt, as in the
+    reference.  On failure returns (None, None) with cm.status set.
+    """
+    cm.status = CHOLMOD_OK
+    if L is None or not L.is_numeric:
+        cm.status = CHOLMOD_INVALID
+        return None, None
+    if sys in (CHOLMOD_P, CHOLMOD_Pt):
+        Bset = None
+    dev = _is_device(B)
+    if Bset is None:
+        if dev:
+            Bc = B if B.dim() == 2 else B.reshape(-1, 1)
+            Bc = Bc.t().contiguous().t() if not Bc.t().is_contiguous() else Bc
+            reuse = (X is not None and _is_device(X) and tuple(X.shape) == tuple(Bc.shape)
+                     and X.dtype == Bc.dtype and X.t().is_contiguous())
+            if not reuse:
+                import torch
+                X = torch.empty_like(Bc.t().contiguous()).t()
+        else:
+            B2 = np.asarray(B, dtype=np.float64)
+            B2 = B2.reshape(L.n, -1) if B2.ndim == 1 else B2
+            Bc = np.asfortranarray(B2)
+            reuse = (isinstance(X, np.ndarray) and X.shape == Bc.shape and X.dtype == np.float64
+                     and X.flags.f_contiguous)
+            if not reuse:
+                X = np.zeros_like(Bc, order="F")
+        if Bc.shape[0] != L.n:
+            cm.status = CHOLMOD_INVALID
+            return None, None
+        if not _solve_into(sys, L, Bc, X, cm):
+            return None, None
+        return X, None
+    if dev:
+        cm.status = CHOLMOD_INVALID          # the reference's Bset path is host-only
+        return None, None
+    b = np.ascontiguousarray(np.asarray(B, dtype=np.float64).reshape(L.n, -1))
+    if b.shape[1] != 1:
+        cm.status = CHOLMOD_INVALID          # "Bset requires a single right-hand side"
+        return None, None
+    bs = np.ascontiguousarray(Bset, dtype=np.int64).reshape(-1)
+    if not (isinstance(X, np.ndarray) and X.dtype == np.float64 and X.size == L.n
+            and X.flags.c_contiguous):
+        X = np.zeros(L.n)
+    xset = np.zeros(L.n, dtype=np.int64)
+    xlen = C.c_int64(0)
+    st = lib().mxc_solve_bset(L._num, int(sys), _ptr(b), _ptr(bs), bs.size, _ptr(X), _ptr(xset),
+                              C.byref(xlen))
+    if st != 0:
+        cm.status = st
+        return None, None
+    return X, xset[:xlen.value].copy()
+
+
+def lsolve_pattern(L: Factor, Bset, sys: int = CHOLMOD_A) -> np.ndarray:
+    """The Xset that solve2(sys, L, b, cm, Bset) returns, from the symbolic
+    factor alone (host; cholmod_l_lsolve_pattern, cholmod_rowfac.c:383, on the
+    simplicial form of L, permuted back for CHOLMOD_A as cholmod_l_solve2 does)."""
+    bs = np.ascontiguousarray(Bset, dtype=np.int64).reshape(-1)
+    xset = np.zeros(max(L.n, 1), dtype=np.int64)
+    xlen = C.c_int64(0)
+    check("mxc_lsolve_pattern", lib().mxc_lsolve_pattern(L._sym, int(sys), _ptr(bs), bs.size, _ptr(xset),
+                                                         C.byref(xlen)))
+    return xset[:xlen.value].copy()
 
 
 def solve_refine(L: Factor, B, cm: Common, tol: float = 1e-10, maxiter: int = 10, segments=None):

@@ -192,6 +192,7 @@ struct mxc_factor {
     // FP64 fallback factor of a low-precision factor whose refinement stalls
     mxc_factor *fallback = nullptr;
     std::vector<int64_t> Ls_host, Ap_host, Ai_host;
+    std::vector<int64_t> IPerm;              // built on the first sparse right-hand-side solve
     // host copy of the factorization's blocks (re-flattened after autotuning)
     std::vector<std::vector<UpdDesc>> blk_upd;        // per level
     std::vector<std::vector<StepBlocks>> blk_steps;   // per level, per 64-column step
@@ -1649,7 +1650,7 @@ extern "C" int mxc_solve(mxc_factor *F, const double *B, int64_t nrhs, double *X
     return chain_status(F);
 }
 
-static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, bool fwd) {
+static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, bool fwd, bool bwd) {
     if (!F || !X || nrhs < 0) return MXC_INVALID;
     if (!F->values_loaded) return MXC_INVALID;
     if (nrhs == 0 || F->n == 0) return MXC_OK;
@@ -1662,7 +1663,7 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
         HIPTRY(hipMemcpyAsync(F->d_x, X, len * sizeof(double), hipMemcpyHostToDevice, sm));
     double *dX = location == MXC_DEVICE ? X : F->d_x;
     if (F->precision == MXC_FP64) {
-        rc = run_sweep<double>(F, dX, nrhs, fwd, !fwd);
+        rc = run_sweep<double>(F, dX, nrhs, fwd, bwd);
     } else {
         float *Xw = (float *)F->d_work;
         std::vector<float> tmp;
@@ -1671,7 +1672,7 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
         HIPTRY(hipStreamSynchronize(sm));
         tmp.assign(h.begin(), h.end());
         HIPTRY(hipMemcpyAsync(Xw, tmp.data(), len * sizeof(float), hipMemcpyHostToDevice, sm));
-        rc = run_sweep<float>(F, Xw, nrhs, fwd, !fwd);
+        rc = run_sweep<float>(F, Xw, nrhs, fwd, bwd);
         if (rc) return rc;
         HIPTRY(hipMemcpyAsync(tmp.data(), Xw, len * sizeof(float), hipMemcpyDeviceToHost, sm));
         HIPTRY(hipStreamSynchronize(sm));
@@ -1686,10 +1687,59 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
 }
 
 extern "C" int mxc_super_lsolve(mxc_factor *F, double *X, int64_t nrhs, int location) {
-    return super_sweep(F, X, nrhs, location, true);
+    return super_sweep(F, X, nrhs, location, true, false);
 }
 extern "C" int mxc_super_ltsolve(mxc_factor *F, double *X, int64_t nrhs, int location) {
-    return super_sweep(F, X, nrhs, location, false);
+    return super_sweep(F, X, nrhs, location, false, true);
+}
+
+// Sparse right-hand side (cholmod_l_solve2 with Bset, cholmod_solve.c:290-640).
+// Yset = the columns of L reached from C = the (permuted) Bset, as
+// cholmod_l_lsolve_pattern orders them (mxc_reach, symbolic.c).  Yset holds
+// every parent of its members, so the solution over
+// Yset needs no other column: the dense device solve restricted to Yset gives
+// those values.  X receives them at Xset = P(Yset) (bset_iperm,
+// t_cholmod_solve_worker.c:252-283); its other entries are left as they were.
+extern "C" int mxc_solve_bset(mxc_factor *F, int sys, const double *B, const int64_t *Bset, int64_t blen,
+                              double *X, int64_t *Xset, int64_t *xlen) {
+    if (!F || !B || !X || !Xset || !xlen || blen < 0 || (blen > 0 && !Bset)) return MXC_INVALID;
+    if (sys < MXC_SYS_A || sys > MXC_SYS_D) return MXC_INVALID;
+    if (!F->values_loaded) return MXC_INVALID;
+    const int64_t n = F->n;
+    for (int64_t p = 0; p < blen; p++)
+        if (Bset[p] < 0 || Bset[p] >= n) return MXC_INVALID;
+    const bool perm = sys == MXC_SYS_A;
+    if (F->IPerm.size() != (size_t)n) {
+        F->IPerm.assign(n, 0);
+        for (int64_t k = 0; k < n; k++) F->IPerm[F->Perm[k]] = k;
+    }
+    std::vector<int64_t> C(blen), Y;
+    for (int64_t p = 0; p < blen; p++) C[p] = perm ? F->IPerm[Bset[p]] : Bset[p];
+    if (sys == MXC_SYS_D) {
+        Y = C;                     // D = I for an LL' factor
+    } else {
+        Y.resize(n);
+        int64_t ylen = 0;
+        int rc = mxc_reach(n, F->nsuper, F->Super.data(), F->Lpi.data(), F->Ls_host.data(), C.data(), blen,
+                           Y.data(), &ylen);
+        if (rc) return rc;
+        Y.resize(ylen);
+    }
+    std::vector<double> y(n, 0.0);
+    for (int64_t p = 0; p < blen; p++) y[C[p]] = B[Bset[p]];
+    if (sys != MXC_SYS_D) {
+        const bool fwd = sys == MXC_SYS_A || sys == MXC_SYS_LDLt || sys == MXC_SYS_LD || sys == MXC_SYS_L;
+        const bool bwd = sys == MXC_SYS_A || sys == MXC_SYS_LDLt || sys == MXC_SYS_DLt || sys == MXC_SYS_Lt;
+        int rc = super_sweep(F, y.data(), 1, MXC_HOST, fwd, bwd);
+        if (rc) return rc;
+    }
+    for (size_t p = 0; p < Y.size(); p++) {
+        const int64_t iold = perm ? F->Perm[Y[p]] : Y[p];
+        X[iold] = y[Y[p]];
+        Xset[p] = iold;
+    }
+    *xlen = (int64_t)Y.size();
+    return MXC_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -533,6 +533,78 @@ int mxc_symbolic_export(const mxc_symbolic *L, int64_t *Perm, int64_t *ColCount,
     return MXC_OK;
 }
 
+// The columns of L reached from the columns C[0..blen) (the pattern of L \ b
+// for b nonzero at C), in the order cholmod_l_lsolve_pattern returns them
+// (cholmod_rowfac.c:383-398 -> row_lsubtree :127-153, :563-568): L taken as
+// the reference's simplicial conversion of the supernodal factor (explicit
+// zeros kept), whose parent of column j is its first off-diagonal row
+// (cholmod_rowfac.c:539), i.e. j + 1 inside j's supernode, else the
+// supernode's first row below its columns, else none.  Each new path from a
+// C entry up to the first column already reached is placed, bottom-up, in
+// front of the earlier paths.  Yset needs room for n entries.
+int mxc_reach(int64_t n, int64_t nsuper, const int64_t *Super, const int64_t *Lpi, const int64_t *Ls,
+              const int64_t *C, int64_t blen, int64_t *Yset, int64_t *ylen) {
+    int64_t *col_super = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    int64_t *path = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    char *seen = (char *)calloc((size_t)(n > 0 ? n : 1), 1);
+    if (!col_super || !path || !seen) {
+        free(col_super); free(path); free(seen);
+        return MXC_OUT_OF_MEMORY;
+    }
+    for (int64_t s = 0; s < nsuper; s++)
+        for (int64_t k = Super[s]; k < Super[s + 1]; k++) col_super[k] = s;
+    int64_t top = n;
+    for (int64_t p = 0; p < blen; p++) {
+        int64_t len = 0;
+        for (int64_t i = C[p]; i != -1 && !seen[i];) {
+            path[len++] = i;
+            seen[i] = 1;
+            const int64_t s = col_super[i];
+            if (i + 1 < Super[s + 1]) {
+                i = i + 1;
+            } else {
+                const int64_t nscol = Super[s + 1] - Super[s], nsrow = Lpi[s + 1] - Lpi[s];
+                i = nsrow > nscol ? Ls[Lpi[s] + nscol] : -1;
+            }
+        }
+        while (len > 0) Yset[--top] = path[--len];
+    }
+    *ylen = n - top;
+    memmove(Yset, Yset + top, sizeof(int64_t) * (size_t)(n - top));
+    free(col_super); free(path); free(seen);
+    return MXC_OK;
+}
+
+int mxc_lsolve_pattern(const mxc_symbolic *S, int sys, const int64_t *Bset, int64_t blen, int64_t *Xset,
+                       int64_t *xlen) {
+    if (!S || !Xset || !xlen || blen < 0 || (blen > 0 && !Bset)) return MXC_INVALID;
+    if (sys < MXC_SYS_A || sys > MXC_SYS_D) return MXC_INVALID;
+    const int64_t n = S->n;
+    for (int64_t p = 0; p < blen; p++)
+        if (Bset[p] < 0 || Bset[p] >= n) return MXC_INVALID;
+    const int perm = sys == MXC_SYS_A && S->Perm != NULL;
+    int64_t *C = (int64_t *)malloc(sizeof(int64_t) * (size_t)(blen > 0 ? blen : 1));
+    int64_t *Pinv = perm ? (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1)) : NULL;
+    if (!C || (perm && !Pinv)) {
+        free(C); free(Pinv);
+        return MXC_OUT_OF_MEMORY;
+    }
+    if (perm)
+        for (int64_t k = 0; k < n; k++) Pinv[S->Perm[k]] = k;
+    for (int64_t p = 0; p < blen; p++) C[p] = perm ? Pinv[Bset[p]] : Bset[p];
+    int rc = MXC_OK;
+    if (sys == MXC_SYS_D) {
+        for (int64_t p = 0; p < blen; p++) Xset[p] = C[p];
+        *xlen = blen;
+    } else {
+        rc = mxc_reach(n, S->nsuper, S->Super, S->Lpi, S->Ls, C, blen, Xset, xlen);
+    }
+    if (rc == MXC_OK && perm)
+        for (int64_t p = 0; p < *xlen; p++) Xset[p] = S->Perm[Xset[p]];
+    free(C); free(Pinv);
+    return rc;
+}
+
 // Wrap a supernodal symbolic factor built elsewhere (e.g. the fields of a
 // cholmod_factor after cholmod_l_analyze: L->Perm, L->super, L->pi, L->px, L->s).
 int mxc_symbolic_import(int64_t n, int64_t nsuper, const int64_t *Perm, const int64_t *Super,

@@ -28,6 +28,10 @@ struct mxc_symbolic {
     int64_t *Ls;        // ssize
 };
 
+// Columns of the supernodal L reached from C[0..blen), in lsolve_pattern order.
+int mxc_reach(int64_t n, int64_t nsuper, const int64_t *Super, const int64_t *Lpi, const int64_t *Ls,
+              const int64_t *C, int64_t blen, int64_t *Yset, int64_t *ylen);
+
 #ifdef __cplusplus
 }
 #endif
