@@ -37,6 +37,14 @@ __global__ void k_perm_scatter(const T *__restrict__ Y, const long long *__restr
     for (int r = 0; r < nrhs; r++) X[pk + r * n] = (double)Y[k + r * n];
 }
 
+// Y[i] = (To) X[i] (grid-stride)
+template <typename Ti, typename To>
+__global__ void k_cast(const Ti *__restrict__ X, long long len, To *__restrict__ Y) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+         i += (long long)gridDim.x * blockDim.x)
+        Y[i] = (To)X[i];
+}
+
 // Cross-lane moves for the wave butterfly, all on the VALU (no LDS traffic):
 // v_permlane32_swap / v_permlane16_swap exchange the upper half of one operand
 // with the lower half of the other across 32- / 16-lane halves, and DPP row
@@ -748,23 +756,51 @@ __global__ __launch_bounds__(256) void k_chain_bwd(const SweepTask *__restrict__
                                                    int *__restrict__ cnt, int *__restrict__ abort) {
     __shared__ T red[4][ZC][NB];
     __shared__ T vs[ZC][NB];
+    __shared__ T tile[4][16][NB + 1];              // per wave: 16 rows x 64 columns of L
     const SweepTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const bool below = t.pad[0] != 0;
     const int nblk = t.np, tb = t.c0 / NB;
     const int col = t.c0 + lane;                   // the lane's column of the supernode
     const bool cok = lane < t.nb;
-    const T *Lc = W + t.lx + (long long)col * t.nsrow + 16 * w;
     // the inverse (diag): column = lane, rows 16w + j (inv[k][c] at winv + k + c nb)
     T iv[16];
 #pragma unroll
     for (int j = 0; j < 16; j++)
         iv[j] = (!below && cok && 16 * w + j < t.nb) ? W[t.winv + 16 * w + j + (long long)lane * t.nb]
                                                      : (T)0;
-    // rows r + 16w .. r + 16w + 15 of the lane's column (zero at or past rend)
-    auto load_rows = [&](T (&l)[16], int r, int rend) {
+    // L rows r + 16w .. r + 16w + 15 of the block's 64 columns (zero at or past
+    // rend), read in column segments: lane = (rows 4fq .. 4fq + 3, column
+    // fc + 16k), so one load covers 16 columns, 32 bytes each.  put() stages
+    // them in the wave's tile and get() reads them back with lane = column:
+    // l[j] = L[r + 16w + j, c0 + lane].
+    const int fq = lane & 3, fc = lane >> 2;
+    const T *Lw = W + t.lx + (long long)(t.c0 + fc) * t.nsrow + 16 * w + 4 * fq;
+    const long long cs = 16LL * t.nsrow;           // 16 columns
+    auto fetch = [&](T (&g)[16], int r, int rend) {
+        const int r4 = r + 16 * w + 4 * fq;
 #pragma unroll
-        for (int j = 0; j < 16; j++) l[j] = (cok && r + 16 * w + j < rend) ? Lc[r + j] : (T)0;
+        for (int k = 0; k < 4; k++) {
+            const bool ok = fc + 16 * k < t.nb;
+#pragma unroll
+            for (int e = 0; e < 4; e++) g[4 * k + e] = (ok && r4 + e < rend) ? Lw[k * cs + r + e] : (T)0;
+        }
+    };
+    auto put = [&](const T (&g)[16]) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) tile[w][4 * fq + e][fc + 16 * k] = g[4 * k + e];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    };
+    auto get = [&](T (&l)[16]) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) l[j] = tile[w][j][lane];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     };
     const int qz = lane >> 4, qj = lane & 15;      // the value lane q loads: row qj, rhs qz
     int seen = 0;
@@ -774,17 +810,18 @@ __global__ __launch_bounds__(256) void k_chain_bwd(const SweepTask *__restrict__
         T acc[ZC];
 #pragma unroll
         for (int z = 0; z < ZC; z++) acc[z] = 0;
-        T lc[16], ln[16];
+        T lc[16], gn[16];
         if (below) {
             // rows [r0, r1): ancestor rows, final before the launch
-            load_rows(lc, t.r0, t.r1);
+            fetch(gn, t.r0, t.r1);
+            put(gn);
             for (int r = t.r0; r < t.r1; r += NB) {
-                if (r + NB < t.r1) load_rows(ln, r + NB, t.r1);
+                get(lc);
+                if (r + NB < t.r1) fetch(gn, r + NB, t.r1);
                 const int rq = r + 16 * w + qj;
                 const T q = (qz < zc && rq < t.r1) ? X[Ls[t.psi + rq] + (long long)(z0 + qz) * n] : (T)0;
                 fma_bcast<T, ZC>(acc, lc, q);
-#pragma unroll
-                for (int j = 0; j < 16; j++) lc[j] = ln[j];
+                if (r + NB < t.r1) put(gn);
             }
 #pragma unroll
             for (int z = 0; z < ZC; z++) red[w][z][lane] = acc[z];
@@ -807,15 +844,18 @@ __global__ __launch_bounds__(256) void k_chain_bwd(const SweepTask *__restrict__
                     if (z < zc) acc[z] += ld_dev(part + ((long long)(t.q0 + k) * nrhs + z0 + z) * NB + lane);
         }
         // blocks nblk-1 .. tb+1 of the supernode's own rows, as they are finished
-        if (tb + 1 < nblk) load_rows(lc, (nblk - 1) * NB, t.nscol);
+        if (tb + 1 < nblk) {
+            fetch(gn, (nblk - 1) * NB, t.nscol);
+            put(gn);
+        }
         for (int b = nblk - 1; b > tb; b--) {
-            if (b - 1 > tb) load_rows(ln, (b - 1) * NB, t.nscol);
+            get(lc);
+            if (b - 1 > tb) fetch(gn, (b - 1) * NB, t.nscol);
             if (seen < nblk - b) seen = chain_wait(cnt + t.p0, nblk - b, abort);
             const int rq = b * NB + 16 * w + qj;
             const T q = (qz < zc && rq < t.nscol) ? ld_dev(X + t.k1 + rq + (long long)(z0 + qz) * n) : (T)0;
             fma_bcast<T, ZC>(acc, lc, q);
-#pragma unroll
-            for (int j = 0; j < 16; j++) lc[j] = ln[j];
+            if (b - 1 > tb) put(gn);
         }
 #pragma unroll
         for (int z = 0; z < ZC; z++) red[w][z][lane] = acc[z];

@@ -1665,19 +1665,14 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
     if (F->precision == MXC_FP64) {
         rc = run_sweep<double>(F, dX, nrhs, fwd, bwd);
     } else {
+        // through the FP32 work vector, converted on the device
         float *Xw = (float *)F->d_work;
-        std::vector<float> tmp;
-        std::vector<double> h(len);   // convert through the FP32 work vector
-        HIPTRY(hipMemcpyAsync(h.data(), dX, len * sizeof(double), hipMemcpyDeviceToHost, sm));
-        HIPTRY(hipStreamSynchronize(sm));
-        tmp.assign(h.begin(), h.end());
-        HIPTRY(hipMemcpyAsync(Xw, tmp.data(), len * sizeof(float), hipMemcpyHostToDevice, sm));
+        const dim3 g(grid_for((int64_t)len, 256, 1LL << 16));
+        hipLaunchKernelGGL((k_cast<double, float>), g, dim3(256), 0, sm, (const double *)dX, (long long)len, Xw);
         rc = run_sweep<float>(F, Xw, nrhs, fwd, bwd);
         if (rc) return rc;
-        HIPTRY(hipMemcpyAsync(tmp.data(), Xw, len * sizeof(float), hipMemcpyDeviceToHost, sm));
-        HIPTRY(hipStreamSynchronize(sm));
-        for (size_t i = 0; i < len; i++) h[i] = tmp[i];
-        HIPTRY(hipMemcpyAsync(dX, h.data(), len * sizeof(double), hipMemcpyHostToDevice, sm));
+        hipLaunchKernelGGL((k_cast<float, double>), g, dim3(256), 0, sm, (const float *)Xw, (long long)len, dX);
+        HIPTRY(hipGetLastError());
     }
     if (rc) return rc;
     if (location != MXC_DEVICE)
