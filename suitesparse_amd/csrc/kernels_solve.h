@@ -495,6 +495,88 @@ __global__ __launch_bounds__(256) void k_sweep_small_fwd(const SweepTask *__rest
     }
 }
 
+// Tiny supernodes (at most TINY_COLS columns and TINY_ROWS rows): one lane
+// solves a whole supernode, so the wide bottom levels of a forest (mostly
+// one-column supernodes with a few rows below) take one lane each instead of
+// a wave each.  A lane walks its rows one by one, so the row limit is low: a
+// supernode with more rows keeps a wave, which takes 64 rows at once.  Same
+// data as the small kernels: the inverse of the diagonal block at winv
+// (element (i, c) at i + c nb), L[r, c] at lx + r + c nsrow.
+constexpr int TINY_COLS = 8;
+constexpr int TINY_ROWS = 8;
+
+// forward: x_b = inv(L_bb) x_b, then x[g(r)] -= L[r, :] x_b for the rows below
+// (ancestor rows, shared with other supernodes: atomics)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_tiny_fwd(const SweepTask *__restrict__ tasks, int ntask,
+                                                        const T *__restrict__ W,
+                                                        const int *__restrict__ Ls, T *__restrict__ X,
+                                                        long long n, int nrhs) {
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti >= ntask) return;
+    const SweepTask t = tasks[ti];
+    const int nb = t.nb;
+    const T *inv = W + t.winv;
+    const T *Lb = W + t.lx;
+    for (int z = 0; z < nrhs; z++) {
+        T *xz = X + (long long)z * n;
+        T xb[TINY_COLS], y[TINY_COLS];
+#pragma unroll
+        for (int c = 0; c < TINY_COLS; c++) xb[c] = c < nb ? xz[t.k1 + c] : (T)0;
+#pragma unroll
+        for (int i = 0; i < TINY_COLS; i++) {
+            T a = 0;
+#pragma unroll
+            for (int c = 0; c <= i; c++)
+                if (i < nb) a += inv[i + c * nb] * xb[c];
+            y[i] = a;
+            if (i < nb) xz[t.k1 + i] = a;
+        }
+        for (int r = nb; r < t.nsrow; r++) {
+            T a = 0;
+#pragma unroll
+            for (int c = 0; c < TINY_COLS; c++)
+                if (c < nb) a += Lb[r + (long long)c * t.nsrow] * y[c];
+            atomic_add(xz + Ls[t.psi + r], -a);
+        }
+    }
+}
+
+// backward: x_b = inv(L_bb)' (x_b - L[below, :]' x[g(below)]); the rows below
+// are ancestor rows, final before the launch
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_tiny_bwd(const SweepTask *__restrict__ tasks, int ntask,
+                                                        const T *__restrict__ W,
+                                                        const int *__restrict__ Ls, T *__restrict__ X,
+                                                        long long n, int nrhs) {
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti >= ntask) return;
+    const SweepTask t = tasks[ti];
+    const int nb = t.nb;
+    const T *inv = W + t.winv;
+    const T *Lb = W + t.lx;
+    for (int z = 0; z < nrhs; z++) {
+        T *xz = X + (long long)z * n;
+        T v[TINY_COLS];
+#pragma unroll
+        for (int c = 0; c < TINY_COLS; c++) v[c] = c < nb ? xz[t.k1 + c] : (T)0;
+        for (int r = nb; r < t.nsrow; r++) {
+            const T xr = xz[Ls[t.psi + r]];
+#pragma unroll
+            for (int c = 0; c < TINY_COLS; c++)
+                if (c < nb) v[c] -= Lb[r + (long long)c * t.nsrow] * xr;
+        }
+#pragma unroll
+        for (int c = 0; c < TINY_COLS; c++) {
+            T a = 0;
+#pragma unroll
+            for (int i = c; i < TINY_COLS; i++)
+                if (i < nb) a += inv[i + c * nb] * v[i];
+            if (c < nb) xz[t.k1 + c] = a;
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_sweep_small_bwd(const SweepTask *__restrict__ tasks, int ntask,
                                                          const T *__restrict__ W,

@@ -121,10 +121,11 @@ static inline int kind_of(int launch_kind) {
 // L_TRAIL2: the part of a panel's trailing update right of the next panel, on
 // the second stream; L_FORK / L_JOIN: the event edges between the two streams;
 // L_FCHAIN / L_BCHAIN: a level's chained sweep launch; L_TRAILD: a trailing
-// launch whose (0, 0) tiles also factor the next step's diagonal blocks.
+// launch whose (0, 0) tiles also factor the next step's diagonal blocks;
+// L_FTINY / L_BTINY: a level's tiny supernodes, one lane each.
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7,
        L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11, L_FCHAIN = 12, L_BCHAIN = 13,
-       L_TRAILD = 14 };
+       L_TRAILD = 14, L_FTINY = 15, L_BTINY = 16 };
 
 struct StepBlocks {
     std::vector<DiagTask> diags;
@@ -784,11 +785,19 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
         int64_t nsrow = S->Lpi[s + 1] - S->Lpi[s];
         return nscol <= NB && nsrow - nscol <= SMALL_ROWS;
     };
+    // tiny supernodes (one lane each) first, then the other small ones (one wave each)
+    auto is_tiny = [&](int64_t s) {
+        return S->Super[s + 1] - S->Super[s] <= TINY_COLS && S->Lpi[s + 1] - S->Lpi[s] <= TINY_ROWS;
+    };
     auto push_small = [&](int l, std::vector<Launch> &out, int kind) {
-        long long b = (long long)st.size();
-        for (int64_t s : by_level[l])
-            if (is_small(s)) st.push_back(make_task(s, 0));
-        if ((long long)st.size() > b) out.push_back(Launch{kind, b, (long long)st.size() - b, 0});
+        const int tkind = kind == L_FSMALL ? L_FTINY : L_BTINY;
+        for (int tiny = 1; tiny >= 0; tiny--) {
+            long long b = (long long)st.size();
+            for (int64_t s : by_level[l])
+                if (is_small(s) && is_tiny(s) == (tiny == 1)) st.push_back(make_task(s, 0));
+            if ((long long)st.size() > b)
+                out.push_back(Launch{tiny ? tkind : kind, b, (long long)st.size() - b, 0});
+        }
     };
     for (int l = 0; l < nlev; l++) {
         push_small(l, F->fwd_launches, L_FSMALL);
@@ -1483,6 +1492,19 @@ static void launch_chain(mxc_factor *F, const Launch &L, T *Xw, int nr) {
     }
 }
 
+// A level's tiny supernodes, one lane each.
+template <typename T>
+static void launch_tiny(mxc_factor *F, const Launch &L, T *Xw, int nr) {
+    const dim3 g((unsigned)((L.count + 255) / 256)), b(256);
+    const T *W = (const T *)F->W;
+    if (L.kind == L_FTINY)
+        hipLaunchKernelGGL(k_sweep_tiny_fwd<T>, g, b, 0, F->stream, F->d_stask + L.begin, (int)L.count, W, F->d_Ls,
+                           Xw, (long long)F->n, nr);
+    else
+        hipLaunchKernelGGL(k_sweep_tiny_bwd<T>, g, b, 0, F->stream, F->d_stask + L.begin, (int)L.count, W, F->d_Ls,
+                           Xw, (long long)F->n, nr);
+}
+
 // After the caller's synchronization of the factor's stream: a chained sweep
 // whose wait gave up is a device error (the flag is cleared for the next call).
 static int chain_status(mxc_factor *F) {
@@ -1519,7 +1541,9 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
                     hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(ncnt, 256, 1LL << 30)), dim3(256),
                                        0, sm, F->d_cnt, ncnt);
                 for (const Launch &L : dir == 0 ? F->fwd_chain : F->bwd_chain) {
-                    if (L.kind == L_FSMALL)
+                    if (L.kind == L_FTINY || L.kind == L_BTINY)
+                        launch_tiny<T>(F, L, Xz, zn);
+                    else if (L.kind == L_FSMALL)
                         hipLaunchKernelGGL(k_sweep_small_fwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256),
                                            0, sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xz,
                                            (long long)n, zn);
@@ -1542,7 +1566,9 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     if (fwd) {
         tick(F, 5, &tt);
         for (const Launch &L : F->fwd_launches) {
-            if (L.kind == L_FSMALL)
+            if (L.kind == L_FTINY)
+                launch_tiny<T>(F, L, Xw, nr);
+            else if (L.kind == L_FSMALL)
                 hipLaunchKernelGGL(k_sweep_small_fwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,
                                    sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xw, (long long)n, nr);
             else
@@ -1555,7 +1581,9 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
     if (bwd) {
         tick(F, 6, &tt);
         for (const Launch &L : F->bwd_launches) {
-            if (L.kind == L_BSMALL)
+            if (L.kind == L_BTINY)
+                launch_tiny<T>(F, L, Xw, nr);
+            else if (L.kind == L_BSMALL)
                 hipLaunchKernelGGL(k_sweep_small_bwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,
                                    sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xw, (long long)n, nr);
             else
