@@ -592,31 +592,34 @@ __global__ __launch_bounds__(256) void k_sweep_small_bwd(const SweepTask *__rest
     T *xb = X + t.k1;
     for (int z0 = 0; z0 < nrhs; z0 += ZS) {
         const int zc = min(ZS, nrhs - z0);
-        // p_k = sum over the rows below of L[r, k] x[g(r)], 16 columns at a time
-        for (int kc = 0; kc < nb; kc += 16) {
-            T v[ZS][16] = {};
-            for (int r0 = nb; r0 < t.nsrow; r0 += 64) {
-                const int r = r0 + lane;
-                const bool active = r < t.nsrow;
-                const T *Lrow = W + t.lx + r;
-                T xr[ZS];
-                const long long g = active ? (long long)Ls[t.psi + r] : 0;
+        // p_k = sum over the rows below of L[r, k] x[g(r)], 16 columns and two
+        // right-hand sides at a time (fewer live accumulators: higher occupancy;
+        // the second pair re-reads the same L rows from cache)
+        for (int kc = 0; kc < nb; kc += 16)
+            for (int zp = 0; zp < zc; zp += 2) {
+                T v[2][16] = {};
+                for (int r0 = nb; r0 < t.nsrow; r0 += 64) {
+                    const int r = r0 + lane;
+                    const bool active = r < t.nsrow;
+                    const T *Lrow = W + t.lx + r;
+                    T xr[2];
+                    const long long g = active ? (long long)Ls[t.psi + r] : 0;
 #pragma unroll
-                for (int z = 0; z < ZS; z++)
-                    xr[z] = (active && z < zc) ? X[g + (long long)(z0 + z) * n] : (T)0;
+                    for (int z = 0; z < 2; z++)
+                        xr[z] = (active && zp + z < zc) ? X[g + (long long)(z0 + zp + z) * n] : (T)0;
 #pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    const T l = (active && kc + j < nb) ? Lrow[(long long)(kc + j) * t.nsrow] : (T)0;
+                    for (int j = 0; j < 16; j++) {
+                        const T l = (active && kc + j < nb) ? Lrow[(long long)(kc + j) * t.nsrow] : (T)0;
 #pragma unroll
-                    for (int z = 0; z < ZS; z++) v[z][j] += l * xr[z];
+                        for (int z = 0; z < 2; z++) v[z][j] += l * xr[z];
+                    }
+                }
+#pragma unroll
+                for (int z = 0; z < 2; z++) {
+                    const T s = wave_sum16(v[z], lane);
+                    if ((lane & 3) == 0) ps[w][zp + z][(kc + qcol) & 63] = s;
                 }
             }
-#pragma unroll
-            for (int z = 0; z < ZS; z++) {
-                const T s = wave_sum16(v[z], lane);
-                if ((lane & 3) == 0) ps[w][z][(kc + qcol) & 63] = s;
-            }
-        }
         wave_sync_lds();
         // v = x_b - p, then x_b = inv(L_bb)' v: lane k forms entry k
 #pragma unroll
