@@ -988,16 +988,17 @@ __global__ void k_gather_vals(const double *__restrict__ Ax, const long long *__
     for (; p < nnz; p += stride) vals[p] = Ax[src[p]];
 }
 
+template <int LPR>
 __global__ __launch_bounds__(256) void k_residual(const long long *__restrict__ rowptr,
                                                   const int *__restrict__ col,
                                                   const double *__restrict__ vals,
                                                   const double *__restrict__ B,
                                                   const double *__restrict__ X,
                                                   double *__restrict__ R, long long n, int nrhs) {
-    // eight lanes per row: the row's values and columns are read coalesced,
-    // four right-hand sides per pass over the row
-    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
-    const int sub = threadIdx.x & 7;
+    // LPR lanes per row (4, 8 or 16, from the average row length): the row's
+    // values and columns are read coalesced, four right-hand sides per pass
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+    const int sub = threadIdx.x & (LPR - 1);
     const bool ok = i < n;
     long long p0 = 0, p1 = 0;
     if (ok) {
@@ -1007,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_residual(const long long *__restrict__ 
     for (int z0 = 0; z0 < nrhs; z0 += 4) {
         const int zc = min(4, nrhs - z0);
         double s[4] = {0, 0, 0, 0};
-        for (long long p = p0 + sub; p < p1; p += 8) {
+        for (long long p = p0 + sub; p < p1; p += LPR) {
             const double v = vals[p];
             const double *x = X + col[p] + (long long)z0 * n;
 #pragma unroll
@@ -1017,9 +1018,8 @@ __global__ __launch_bounds__(256) void k_residual(const long long *__restrict__ 
 #pragma unroll
         for (int z = 0; z < 4; z++) {
             double u = s[z];
-            u += __shfl_xor(u, 4);
-            u += __shfl_xor(u, 2);
-            u += __shfl_xor(u, 1);
+#pragma unroll
+            for (int m = LPR / 2; m >= 1; m >>= 1) u += __shfl_xor(u, m);
             if (ok && sub == 0 && z < zc) {
                 const long long q = i + (long long)(z0 + z) * n;
                 R[q] = B[q] - u;

@@ -1768,9 +1768,20 @@ extern "C" int mxc_solve_bset(mxc_factor *F, int sys, const double *B, const int
 // ---------------------------------------------------------------------------
 // residual and iterative refinement
 // ---------------------------------------------------------------------------
+// lanes per row from the average row length: 4 (2D 5-point), 8 (3D 7-point), 16 (27-point)
 static void launch_residual(mxc_factor *F, const double *dB, const double *dX, double *dR, int64_t nrhs) {
-    hipLaunchKernelGGL(k_residual, dim3(grid_for(F->n * 8, 256, 1LL << 30)), dim3(256), 0, F->stream,
-                       F->d_rowptr, F->d_col, F->d_vals, dB, dX, dR, (long long)F->n, (int)nrhs);
+    const double avg = F->n > 0 ? (double)F->nnz_full / (double)F->n : 0.0;
+    const int lpr = avg <= 6.0 ? 4 : avg <= 20.0 ? 8 : 16;
+    const dim3 g(grid_for(F->n * lpr, 256, 1LL << 30)), b(256);
+    if (lpr == 4)
+        hipLaunchKernelGGL(k_residual<4>, g, b, 0, F->stream, F->d_rowptr, F->d_col, F->d_vals, dB, dX, dR,
+                           (long long)F->n, (int)nrhs);
+    else if (lpr == 8)
+        hipLaunchKernelGGL(k_residual<8>, g, b, 0, F->stream, F->d_rowptr, F->d_col, F->d_vals, dB, dX, dR,
+                           (long long)F->n, (int)nrhs);
+    else
+        hipLaunchKernelGGL(k_residual<16>, g, b, 0, F->stream, F->d_rowptr, F->d_col, F->d_vals, dB, dX, dR,
+                           (long long)F->n, (int)nrhs);
 }
 
 extern "C" int mxc_residual(mxc_factor *F, const double *B, const double *X, int64_t nrhs, double *R,
