@@ -840,4 +840,72 @@ __global__ __launch_bounds__(256) void k_potrf_small(const DiagTask *__restrict_
     }
 }
 
+// Diagonal blocks of at most TINY_DIAG columns (the bottom levels of a forest:
+// mostly one to five columns): one lane factors and inverts a whole block, with
+// potrf16_wave's arithmetic: pivot d_jj (raised to rho a_jj when SAFE and
+// below it), a non-positive pivot reported and replaced by 1, r_j =
+// 1/sqrt(d_jj), L_jj = d_jj r_j, L_ij = a_ij r_j, the inverse by forward
+// substitution with r_j.
+constexpr int TINY_DIAG = 6;
+
+template <typename T, bool SAFE>
+__global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__ tasks, int ntask,
+                                                    T *__restrict__ W, int *__restrict__ fail,
+                                                    const double *__restrict__ adiag, double rho,
+                                                    int *__restrict__ npert) {
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti >= ntask) return;
+    const DiagTask t = tasks[ti];
+    const int nb = t.nb;
+    T a[TINY_DIAG][TINY_DIAG], x[TINY_DIAG][TINY_DIAG], rinv[TINY_DIAG];
+#pragma unroll
+    for (int i = 0; i < TINY_DIAG; i++)
+#pragma unroll
+        for (int c = 0; c <= i; c++)
+            a[i][c] = i < nb ? W[t.off + i + (long long)c * t.ld] : (i == c ? (T)1 : (T)0);
+    int npj = 0, badj = TINY_DIAG;
+#pragma unroll
+    for (int j = 0; j < TINY_DIAG; j++) {
+        const bool real = j < nb;
+        T djj = a[j][j];
+        if (SAFE) {
+            const T floor_j = real ? (T)(rho * adiag[t.col0 + j]) : (T)0;
+            const bool low = real && djj <= floor_j && floor_j > (T)0;
+            djj = low ? floor_j : djj;
+            npj += low ? 1 : 0;
+        }
+        const bool bad = !(djj > (T)0);
+        badj = (bad && real && badj == TINY_DIAG) ? j : badj;
+        djj = bad ? (T)1 : djj;
+        const T rj = rsqrt_nr(djj);
+        rinv[j] = rj;
+        a[j][j] = djj * rj;
+#pragma unroll
+        for (int i = j + 1; i < TINY_DIAG; i++) a[i][j] *= rj;
+#pragma unroll
+        for (int c = j + 1; c < TINY_DIAG; c++)
+#pragma unroll
+            for (int i = c; i < TINY_DIAG; i++) a[i][c] -= a[i][j] * a[c][j];
+    }
+    if (SAFE && npj > 0) atomicAdd(npert, npj);
+    if (badj < nb) atomicMin(fail, t.col0 + badj);
+#pragma unroll
+    for (int c = 0; c < TINY_DIAG; c++)
+#pragma unroll
+        for (int i = c; i < TINY_DIAG; i++) {
+            T sacc = (i == c) ? (T)1 : (T)0;
+#pragma unroll
+            for (int k = c; k < i; k++) sacc -= a[i][k] * x[k][c];
+            x[i][c] = sacc * rinv[i];
+        }
+#pragma unroll
+    for (int c = 0; c < TINY_DIAG; c++)
+#pragma unroll
+        for (int i = 0; i < TINY_DIAG; i++)
+            if (i < nb && c < nb) {
+                if (c <= i) W[t.off + i + (long long)c * t.ld] = a[i][c];
+                W[t.winv + i + (long long)c * nb] = (c <= i) ? x[i][c] : (T)0;
+            }
+}
+
 }  // namespace mxc

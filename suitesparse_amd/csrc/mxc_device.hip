@@ -122,10 +122,11 @@ static inline int kind_of(int launch_kind) {
 // the second stream; L_FORK / L_JOIN: the event edges between the two streams;
 // L_FCHAIN / L_BCHAIN: a level's chained sweep launch; L_TRAILD: a trailing
 // launch whose (0, 0) tiles also factor the next step's diagonal blocks;
-// L_FTINY / L_BTINY: a level's tiny supernodes, one lane each.
+// L_FTINY / L_BTINY: a level's tiny supernodes, one lane each; L_DIAG_TINY:
+// diagonal blocks of at most TINY_DIAG columns, one lane each.
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7,
        L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11, L_FCHAIN = 12, L_BCHAIN = 13,
-       L_TRAILD = 14, L_FTINY = 15, L_BTINY = 16 };
+       L_TRAILD = 14, L_FTINY = 15, L_BTINY = 16, L_DIAG_TINY = 17 };
 
 struct StepBlocks {
     std::vector<DiagTask> diags;
@@ -365,16 +366,20 @@ static int flatten_upload(mxc_factor *F) {
         auto edge_of = [&](const UpdDesc &u) { return F->tune[kind_of(L_TRAIL)][tile_bucket(u.M, u.N, u.K)]; };
         for (size_t t = 0; t < nst; t++) {
             const StepBlocks &sb = F->blk_steps[l][t];
-            // blocks of at most 16 columns: one wave each; wider blocks: a workgroup each
-            for (int small = 1; small >= 0; small--) {
+            // blocks of at most TINY_DIAG columns: one lane each; at most 16: one
+            // wave each; wider blocks: a workgroup each
+            for (int cls = 2; cls >= 0; cls--) {
                 long long db = (long long)diags.size();
                 for (size_t q = 0; q < sb.diags.size(); q++) {
                     const DiagTask &d = sb.diags[q];
                     if (std::binary_search(fused_s.begin(), fused_s.end(), sb.diag_s[q])) continue;
-                    if ((d.nb <= 16) == (small == 1)) diags.push_back(d);
+                    const int c = d.nb <= TINY_DIAG ? 2 : d.nb <= 16 ? 1 : 0;
+                    if (c == cls) diags.push_back(d);
                 }
                 long long dc = (long long)diags.size() - db;
-                if (dc > 0) F->factor_launches.push_back(Launch{small ? L_DIAG_SMALL : L_DIAG, db, dc, 0});
+                if (dc > 0)
+                    F->factor_launches.push_back(
+                        Launch{cls == 2 ? L_DIAG_TINY : cls == 1 ? L_DIAG_SMALL : L_DIAG, db, dc, 0});
             }
             emit(L_TRSM, sb.trsm, false);
             // At the end of panel P: wait for the second-stream update of panel
@@ -1236,7 +1241,8 @@ static int enqueue_factor(mxc_factor *F) {
             }
             continue;
         }
-        const int cls = L.kind == L_UPD ? 1 : (L.kind == L_DIAG || L.kind == L_DIAG_SMALL) ? 2
+        const int cls = L.kind == L_UPD ? 1
+                      : (L.kind == L_DIAG || L.kind == L_DIAG_SMALL || L.kind == L_DIAG_TINY) ? 2
                       : L.kind == L_TRSM ? 3 : 4;
         tick(F, cls, &tt);
         switch (L.kind) {
@@ -1265,6 +1271,19 @@ static int enqueue_factor(mxc_factor *F) {
                     hipLaunchKernelGGL((k_potrf_small<T, false>), dim3(g), dim3(256), 0, sm,
                                        F->d_diag + L.begin, (int)L.count, W, F->d_fail,
                                        (const double *)F->d_adiag, F->rho, F->d_fail + 1);
+                F->counts[1]++;
+                break;
+            }
+            case L_DIAG_TINY: {
+                const unsigned g = (unsigned)((L.count + 255) / 256);
+                if (F->rho > 0)
+                    hipLaunchKernelGGL((k_potrf_tiny<T, true>), dim3(g), dim3(256), 0, sm, F->d_diag + L.begin,
+                                       (int)L.count, W, F->d_fail, (const double *)F->d_adiag, F->rho,
+                                       F->d_fail + 1);
+                else
+                    hipLaunchKernelGGL((k_potrf_tiny<T, false>), dim3(g), dim3(256), 0, sm, F->d_diag + L.begin,
+                                       (int)L.count, W, F->d_fail, (const double *)F->d_adiag, F->rho,
+                                       F->d_fail + 1);
                 F->counts[1]++;
                 break;
             }
