@@ -908,4 +908,43 @@ __global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__
             }
 }
 
+// Panel TRSM of a tiny supernode (K <= TINY_DIAG columns, at most
+// TINY_TRSM_ROWS rows below): one lane forms L2 = S2 inv(L11)' row by row, in
+// place, as MODE_STORE does for a tile (C[i, j] = sum_k Y[i, k] X[j, k] with X
+// the inverse at x_off + j + k ldx), and for a split-FP16 factor also writes
+// the value's hi/lo pair scaled by its row's exponent.
+constexpr int TINY_TRSM_ROWS = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_tiny(const UpdDesc *__restrict__ descs,
+                                                   const Tile *__restrict__ tiles, int ntile,
+                                                   T *__restrict__ W, const int *__restrict__ Ls,
+                                                   unsigned *__restrict__ Lh, const double *__restrict__ adiag) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ntile) return;
+    const UpdDesc d = descs[tiles[e].desc];
+    T inv[TINY_DIAG][TINY_DIAG];
+#pragma unroll
+    for (int j = 0; j < TINY_DIAG; j++)
+#pragma unroll
+        for (int k = 0; k < TINY_DIAG; k++)
+            inv[j][k] = (j < d.N && k < d.K) ? W[d.x_off + j + (long long)k * d.ldx] : (T)0;
+    for (int i = 0; i < d.M; i++) {
+        T y[TINY_DIAG];
+#pragma unroll
+        for (int k = 0; k < TINY_DIAG; k++) y[k] = k < d.K ? W[d.y_off + i + (long long)k * d.ldy] : (T)0;
+        const int ex = Lh ? row_scale_exp(adiag, Ls[d.ls_off + i]) : 0;
+#pragma unroll
+        for (int j = 0; j < TINY_DIAG; j++) {
+            if (j >= d.N) break;
+            T c = 0;
+#pragma unroll
+            for (int k = 0; k < TINY_DIAG; k++) c += y[k] * inv[j][k];
+            const long long q = d.c_off + (long long)j * d.ldc + i;
+            W[q] = c;
+            if (std::is_same<T, float>::value && Lh) Lh[q] = split_pack((float)c, ex);
+        }
+    }
+}
+
 }  // namespace mxc

@@ -123,10 +123,11 @@ static inline int kind_of(int launch_kind) {
 // L_FCHAIN / L_BCHAIN: a level's chained sweep launch; L_TRAILD: a trailing
 // launch whose (0, 0) tiles also factor the next step's diagonal blocks;
 // L_FTINY / L_BTINY: a level's tiny supernodes, one lane each; L_DIAG_TINY:
-// diagonal blocks of at most TINY_DIAG columns, one lane each.
+// diagonal blocks of at most TINY_DIAG columns, one lane each; L_TRSM_TINY:
+// the panel TRSM of tiny supernodes, one lane each.
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7,
        L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11, L_FCHAIN = 12, L_BCHAIN = 13,
-       L_TRAILD = 14, L_FTINY = 15, L_BTINY = 16, L_DIAG_TINY = 17 };
+       L_TRAILD = 14, L_FTINY = 15, L_BTINY = 16, L_DIAG_TINY = 17, L_TRSM_TINY = 18 };
 
 struct StepBlocks {
     std::vector<DiagTask> diags;
@@ -381,7 +382,23 @@ static int flatten_upload(mxc_factor *F) {
                     F->factor_launches.push_back(
                         Launch{cls == 2 ? L_DIAG_TINY : cls == 1 ? L_DIAG_SMALL : L_DIAG, db, dc, 0});
             }
-            emit(L_TRSM, sb.trsm, false);
+            {
+                // TRSM of supernodes with at most TINY_DIAG columns and TINY_TRSM_ROWS rows below: one lane each
+                std::vector<UpdDesc> big;
+                long long b = (long long)tiles.size();
+                for (const UpdDesc &u : sb.trsm) {
+                    if (u.K <= TINY_DIAG && u.M <= TINY_TRSM_ROWS) {
+                        descs.push_back(u);
+                        descs.back().pd = -1;
+                        tiles.push_back(Tile{(int)descs.size() - 1, 0, 0});
+                    } else {
+                        big.push_back(u);
+                    }
+                }
+                const long long c = (long long)tiles.size() - b;
+                if (c > 0) F->factor_launches.push_back(Launch{L_TRSM_TINY, b, c, 0});
+                emit(L_TRSM, big, false);
+            }
             // At the end of panel P: wait for the second-stream update of panel
             // P-1 (it also wrote the columns of panel P+1), then start panel P's
             // own second-stream update (columns right of panel P+1; it needs only
@@ -1243,7 +1260,7 @@ static int enqueue_factor(mxc_factor *F) {
         }
         const int cls = L.kind == L_UPD ? 1
                       : (L.kind == L_DIAG || L.kind == L_DIAG_SMALL || L.kind == L_DIAG_TINY) ? 2
-                      : L.kind == L_TRSM ? 3 : 4;
+                      : (L.kind == L_TRSM || L.kind == L_TRSM_TINY) ? 3 : 4;
         tick(F, cls, &tt);
         switch (L.kind) {
             case L_UPD:
@@ -1285,6 +1302,14 @@ static int enqueue_factor(mxc_factor *F) {
                                        (int)L.count, W, F->d_fail, (const double *)F->d_adiag, F->rho,
                                        F->d_fail + 1);
                 F->counts[1]++;
+                break;
+            }
+            case L_TRSM_TINY: {
+                const bool h16 = std::is_same<T, float>::value && F->split16 && F->d_Lh;
+                hipLaunchKernelGGL(k_trsm_tiny<T>, dim3((unsigned)((L.count + 255) / 256)), dim3(256), 0, sm,
+                                   F->d_desc, F->d_tiles + L.begin, (int)L.count, W, F->d_Ls,
+                                   h16 ? F->d_Lh : nullptr, (const double *)F->d_adiag);
+                F->counts[2]++;
                 break;
             }
             case L_TRSM:
