@@ -181,7 +181,7 @@ __device__ __forceinline__ void pd_tail(const PdArgs &pda, int pd, T *__restrict
 }
 
 template <typename T, int MODE, int TS, bool PD = false>
-__global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile(const UpdDesc *__restrict__ descs,
+__global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile(const UpdDesc *__restrict__ descs,
                                                               const Tile *__restrict__ tiles,
                                                               T *__restrict__ W,
                                                               const int *__restrict__ Ls,
@@ -189,8 +189,8 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
                                                               const double *__restrict__ adiag,
                                                               PdArgs pda = PdArgs{}) {
     typedef typename MfmaOp<T>::acc_t acc_t;
-    constexpr int NT = TS == 32 ? 64 : 256;     // threads
-    constexpr int WB = TS == 32 ? 32 : TS / 2;  // edge of a wave's block of C
+    constexpr int NT = TS <= 32 ? 64 : 256;     // threads (tiles of 16 and 32: one wave)
+    constexpr int WB = TS <= 32 ? TS : TS / 2;  // edge of a wave's block of C
     constexpr int MB = WB / 16;                 // 16x16 MFMA blocks per wave edge
     constexpr int KG = NT / TS;                 // k rows staged per pass
     constexpr int PER = (BK * TS) / NT;         // elements of X (and of Y) each thread stages
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(TS == 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
     const UpdDesc d = descs[tl.desc];
     const int i0 = tl.ti * TS, j0 = tl.tj * TS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = TS == 32 ? 0 : (wave & 1), wc = TS == 32 ? 0 : (wave >> 1);
+    const int wr = TS <= 32 ? 0 : (wave & 1), wc = TS <= 32 ? 0 : (wave >> 1);
 
     if (MODE == MODE_SCATTER) {
         for (int e = tid; e < 2 * TS; e += NT) {

@@ -312,12 +312,24 @@ static int flatten_upload(mxc_factor *F) {
     auto emit = [&](int kind, const std::vector<UpdDesc> &blocks, bool lower_only,
                     const std::vector<const DiagTask *> *follow = nullptr, std::vector<char> *fused = nullptr,
                     const std::vector<int> *force = nullptr) {
+        // FP64 descendant updates of at most 16 rows and columns take 16-wide tiles when a
+        // level holds very many of them (the bottom levels of a forest of small systems:
+        // millions); a 3D grid's levels (at most ~30,000 at 64^3) keep 32-wide tiles, which
+        // measured faster there.  FP32 measured slower with them.
+        auto tiny16 = [&](const UpdDesc &u) {
+            return F->precision == MXC_FP64 && kind == L_UPD && u.M <= 16 && u.N <= 16 &&
+                   F->tune[kind_of(kind)][tile_bucket(u.M, u.N, u.K)] == 32;
+        };
+        long long n16 = 0;
+        for (const UpdDesc &u : blocks) n16 += tiny16(u) ? 1 : 0;
+        const bool use16 = n16 >= 200000;
         auto edge = [&](size_t bi) {
             const UpdDesc &u = blocks[bi];
-            return (force && (*force)[bi] > 0) ? (*force)[bi] : F->tune[kind_of(kind)][tile_bucket(u.M, u.N, u.K)];
+            if (force && (*force)[bi] > 0) return (*force)[bi];
+            return (use16 && tiny16(u)) ? 16 : F->tune[kind_of(kind)][tile_bucket(u.M, u.N, u.K)];
         };
         long long total = 0;
-        for (int ts : {32, 64, 128}) {
+        for (int ts : {16, 32, 64, 128}) {
             long long b = (long long)tiles.size();
             // tails only in launches that fit on the device at once (a tail's
             // kernel runs at lower occupancy)
@@ -1198,7 +1210,7 @@ static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = n
     if constexpr (std::is_same<T, float>::value && MODE != MODE_STORE) {
         // FP32 factor: split-FP16 products on the FP16 MFMA, operands from the
         // pairs the TRSM wrote (32-wide tiles are latency-bound: FP32 MFMA)
-        if (h16 && L.tile != 32) {
+        if (h16 && L.tile > 32) {
             if (L.tile == 64)
                 hipLaunchKernelGGL((k_tile_h<MODE, 64, PD>), g, dim3(256), 0, st, F->d_desc, F->d_tiles + L.begin,
                                    W, (const unsigned *)F->d_Lh, F->d_Ls, adiag, pda);
@@ -1210,6 +1222,11 @@ static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = n
     }
     unsigned *Lh = (MODE == MODE_STORE && h16) ? F->d_Lh : nullptr;
     if constexpr (!PD) {
+        if (L.tile == 16) {
+            hipLaunchKernelGGL((k_tile<T, MODE, 16>), g, dim3(64), 0, st, F->d_desc, F->d_tiles + L.begin, W,
+                               F->d_Ls, Lh, adiag, pda);
+            return;
+        }
         if (L.tile == 32) {
             hipLaunchKernelGGL((k_tile<T, MODE, 32>), g, dim3(64), 0, st, F->d_desc, F->d_tiles + L.begin, W,
                                F->d_Ls, Lh, adiag, pda);
