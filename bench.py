@@ -319,7 +319,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
-              "(the factorization would shard along the elimination tree, see DESIGN.md)",
+              "(the factorization would shard along the elimination tree: independent subtrees per GPU, "
+              "then one point-to-point exchange of the separator updates)",
               file=sys.stderr)
         sys.exit(2)
 
