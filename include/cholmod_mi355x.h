@@ -116,9 +116,10 @@ int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const int64_t *A
 
 void mxc_factor_free(mxc_factor *F);
 
-/* info[0..8] = n, nsuper, xsize, levels, update pairs, update tiles,
+/* info[0..9] = n, nsuper, xsize, levels, update pairs, update tiles,
  *              device bytes held, precision, pivots perturbed by the
- *              low-precision safeguard in the last factorization;
+ *              low-precision safeguard in the last factorization, 1 if the
+ *              last factorization escalated to FP64 (see mxc_factorize);
  * dinfo[0..5] = fl, lnz, and the algorithmic flops of one factorization by
  *               kernel class: descendant updates (dsyrk + dgemm counts), diagonal
  *               POTRF, panel TRSM, in-supernode trailing updates */
@@ -128,7 +129,11 @@ int mxc_factor_info(const mxc_factor *F, int64_t *info, double *dinfo);
  * create, unpermuted).  Ax is a host or device pointer (location).  On
  * return *minor = n on success, else the first failing column, with
  * MXC_NOT_POSDEF returned (L then holds the factor of the leading minor
- * columns and zeros after, as CHOLMOD leaves it). */
+ * columns and zeros after, as CHOLMOD leaves it).  A low-precision factor
+ * (MXC_FP32 with a pivot ratio rho > 0) that meets a pivot <= 0 escalates:
+ * the same values are factored in FP64, whose result decides the status and
+ * minor, and the FP64 factor serves every solve, L copy and rcond until the
+ * next mxc_factorize (mxc_factor_info info[9] = 1). */
 int mxc_factorize(mxc_factor *F, const double *Ax, int location, double beta, int64_t *minor);
 
 /* Copy the factor to host memory as double, in CHOLMOD's L->x layout. */

@@ -46,6 +46,8 @@ def lib():
         L.ref_get_symbolic.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         L.ref_factorize.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                     C.POINTER(C.c_int64)]
+        L.ref_factorize_p.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int64)]
         L.ref_get_Lx.argtypes = [C.c_void_p, _f64p]
         L.ref_solve.argtypes = [C.c_void_p, _f64p, C.c_int64, _f64p, C.POINTER(C.c_double)]
         L.ref_residual.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int64, _f64p]
@@ -122,6 +124,17 @@ class RefFactor:
         minor = C.c_int64(0)
         xs = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
         st = lib().ref_factorize(self._h, _ptr(xs), C.byref(t), C.byref(minor))
+        self.t_factor = t.value
+        self.minor = minor.value
+        self.status = st
+        return st
+
+    def factorize_p(self, beta: float, x=None):
+        """cholmod_l_factorize_p: factor A + beta*I (fset NULL)."""
+        t = C.c_double(0)
+        minor = C.c_int64(0)
+        xs = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        st = lib().ref_factorize_p(self._h, _ptr(xs), float(beta), C.byref(t), C.byref(minor))
         self.t_factor = t.value
         self.minor = minor.value
         self.status = st

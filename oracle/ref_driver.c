@@ -157,6 +157,29 @@ int ref_factorize(void *vh, const double *Ax_new, double *t_factor, int64_t *min
     return cm->status;
 }
 
+// cholmod_l_factorize_p (cholmod.h:2794) with beta: factors A + beta[0]*I
+// (fset = NULL, the symmetric case of CHOLMOD/Cholesky/cholmod_factorize.c:93-418).
+int ref_factorize_p(void *vh, const double *Ax_new, double beta, double *t_factor, int64_t *minor) {
+    ref_handle *h = (ref_handle *)vh;
+    cholmod_common *cm = &h->cm;
+    if (Ax_new) {
+        int64_t nnz = ((int64_t *)h->A->p)[h->A->ncol];
+        if (h->dtype == CHOLMOD_SINGLE) {
+            float *x = (float *)h->A->x;
+            for (int64_t k = 0; k < nnz; k++) x[k] = (float)Ax_new[k];
+        } else {
+            memcpy(h->A->x, Ax_new, sizeof(double) * nnz);
+        }
+    }
+    double b[2] = {beta, 0};
+    double t0 = now_s();
+    int ok = cholmod_l_factorize_p(h->A, b, NULL, 0, h->L, cm);
+    if (t_factor) *t_factor = now_s() - t0;
+    if (minor) *minor = (int64_t)h->L->minor;
+    if (!ok) return cm->status != 0 ? cm->status : -99;
+    return cm->status;
+}
+
 // Copy L->x (supernodal values) out as double.
 int ref_get_Lx(void *vh, double *Lx) {
     ref_handle *h = (ref_handle *)vh;

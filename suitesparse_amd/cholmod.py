@@ -89,6 +89,27 @@ def _is_device(a) -> bool:
     return hasattr(a, "is_cuda") and bool(a.is_cuda)
 
 
+def _device_ok(t, cm: Common) -> bool:
+    """A device tensor the C layer may read or write as doubles: float64 and on
+    the Common's device (the C layer takes a raw pointer and a length)."""
+    import torch
+    return t.dtype == torch.float64 and t.device.index in (None, cm.device) and t.device.type == "cuda"
+
+
+def _same_pattern(A, L) -> bool:
+    """A's pattern is the one L was analyzed with: the device plan places each
+    value by its position in that pattern (the reference instead scatters A
+    through its own pattern, so a different pattern must not be accepted)."""
+    Ap, Ai = L.A_pattern
+    p, i = A.p, A.i
+    if p is Ap and i is Ai:
+        return True
+    p = np.asarray(p)
+    i = np.asarray(i)
+    return (p.shape == Ap.shape and i.shape == Ai.shape and np.array_equal(p, Ap)
+            and np.array_equal(i, Ai))
+
+
 class Factor:
     """Symbolic + (device-resident) numeric supernodal factor."""
 
@@ -147,11 +168,11 @@ class Factor:
         return out
 
     def info(self) -> dict:
-        info = np.zeros(9, dtype=np.int64)
+        info = np.zeros(10, dtype=np.int64)
         dinfo = np.zeros(6)
         check("mxc_factor_info", lib().mxc_factor_info(self._num, _ptr(info), _ptr(dinfo)))
         keys = ("n", "nsuper", "xsize", "levels", "pairs", "update_tiles", "device_bytes", "precision",
-                "perturbed_pivots")
+                "perturbed_pivots", "escalated_fp64")
         return dict(zip(keys, (int(v) for v in info)))
 
     def set_timing(self, on, classes=None):
@@ -202,7 +223,7 @@ class Factor:
 
     def kernel_flops(self) -> dict:
         """Algorithmic flops of one factorization by kernel class."""
-        info = np.zeros(9, dtype=np.int64)
+        info = np.zeros(10, dtype=np.int64)
         dinfo = np.zeros(6)
         check("mxc_factor_info", lib().mxc_factor_info(self._num, _ptr(info), _ptr(dinfo)))
         return {"update": float(dinfo[2]), "potrf": float(dinfo[3]), "trsm": float(dinfo[4]),
@@ -286,18 +307,26 @@ def factorize_p(A, beta, L: Factor, cm: Common, values=None) -> bool:
     """Numeric factorization of A + beta*I with L's pattern.  ``values`` (a numpy
     array or CUDA tensor of tril(A)'s values) overrides A.x."""
     cm.status = CHOLMOD_OK
-    if L is None or A.n != L.n:
+    if L is None or A.n != L.n or not _same_pattern(A, L):
         cm.status = CHOLMOD_INVALID
         return False
+    nnz_a = int(L.A_pattern[0][-1]) if L.n > 0 else 0
+    x = A.x if values is None else values
+    loc = _lib.DEVICE if _is_device(x) else _lib.HOST
+    if loc == _lib.DEVICE:
+        if not (_device_ok(x, cm) and x.dim() == 1 and x.is_contiguous() and x.numel() == nnz_a):
+            cm.status = CHOLMOD_INVALID
+            return False
+    else:
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        if x.size != nnz_a:
+            cm.status = CHOLMOD_INVALID
+            return False
     if not cm.start():
         return False
     h = L._numeric(cm)
     if h is None:
         return False
-    x = A.x if values is None else values
-    loc = _lib.DEVICE if _is_device(x) else _lib.HOST
-    if loc == _lib.HOST:
-        x = np.ascontiguousarray(x, dtype=np.float64)
     b = float(beta[0] if np.ndim(beta) else beta)
     minor = C.c_int64(0)
     st = lib().mxc_factorize(h, _ptr(x), loc, b, C.byref(minor))
@@ -328,6 +357,9 @@ def solve(sys: int, L: Factor, B, cm: Common):
     dev = _is_device(B)
     if dev:
         import torch
+        if not _device_ok(B, cm) or B.dim() not in (1, 2):
+            cm.status = CHOLMOD_INVALID
+            return None
         Bc = B if B.dim() == 2 else B.reshape(-1, 1)
         Bc = Bc.t().contiguous().t() if not Bc.t().is_contiguous() else Bc   # column-major
         X = torch.empty_like(Bc.t().contiguous()).t()
@@ -419,6 +451,9 @@ def solve2(sys: int, L: Factor, B, cm: Common, Bset=None, X=None):
     if sys in (CHOLMOD_P, CHOLMOD_Pt):
         Bset = None
     dev = _is_device(B)
+    if dev and (not _device_ok(B, cm) or B.dim() not in (1, 2)):
+        cm.status = CHOLMOD_INVALID
+        return None, None
     if Bset is None:
         if dev:
             Bc = B if B.dim() == 2 else B.reshape(-1, 1)
@@ -480,15 +515,25 @@ def solve_refine(L: Factor, B, cm: Common, tol: float = 1e-10, maxiter: int = 10
     ``segments`` (row offsets of independent systems) makes the stopping test
     per system.  Returns (X, info) with info = {iterations, relres, relres0}."""
     cm.status = CHOLMOD_OK
+    if L is None or not L.is_numeric:
+        cm.status = CHOLMOD_INVALID
+        return None, None
     dev = _is_device(B)
     if dev:
         import torch
+        if not _device_ok(B, cm) or B.dim() not in (1, 2) or B.shape[0] != L.n:
+            cm.status = CHOLMOD_INVALID
+            return None, None
         Bc = B if B.dim() == 2 else B.reshape(-1, 1)
         Bc = Bc.t().contiguous().t()
         X = torch.empty_like(Bc.t().contiguous()).t()
         loc = _lib.DEVICE
     else:
-        Bc = np.asfortranarray(np.asarray(B, dtype=np.float64).reshape(L.n, -1))
+        B2 = np.asarray(B, dtype=np.float64)
+        if B2.shape[0] != L.n or B2.ndim not in (1, 2):
+            cm.status = CHOLMOD_INVALID
+            return None, None
+        Bc = np.asfortranarray(B2.reshape(L.n, -1))
         X = np.zeros_like(Bc, order="F")
         loc = _lib.HOST
     out = np.zeros(4)

@@ -623,7 +623,7 @@ __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, i
         const bool real = R + j < nb;
         if (SAFE) {
             const T floor_j = lane_bcast(floor_l, j);
-            const bool low = real && djj <= floor_j && floor_j > (T)0;
+            const bool low = real && djj > (T)0 && djj <= floor_j && floor_j > (T)0;
             djj = low ? floor_j : djj;
             npj += low ? 1 : 0;
         }
@@ -662,11 +662,12 @@ __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, i
     return badj;
 }
 
-// SAFE (rho > 0, low-precision factor): a pivot d_jj with d_jj <= rho * a_jj
+// SAFE (rho > 0, low-precision factor): a pivot d_jj with 0 < d_jj <= rho * a_jj
 // (a_jj the assembled diagonal of A) signals breakdown; it is replaced by
 // rho * a_jj and counted in *npert, and the FP64 refinement (or the FP64
-// fallback factor) repairs the perturbation.  !SAFE: exact semantics, a pivot
-// <= 0 stops.
+// fallback factor) repairs the perturbation.  A pivot <= 0 is never raised: it
+// is reported as a failing column in both modes (the host then lets an FP64
+// factorization decide whether the matrix is positive definite).
 //
 // potrf_block: the whole workgroup (256 threads) factors diagonal block t in
 // LDS (S: the block, X: its inverse, both NB x DS row-major; *sfail: a shared
@@ -870,7 +871,7 @@ __global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__
         T djj = a[j][j];
         if (SAFE) {
             const T floor_j = real ? (T)(rho * adiag[t.col0 + j]) : (T)0;
-            const bool low = real && djj <= floor_j && floor_j > (T)0;
+            const bool low = real && djj > (T)0 && djj <= floor_j && floor_j > (T)0;
             djj = low ? floor_j : djj;
             npj += low ? 1 : 0;
         }

@@ -192,8 +192,12 @@ struct mxc_factor {
     int64_t minor = 0;                     // L->minor of the last factorization
     int64_t nperturbed = 0;                // pivots raised by the low-precision safeguard
     double rho = 0;                        // pivot-ratio threshold (0: exact semantics)
-    // FP64 fallback factor of a low-precision factor whose refinement stalls
+    // FP64 fallback factor of a low-precision factor whose refinement stalls,
+    // or whose last factorization met a pivot <= 0 (escalated: the FP64
+    // factor then decides the status and serves every solve until the next
+    // factorization)
     mxc_factor *fallback = nullptr;
+    bool escalated = false;
     std::vector<int64_t> Ls_host, Ap_host, Ai_host;
     std::vector<int64_t> IPerm;              // built on the first sparse right-hand-side solve
     // host copy of the factorization's blocks (re-flattened after autotuning)
@@ -1122,6 +1126,7 @@ extern "C" int mxc_factor_info(const mxc_factor *F, int64_t *info, double *dinfo
         info[6] = (int64_t)F->bytes;
         info[7] = F->precision;
         info[8] = F->nperturbed;
+        info[9] = F->escalated ? 1 : 0;
     }
     if (dinfo) {
         dinfo[0] = F->fl;
@@ -1390,12 +1395,22 @@ static int run_factor(mxc_factor *F) {
 // copy of the assembled values.
 template <typename T>
 static int finish_not_posdef(mxc_factor *F, int fail_col, double beta);
+static int fp64_fallback(mxc_factor *F);
+
+// The factor that serves solves: the FP64 one after an escalation.
+static inline mxc_factor *active(mxc_factor *F) {
+    return (F && F->escalated && F->fallback) ? F->fallback : F;
+}
+static inline const mxc_factor *active(const mxc_factor *F) {
+    return (F && F->escalated && F->fallback) ? F->fallback : F;
+}
 
 extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, double beta,
                              int64_t *minor) {
     if (!F || (!Ax && F->nnz_a > 0)) return MXC_INVALID;
     HIPTRY(hipSetDevice(g_device));
     hipStream_t sm = F->stream;
+    F->escalated = false;
     if (F->nnz_a > 0)
         HIPTRY(hipMemcpyAsync(F->d_Ax, Ax, F->nnz_a * sizeof(double),
                               location == MXC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, sm));
@@ -1412,6 +1427,17 @@ extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, doub
         F->minor = F->n;
         if (minor) *minor = F->n;
         return MXC_OK;
+    }
+    if (F->precision != MXC_FP64 && F->rho > 0) {
+        // A pivot <= 0 in the low-precision factor: FP64 decides.  The FP64
+        // factor of the same values either succeeds (and serves the solves)
+        // or reports CHOLMOD's not-positive-definite result (its minor and L).
+        rc = fp64_fallback(F);
+        if (rc != MXC_OK && rc != MXC_NOT_POSDEF) return rc;
+        F->escalated = true;
+        F->minor = F->fallback->minor;
+        if (minor) *minor = F->minor;
+        return rc;
     }
     F->minor = fail;
     rc = F->precision == MXC_FP64 ? finish_not_posdef<double>(F, fail, beta)
@@ -1509,6 +1535,7 @@ static int finish_not_posdef(mxc_factor *F, int fail_col, double beta) {
 
 extern "C" int mxc_factor_get_Lx(const mxc_factor *F, double *Lx) {
     if (!F || !Lx) return MXC_INVALID;
+    F = active(F);
     HIPTRY(hipSetDevice(g_device));
     if (F->precision == MXC_FP64) {
         HIPTRY(hipMemcpyAsync(Lx, F->W, F->xsize * sizeof(double), hipMemcpyDeviceToHost, F->stream));
@@ -1527,7 +1554,7 @@ extern "C" int mxc_factor_get_Lx(const mxc_factor *F, double *Lx) {
 // ---------------------------------------------------------------------------
 // A level's chained sweep launch over nr <= CHAIN_RHS columns (one pass).
 template <typename T, bool FWD>
-static void launch_chain(mxc_factor *F, const Launch &L, T *Xw, int nr) {
+static void launch_chain(mxc_factor *F, const Launch &L, T *Xw, int nr, T *part) {
     hipStream_t sm = F->stream;
     const T *W = (const T *)F->W;
     const SweepTask *tk = F->d_stask + L.begin;
@@ -1542,13 +1569,13 @@ static void launch_chain(mxc_factor *F, const Launch &L, T *Xw, int nr) {
             hipLaunchKernelGGL((k_chain_fwd<T, 4>), g, b, 0, sm, tk, W, F->d_Ls, Xw, n, nr, F->d_cnt, F->d_abort);
     } else {
         if (nr == 1)
-            hipLaunchKernelGGL((k_chain_bwd<T, 1>), g, b, 0, sm, tk, W, F->d_Ls, Xw, (T *)F->d_part, n, nr,
+            hipLaunchKernelGGL((k_chain_bwd<T, 1>), g, b, 0, sm, tk, W, F->d_Ls, Xw, part, n, nr,
                                F->d_cnt, F->d_abort);
         else if (nr == 2)
-            hipLaunchKernelGGL((k_chain_bwd<T, 2>), g, b, 0, sm, tk, W, F->d_Ls, Xw, (T *)F->d_part, n, nr,
+            hipLaunchKernelGGL((k_chain_bwd<T, 2>), g, b, 0, sm, tk, W, F->d_Ls, Xw, part, n, nr,
                                F->d_cnt, F->d_abort);
         else
-            hipLaunchKernelGGL((k_chain_bwd<T, 4>), g, b, 0, sm, tk, W, F->d_Ls, Xw, (T *)F->d_part, n, nr,
+            hipLaunchKernelGGL((k_chain_bwd<T, 4>), g, b, 0, sm, tk, W, F->d_Ls, Xw, part, n, nr,
                                F->d_cnt, F->d_abort);
     }
 }
@@ -1580,9 +1607,12 @@ static int chain_status(mxc_factor *F) {
     return MXC_GPU_PROBLEM;
 }
 
+// part: the backward sweep's partial-sum slots (nslots x 64 x nrhs of T);
+// nullptr = the factor's own buffer.
 template <typename T>
-static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
+static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd, T *part = nullptr) {
     hipStream_t sm = F->stream;
+    if (!part) part = (T *)F->d_part;
     const T *W = (const T *)F->W;
     size_t tt = 0;
     int n = (int)F->n;
@@ -1613,9 +1643,9 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
                                            0, sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xz,
                                            (long long)n, zn);
                     else if (dir == 0)
-                        launch_chain<T, true>(F, L, Xz, zn);
+                        launch_chain<T, true>(F, L, Xz, zn, part);
                     else
-                        launch_chain<T, false>(F, L, Xz, zn);
+                        launch_chain<T, false>(F, L, Xz, zn, part);
                     F->counts[4]++;
                 }
             }
@@ -1649,7 +1679,7 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd)
                                    sm, F->d_stask + L.begin, (int)L.count, W, F->d_Ls, Xw, (long long)n, nr);
             else
                 hipLaunchKernelGGL(k_sweep_bwd<T>, dim3((unsigned)L.count), dim3(256), 0, sm,
-                                   F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, (T *)F->d_part,
+                                   F->d_stask + L.begin, L.tile, W, F->d_Ls, Xw, part,
                                    (long long)n, nr);
             F->counts[4]++;
         }
@@ -1722,6 +1752,7 @@ static int stage_in(mxc_factor *F, const double *B, int64_t nrhs, int location, 
 
 extern "C" int mxc_solve(mxc_factor *F, const double *B, int64_t nrhs, double *X, int location) {
     if (!F || !B || !X || nrhs < 0) return MXC_INVALID;
+    F = active(F);
     if (!F->values_loaded) return MXC_INVALID;
     if (nrhs == 0 || F->n == 0) return MXC_OK;
     HIPTRY(hipSetDevice(g_device));
@@ -1741,6 +1772,7 @@ extern "C" int mxc_solve(mxc_factor *F, const double *B, int64_t nrhs, double *X
 
 static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, bool fwd, bool bwd) {
     if (!F || !X || nrhs < 0) return MXC_INVALID;
+    F = active(F);
     if (!F->values_loaded) return MXC_INVALID;
     if (nrhs == 0 || F->n == 0) return MXC_OK;
     HIPTRY(hipSetDevice(g_device));
@@ -1788,6 +1820,7 @@ extern "C" int mxc_solve_bset(mxc_factor *F, int sys, const double *B, const int
                               double *X, int64_t *Xset, int64_t *xlen) {
     if (!F || !B || !X || !Xset || !xlen || blen < 0 || (blen > 0 && !Bset)) return MXC_INVALID;
     if (sys < MXC_SYS_A || sys > MXC_SYS_D) return MXC_INVALID;
+    F = active(F);
     if (!F->values_loaded) return MXC_INVALID;
     const int64_t n = F->n;
     for (int64_t p = 0; p < blen; p++)
@@ -1848,6 +1881,7 @@ static void launch_residual(mxc_factor *F, const double *dB, const double *dX, d
 extern "C" int mxc_residual(mxc_factor *F, const double *B, const double *X, int64_t nrhs, double *R,
                             int location) {
     if (!F || !B || !X || !R || nrhs < 0) return MXC_INVALID;
+    F = active(F);
     if (!F->values_loaded) return MXC_INVALID;
     if (nrhs == 0 || F->n == 0) return MXC_OK;
     HIPTRY(hipSetDevice(g_device));
@@ -1996,6 +2030,8 @@ static int solve_refine_impl(mxc_factor *F, const double *B, int64_t nrhs, doubl
     if (!F->values_loaded) return MXC_INVALID;
     if (nrhs == 0 || F->n == 0) return MXC_OK;
     HIPTRY(hipSetDevice(g_device));
+    const bool escalated = F->escalated && F->fallback;
+    F = active(F);
     int rc = ensure_work(F, nrhs);
     if (rc) return rc;
     const double *dB;
@@ -2005,7 +2041,7 @@ static int solve_refine_impl(mxc_factor *F, const double *B, int64_t nrhs, doubl
     rc = F->precision == MXC_FP64 ? refine_device<double>(F, dB, nrhs, dX, tol, maxiter, nseg, segoff, o)
                                   : refine_device<float>(F, dB, nrhs, dX, tol, maxiter, nseg, segoff, o);
     if (rc) return rc;
-    int used_fallback = 0;
+    int used_fallback = escalated ? 1 : 0;
     if (F->precision != MXC_FP64 && !(o[1] <= tol)) {
         // the low-precision factor did not reach tol: refactor in FP64 and refine with it
         if ((rc = fp64_fallback(F))) return rc;
@@ -2202,6 +2238,7 @@ extern "C" int mxc_solve_refine_segments(mxc_factor *F, const double *B, int64_t
 // t_cholmod_rcond_worker.c:53-143).
 extern "C" int mxc_rcond(mxc_factor *F, double *rcond) {
     if (!F || !rcond) return MXC_INVALID;
+    F = active(F);
     if (!F->values_loaded) return MXC_INVALID;
     if (F->n == 0) { *rcond = 1; return MXC_OK; }
     if (F->minor < F->n) { *rcond = 0; return MXC_OK; }

@@ -116,14 +116,18 @@ def _near_singular_pairs(m, delta, seed=0):
 
 
 def test_fp32_breakdown_falls_back_to_fp64(gpu):
-    """FP32 pivots that break down (pivot ratio ~1e-9 < rho = 1e-6) are raised and
-    counted; refinement through the perturbed FP32 factor cannot reach 1e-10, so
-    the FP64 twin factor is used and does."""
+    """FP32 pivots that break down (true pivot ratio ~1e-9 < rho = 1e-6; in FP32
+    the computed pivots scatter around 0) are raised and counted when positive,
+    and a pivot <= 0 escalates the factorization to FP64, which finds the
+    matrix positive definite (status OK).  Either way refinement through the FP32
+    factor cannot reach 1e-10, so the FP64 factor is used and does."""
     A = _near_singular_pairs(200, 1e-9)
     cm = cholmod.Common(precision="fp32")
     L = cholmod.analyze_p(A, np.arange(A.n), cm)
-    assert cholmod.factorize(A, L, cm) and cm.status == 0
-    assert L.info()["perturbed_pivots"] > 0
+    assert cholmod.factorize(A, L, cm) and cm.status == 0 and L.minor == A.n
+    inf = L.info()
+    print(inf, flush=True)
+    assert inf["perturbed_pivots"] > 0 or inf["escalated_fp64"] == 1
     # B = A X_true with X_true = O(1): then ||x|| ~ ||b||, so FP64's backward error
     # (~eps ||A|| ||x|| / ||b||) is far below 1e-10, while b's components along the
     # ~2e-9 eigenvectors (relative size ~1e-9) stall refinement through the
@@ -137,3 +141,29 @@ def test_fp32_breakdown_falls_back_to_fp64(gpu):
     assert info["relres"] <= 1e-10
     R = B - A.matvec(X)
     assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) <= 1e-9
+
+
+@pytest.mark.parametrize("side,sigma", [(40, 1.0), (30, 0.05)])
+def test_fp32_indefinite_reports_not_posdef(side, sigma, gpu, ref_lib):
+    """An indefinite matrix with a positive diagonal (5-point Laplacian minus
+    sigma*I, A_ii = 4 - sigma > 0) factored in FP32 with the pivot safeguard on:
+    a pivot <= 0 is not raised, the factorization escalates to FP64, and the
+    result is the reference's: status CHOLMOD_NOT_POSDEF, L->minor and L."""
+    A0 = problems.laplace2d_5pt(side)
+    col = np.repeat(np.arange(A0.n), np.diff(A0.p))
+    x = A0.x.copy()
+    x[A0.i == col] -= sigma
+    A = problems.SymMatrix(n=A0.n, p=A0.p, i=A0.i, x=x)
+    R = ref_lib.RefFactor(A, ref_lib.AMD)
+    sym = R.symbolic()
+    assert R.factorize() == 1
+    Lref = R.Lx()
+    cm = cholmod.Common(precision="fp32")
+    L = cholmod.analyze_p(A, sym["Perm"], cm)
+    assert cholmod.factorize(A, L, cm)
+    print(f"ref minor {R.minor}; fp32 status {cm.status} minor {L.minor} {L.info()}", flush=True)
+    assert cm.status == cholmod.CHOLMOD_NOT_POSDEF and L.minor == R.minor
+    assert L.info()["escalated_fp64"] == 1
+    err = np.abs(L.x - Lref).max() / np.abs(Lref).max()
+    assert err <= 1e-12, err
+    assert cholmod.rcond(L, cm) == 0.0
