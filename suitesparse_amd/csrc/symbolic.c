@@ -15,11 +15,11 @@
 //                                 skeleton-leaf method of Gilbert, Ng and Peyton)
 //   Super, Lpi, Lpx, Ls, sizes   CHOLMOD/Supernodal/cholmod_super_symbolic.c:391-928
 //
-// The supernode partition depends on tie-breaking details (the fundamental
-// supernode test, the merge sweep order and the path-compressed Merged links,
-// integer vs double arithmetic in the zero-count test), so those steps follow
-// the reference's sequence exactly; everything else is written from the
-// mathematics.
+// The supernode partition depends on the exact rules CHOLMOD applies (the
+// fundamental-supernode test, the order of the relaxed-merge sweep and the
+// integer / double arithmetic of its zero-count test), so those two rules are
+// stated as CHOLMOD states them; the row structure, the sizes and everything
+// else are written from their definitions.
 
 #include <math.h>
 #include <stdint.h>
@@ -225,181 +225,255 @@ static int column_counts(Int n, const Int *Fp, const Int *Fi, const Int *Parent,
     return MXC_OK;
 }
 
-// Relaxed supernodes and the supernodal pattern (cholmod_super_symbolic.c:391-928).
-static int super_symbolic(Int n, const Int *Sp, const Int *Si, const Int *Parent,
-                          const Int *ColCount, const Int nrelax[3], const double zrelax_in[3],
-                          mxc_symbolic *L) {
-    int rc = MXC_OUT_OF_MEMORY;
-    Int *Nchild = (Int *)calloc((size_t)n + 1, sizeof(Int));
-    Int *Super = (Int *)xmalloc((size_t)n + 1, sizeof(Int));
-    Int *SuperMap = (Int *)xmalloc((size_t)n, sizeof(Int));
-    Int *Sparent = (Int *)xmalloc((size_t)n, sizeof(Int));
-    Int *Snz = (Int *)xmalloc((size_t)n, sizeof(Int));
-    Int *Merged = (Int *)xmalloc((size_t)n, sizeof(Int));
-    Int *Nscol = (Int *)xmalloc((size_t)n, sizeof(Int));
-    Int *Zeros = (Int *)xmalloc((size_t)n, sizeof(Int));
-    Int *Flag = (Int *)xmalloc((size_t)n, sizeof(Int));
-    Int *Lpi2 = (Int *)xmalloc((size_t)n + 1, sizeof(Int));
+// ---------------------------------------------------------------------------
+// Supernodes.  Three steps, each in its own function:
+//   1. amalgamate(): the column partition.  Two rules decide it and are kept
+//      as CHOLMOD states them, because the partition (hence Super, Lpx and
+//      everything the numeric phase does) must be identical:
+//        - a column joins the run of the column before it when it is that
+//          column's etree parent, that column's only child, and the column
+//          before has exactly one more entry
+//          (cholmod_super_symbolic.c:391-438);
+//        - relaxed merging visits the runs from the second-to-last down to
+//          the first and lets a run absorb the run right after it when that
+//          run is (after earlier absorptions) its parent and the zero count
+//          of the union passes the nrelax/zrelax test, evaluated with the
+//          same mixed integer / double arithmetic (:470-600).
+//   2. supernode_rows(): the row structure of every supernode, sorted.  It is
+//      defined by the partition alone: the supernode's own columns, then every
+//      row k beyond them with L(k, j) != 0 for some column j of the supernode
+//      or of a supernode below it in the supernodal etree.  Built bottom-up as
+//      a union (A's entries below the supernode plus its children's rows).
+//   3. update_extents(): maxesize (most rows below a supernode's columns) and
+//      maxcsize (largest ndrow1 x ndrow2 update block, ndrow1 the rows of a
+//      descendant that fall in one ancestor, ndrow2 those rows and all below).
+// ---------------------------------------------------------------------------
+
+// One run of columns during amalgamation.
+typedef struct {
+    Int width;    // columns in the run (grows as parents are absorbed)
+    Int rows;     // entries of the run's first column of L, diagonal included
+    Int zeros;    // explicit zeros that absorbing has introduced
+    Int up;       // the run holding the etree parent of the run's last column
+    Int absorber; // EMPTY while the run is alive, else the run that absorbed it
+} run_t;
+
+static Int run_alive(run_t *r, Int t) {
+    Int top = t;
+    while (r[top].absorber != EMPTY) top = r[top].absorber;
+    while (r[t].absorber != EMPTY) {           // shorten the chain for later finds
+        Int nx = r[t].absorber;
+        r[t].absorber = top;
+        t = nx;
+    }
+    return top;
+}
+
+// May run c absorb the run right after it (p)?  *zeros_out receives the zero
+// count of the union.
+static int relaxed_ok(const run_t *c, const run_t *p, const Int nrelax[3], const double zrelax[3],
+                      Int *zeros_out) {
+    const Int w = c->width + p->width;
+    *zeros_out = p->zeros;
+    if (w <= nrelax[0]) return 1;
+    // c's columns fill up to p's first-column height: the new zeros, counted
+    // both in integers (kept) and in doubles (tested)
+    const Int add = c->width * (p->rows + c->width - c->rows);
+    const double add_d = c->width * ((double)p->rows + c->width - (double)c->rows);
+    if (add_d == 0) return 1;
+    *zeros_out = p->zeros + add;
+    const double wd = (double)w;
+    const double size_d = wd * (wd + 1) / 2 + wd * ((double)p->rows - p->width);
+    const double frac = ((double)p->zeros + add_d) / size_d;
+    const int small = (w <= nrelax[1] && frac < zrelax[0]) || (w <= nrelax[2] && frac < zrelax[1]) ||
+                      frac < zrelax[2];
+    return small && size_d < ((double)INT64_MAX) / sizeof(double);
+}
+
+// Column partition: on return Super[0..*nsuper] and Srows[0..*nsuper) (rows of
+// each supernode, own columns included).
+static int amalgamate(Int n, const Int *Parent, const Int *ColCount, const Int nrelax[3],
+                      const double zrelax_in[3], Int *Super, Int *Srows, Int *nsuper_out) {
     double zrelax[3];
-    if (!Nchild || !Super || !SuperMap || !Sparent || !Snz || !Merged || !Nscol || !Zeros ||
-        !Flag || !Lpi2)
-        goto done;
     for (int t = 0; t < 3; t++) zrelax[t] = isnan(zrelax_in[t]) ? 0 : zrelax_in[t];
-
+    Int *kids = (Int *)calloc((size_t)n + 1, sizeof(Int));
+    Int *run_of = (Int *)xmalloc((size_t)n, sizeof(Int));
+    Int *first = (Int *)xmalloc((size_t)n + 1, sizeof(Int));
+    run_t *run = (run_t *)xmalloc((size_t)n, sizeof(run_t));
+    if (!kids || !run_of || !first || !run) {
+        free(kids); free(run_of); free(first); free(run);
+        return MXC_OUT_OF_MEMORY;
+    }
     for (Int j = 0; j < n; j++)
-        if (Parent[j] != EMPTY) Nchild[Parent[j]]++;
-
-    // fundamental supernodes: j joins j-1 iff j is j-1's parent, its only child,
-    // and column j-1's pattern is column j's plus the diagonal
-    Int nfsuper = (n == 0) ? 0 : 1;
-    Super[0] = 0;
-    for (Int j = 1; j < n; j++) {
-        if (Parent[j - 1] != j || ColCount[j - 1] != ColCount[j] + 1 || Nchild[j] > 1)
-            Super[nfsuper++] = j;
-    }
-    Super[nfsuper] = n;
-    for (Int s = 0; s < nfsuper; s++)
-        for (Int k = Super[s]; k < Super[s + 1]; k++) SuperMap[k] = s;
-    for (Int s = 0; s < nfsuper; s++) {
-        Int par = Parent[Super[s + 1] - 1];
-        Sparent[s] = (par == EMPTY) ? EMPTY : SuperMap[par];
-    }
-    for (Int s = 0; s < nfsuper; s++) {
-        Merged[s] = EMPTY;
-        Nscol[s] = Super[s + 1] - Super[s];
-        Zeros[s] = 0;
-        Snz[s] = ColCount[Super[s]];
-    }
-    // relaxed amalgamation: sweep s downward, trying to absorb s+1 when it is
-    // (after earlier merges) s's parent
-    for (Int s = nfsuper - 2; s >= 0; s--) {
-        Int ss = Sparent[s];
-        if (ss == EMPTY) continue;
-        for (ss = Sparent[s]; Merged[ss] != EMPTY; ss = Merged[ss]) ;
-        Int sparent = ss;
-        for (ss = Sparent[s]; Merged[ss] != EMPTY;) {
-            Int snext = Merged[ss];
-            Merged[ss] = sparent;
-            ss = snext;
+        if (Parent[j] != EMPTY) kids[Parent[j]]++;
+    // fundamental runs
+    Int nrun = 0;
+    for (Int j = 0; j < n; j++) {
+        const int chain = j > 0 && Parent[j - 1] == j && ColCount[j - 1] == ColCount[j] + 1 && kids[j] <= 1;
+        if (!chain) {
+            first[nrun] = j;
+            run[nrun].width = 0;
+            run[nrun].rows = ColCount[j];
+            run[nrun].zeros = 0;
+            run[nrun].absorber = EMPTY;
+            nrun++;
         }
-        if (sparent != s + 1) continue;
-        Int nscol0 = Nscol[s], nscol1 = Nscol[s + 1];
-        Int ns = nscol0 + nscol1;
-        Int totzeros = Zeros[s + 1];
-        double lnz1 = (double)Snz[s + 1];
-        int merge;
-        if (ns <= nrelax[0]) {
-            merge = 1;
-        } else {
-            double lnz0 = (double)Snz[s];
-            double xnewzeros = nscol0 * (lnz1 + nscol0 - lnz0);
-            Int newzeros = nscol0 * (Snz[s + 1] + nscol0 - Snz[s]);
-            if (xnewzeros == 0) {
-                merge = 1;
-            } else {
-                double xtotzeros = ((double)totzeros) + xnewzeros;
-                double xns = (double)ns;
-                double xtotsize = (xns * (xns + 1) / 2) + xns * (lnz1 - nscol1);
-                double z = xtotzeros / xtotsize;
-                totzeros += newzeros;
-                merge = ((ns <= nrelax[1] && z < zrelax[0]) || (ns <= nrelax[2] && z < zrelax[1]) ||
-                         (z < zrelax[2])) &&
-                        (xtotsize < ((double)INT64_MAX) / sizeof(double));
+        run[nrun - 1].width++;
+        run_of[j] = nrun - 1;
+    }
+    first[nrun] = n;
+    for (Int t = 0; t < nrun; t++) {
+        const Int par = Parent[first[t + 1] - 1];
+        run[t].up = par == EMPTY ? EMPTY : run_of[par];
+    }
+    // relaxed merging, last-but-one run first
+    for (Int t = nrun - 2; t >= 0; t--) {
+        if (run[t].up == EMPTY || run_alive(run, run[t].up) != t + 1) continue;
+        Int z;
+        if (!relaxed_ok(&run[t], &run[t + 1], nrelax, zrelax, &z)) continue;
+        run[t].zeros = z;
+        run[t].rows = run[t].width + run[t + 1].rows;
+        run[t].width += run[t + 1].width;
+        run[t + 1].absorber = t;
+    }
+    Int ns = 0;
+    for (Int t = 0; t < nrun; t++) {
+        if (run[t].absorber != EMPTY) continue;
+        Super[ns] = first[t];
+        Srows[ns] = run[t].rows;
+        ns++;
+    }
+    Super[ns] = n;
+    *nsuper_out = ns;
+    free(kids); free(run_of); free(first); free(run);
+    return MXC_OK;
+}
+
+static int cmp_int64(const void *a, const void *b) {
+    const Int x = *(const Int *)a, y = *(const Int *)b;
+    return (x > y) - (x < y);
+}
+
+// Row lists of the supernodes (see the section comment).  Fp/Fi: the lower
+// pattern of the permuted matrix (column j: rows i >= j).  Lpi is set from
+// Srows; the union must reproduce those counts.
+static int supernode_rows(Int n, Int nsuper, const Int *Super, const Int *Srows, const Int *Fp,
+                          const Int *Fi, const Int *Parent, const Int *Lpi, Int *Ls) {
+    Int *owner = (Int *)xmalloc((size_t)n, sizeof(Int));
+    Int *seen = (Int *)xmalloc((size_t)n, sizeof(Int));
+    Int *head = (Int *)xmalloc((size_t)nsuper, sizeof(Int));
+    Int *next = (Int *)xmalloc((size_t)nsuper, sizeof(Int));
+    int rc = MXC_OUT_OF_MEMORY;
+    if (!owner || !seen || !head || !next) goto out;
+    for (Int s = 0; s < nsuper; s++) {
+        head[s] = EMPTY;
+        for (Int k = Super[s]; k < Super[s + 1]; k++) owner[k] = s;
+    }
+    for (Int k = 0; k < n; k++) seen[k] = EMPTY;
+    // child lists (a child's columns precede its parent's in a postorder)
+    for (Int s = nsuper - 1; s >= 0; s--) {
+        const Int par = Parent[Super[s + 1] - 1];
+        if (par == EMPTY) continue;
+        next[s] = head[owner[par]];
+        head[owner[par]] = s;
+    }
+    rc = MXC_INVALID;
+    for (Int s = 0; s < nsuper; s++) {
+        const Int c0 = Super[s], c1 = Super[s + 1];
+        Int *out = Ls + Lpi[s];
+        const Int cap = Srows[s];
+        Int cnt = 0;
+        for (Int k = c0; k < c1; k++) {
+            out[cnt++] = k;
+            seen[k] = s;
+        }
+        for (Int j = c0; j < c1; j++)
+            for (Int p = Fp[j]; p < Fp[j + 1]; p++) {
+                const Int i = Fi[p];
+                if (i < c1 || seen[i] == s) continue;
+                if (cnt == cap) goto out;
+                seen[i] = s;
+                out[cnt++] = i;
+            }
+        for (Int c = head[s]; c != EMPTY; c = next[c]) {
+            const Int own = Super[c + 1] - Super[c];
+            for (Int q = Lpi[c] + own; q < Lpi[c + 1]; q++) {
+                const Int i = Ls[q];
+                if (i < c1 || seen[i] == s) continue;
+                if (cnt == cap) goto out;
+                seen[i] = s;
+                out[cnt++] = i;
             }
         }
-        if (merge) {
-            Zeros[s] = totzeros;
-            Merged[s + 1] = s;
-            Snz[s] = nscol0 + Snz[s + 1];
-            Nscol[s] += Nscol[s + 1];
-        }
+        if (cnt != cap) goto out;
+        qsort(out + (c1 - c0), (size_t)(cnt - (c1 - c0)), sizeof(Int), cmp_int64);
     }
-    Int nsuper = 0;
-    for (Int s = 0; s < nfsuper; s++) {
-        if (Merged[s] == EMPTY) {
-            Super[nsuper] = Super[s];
-            Snz[nsuper] = Snz[s];
-            nsuper++;
-        }
-    }
-    Super[nsuper] = n;
+    rc = MXC_OK;
+out:
+    free(owner); free(seen); free(head); free(next);
+    return rc;
+}
+
+static int update_extents(Int n, Int nsuper, const Int *Super, const Int *Lpi, const Int *Ls,
+                          Int *maxcsize, Int *maxesize) {
+    Int *owner = (Int *)xmalloc((size_t)n, sizeof(Int));
+    if (!owner) return MXC_OUT_OF_MEMORY;
     for (Int s = 0; s < nsuper; s++)
-        for (Int k = Super[s]; k < Super[s + 1]; k++) SuperMap[k] = s;
-    for (Int s = 0; s < nsuper; s++) {
-        Int par = Parent[Super[s + 1] - 1];
-        Sparent[s] = (par == EMPTY) ? EMPTY : SuperMap[par];
+        for (Int k = Super[s]; k < Super[s + 1]; k++) owner[k] = s;
+    Int cmax = 1, emax = 1;
+    for (Int d = 0; d < nsuper; d++) {
+        const Int end = Lpi[d + 1];
+        Int q = Lpi[d] + (Super[d + 1] - Super[d]);
+        if (end - q > emax) emax = end - q;
+        while (q < end) {                       // one ancestor's rows per pass
+            const Int anc = owner[Ls[q]];
+            Int r = q + 1;
+            while (r < end && owner[Ls[r]] == anc) r++;
+            const Int c = (r - q) * (end - q);
+            if (c > cmax) cmax = c;
+            q = r;
+        }
     }
-    Int ssize = 0, xsize = 0;
-    for (Int s = 0; s < nsuper; s++) {
-        ssize += Snz[s];
-        xsize += (Super[s + 1] - Super[s]) * Snz[s];
-    }
+    *maxcsize = cmax;
+    *maxesize = emax;
+    free(owner);
+    return MXC_OK;
+}
+
+// Supernodal symbolic factor of the permuted matrix (lower pattern Fp/Fi),
+// given its etree and column counts: the fields CHOLMOD's super_symbolic2
+// produces (cholmod_super_symbolic.c:147-938, CPU mode).
+static int super_symbolic(Int n, const Int *Fp, const Int *Fi, const Int *Parent, const Int *ColCount,
+                          const Int nrelax[3], const double zrelax[3], mxc_symbolic *L) {
+    Int *Super = (Int *)xmalloc((size_t)n + 1, sizeof(Int));
+    Int *Srows = (Int *)xmalloc((size_t)n + 1, sizeof(Int));
+    Int nsuper = 0;
+    int rc = MXC_OUT_OF_MEMORY;
+    if (!Super || !Srows) goto done;
+    rc = amalgamate(n, Parent, ColCount, nrelax, zrelax, Super, Srows, &nsuper);
+    if (rc != MXC_OK) goto done;
+    rc = MXC_OUT_OF_MEMORY;
     L->nsuper = nsuper;
-    L->ssize = ssize > 1 ? ssize : 1;
-    L->xsize = xsize > 1 ? xsize : 1;
     L->Super = (Int *)xmalloc((size_t)nsuper + 1, sizeof(Int));
     L->Lpi = (Int *)xmalloc((size_t)nsuper + 1, sizeof(Int));
     L->Lpx = (Int *)xmalloc((size_t)nsuper + 1, sizeof(Int));
-    L->Ls = (Int *)calloc((size_t)L->ssize, sizeof(Int));
-    if (!L->Super || !L->Lpi || !L->Lpx || !L->Ls) goto done;
+    if (!L->Super || !L->Lpi || !L->Lpx) goto done;
     memcpy(L->Super, Super, sizeof(Int) * (size_t)(nsuper + 1));
-    Int p = 0;
-    for (Int s = 0; s < nsuper; s++) { L->Lpi[s] = p; p += Snz[s]; }
-    L->Lpi[nsuper] = p;
-    p = 0;
+    L->Lpi[0] = 0;
+    L->Lpx[0] = 0;
     for (Int s = 0; s < nsuper; s++) {
-        L->Lpx[s] = p;
-        p += (Super[s + 1] - Super[s]) * Snz[s];
+        L->Lpi[s + 1] = L->Lpi[s] + Srows[s];
+        L->Lpx[s + 1] = L->Lpx[s] + (Super[s + 1] - Super[s]) * Srows[s];
     }
-    L->Lpx[nsuper] = p;
-    // row indices: each supernode's own columns first, then, for every later
-    // column k, the supernodes on the path from SuperMap[i] (A(i,k) != 0, i < k1)
-    // up to the first one already holding k
-    for (Int s = 0; s < nsuper; s++) Lpi2[s] = L->Lpi[s];
-    for (Int s = 0; s < nsuper; s++) Flag[s] = EMPTY;
-    Int mark = 0;
-    Int *Ls = L->Ls;
-    for (Int s = 0; s < nsuper; s++) {
-        Int k1 = Super[s], k2 = Super[s + 1];
-        for (Int k = k1; k < k2; k++) Ls[Lpi2[s]++] = k;
-        for (Int k = k1; k < k2; k++) {
-            mark++;
-            Flag[s] = mark;
-            for (Int q = Sp[k]; q < Sp[k + 1]; q++) {
-                Int i = Si[q];
-                if (i >= k1) break;      // S columns are sorted
-                for (Int si = SuperMap[i]; Flag[si] < mark; si = Sparent[si]) {
-                    Ls[Lpi2[si]++] = k;
-                    Flag[si] = mark;
-                }
-            }
-        }
-    }
-    // largest update block and largest off-diagonal row count
-    Int maxcsize = 1, maxesize = 1;
-    for (Int d = 0; d < nsuper; d++) {
-        Int nscol = Super[d + 1] - Super[d];
-        Int q = L->Lpi[d] + nscol, plast = q, pend = L->Lpi[d + 1];
-        Int esize = pend - q;
-        if (esize > maxesize) maxesize = esize;
-        Int slast = (q == pend) ? EMPTY : SuperMap[Ls[q]];
-        for (; q <= pend; q++) {
-            Int s = (q == pend) ? EMPTY : SuperMap[Ls[q]];
-            if (s != slast) {
-                Int ndrow1 = q - plast, ndrow2 = pend - plast;
-                Int csize = ndrow2 * ndrow1;
-                if (csize > maxcsize) maxcsize = csize;
-                plast = q;
-                slast = s;
-            }
-        }
-    }
-    L->maxcsize = maxcsize;
-    L->maxesize = maxesize;
-    rc = MXC_OK;
+    L->ssize = L->Lpi[nsuper] > 1 ? L->Lpi[nsuper] : 1;
+    L->xsize = L->Lpx[nsuper] > 1 ? L->Lpx[nsuper] : 1;
+    L->Ls = (Int *)calloc((size_t)L->ssize, sizeof(Int));
+    if (!L->Ls) goto done;
+    rc = supernode_rows(n, nsuper, Super, Srows, Fp, Fi, Parent, L->Lpi, L->Ls);
+    if (rc != MXC_OK) goto done;
+    rc = update_extents(n, nsuper, Super, L->Lpi, L->Ls, &L->maxcsize, &L->maxesize);
 done:
-    free(Nchild); free(Super); free(SuperMap); free(Sparent); free(Snz); free(Merged);
-    free(Nscol); free(Zeros); free(Flag); free(Lpi2);
+    free(Super); free(Srows);
     return rc;
 }
 
@@ -489,7 +563,7 @@ int mxc_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, int ordering,
     {
         const Int nrelax[3] = {4, 16, 48};
         const double zrelax[3] = {0.8, 0.1, 0.05};
-        rc = super_symbolic(n, Sp, Si, Parent, ColCount, nrelax, zrelax, L);
+        rc = super_symbolic(n, Fp, Fi, Parent, ColCount, nrelax, zrelax, L);
         if (rc != MXC_OK) goto fail;
     }
     L->Perm = Perm;
