@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import ctypes as C
 import dataclasses
+import threading
 
 import numpy as np
 
@@ -114,6 +115,7 @@ class Factor:
     """Symbolic + (device-resident) numeric supernodal factor."""
 
     def __init__(self, sym_handle, A, cm: Common):
+        self._free_lock = threading.Lock()
         self._sym = sym_handle
         self._num = None
         self._num_precision = None
@@ -240,12 +242,18 @@ class Factor:
         return out
 
     def free(self):
-        if self._num is not None:
-            lib().mxc_factor_free(self._num)
-            self._num = None
-        if self._sym is not None:
-            lib().mxc_symbolic_free(self._sym)
-            self._sym = None
+        """Release the device factor and the symbolic factor (once; safe to call
+        from any thread and more than once)."""
+        lock = getattr(self, "_free_lock", None)
+        if lock is None:
+            return
+        with lock:
+            num, self._num = self._num, None
+            sym, self._sym = self._sym, None
+        if num is not None:
+            lib().mxc_factor_free(num)
+        if sym is not None:
+            lib().mxc_symbolic_free(sym)
 
     def __del__(self):
         try:

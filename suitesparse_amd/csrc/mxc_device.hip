@@ -58,10 +58,14 @@ extern "C" const char *mxc_last_error(void) { return g_err; }
 
 static int g_device = 0;
 
-// Graph capture and instantiation, factor creation and release, and work-buffer
-// (re)allocation are serialized across host threads (each is rare: once per
-// plan, per right-hand-side count or per growth), so that no thread captures
-// while another allocates, frees or creates streams.  Recursive: releasing a
+// HIP runtime calls that create or destroy runtime objects are serialized
+// across host threads: graph capture and instantiation, graph launches, stream
+// and event creation, device allocation and release (plan upload, work-buffer
+// growth, factor release).  Each is short or rare (once per plan, per
+// right-hand-side count or per growth), so no thread captures, launches a
+// graph or frees memory while another does.  The host part of a plan build
+// (pattern analysis, block lists, launch lists) runs outside the lock, so slot
+// threads of a server build plans concurrently.  Recursive: releasing a
 // factor also releases its FP64 fallback.
 static std::recursive_mutex g_rt_mutex;
 
@@ -509,6 +513,7 @@ static int flatten_upload(mxc_factor *F) {
     }
     F->nupd_tiles = nupd;
     if (descs.size() >= (size_t)INT32_MAX || tiles.size() >= (size_t)INT32_MAX) return MXC_TOO_LARGE;
+    std::lock_guard<std::recursive_mutex> lk0(g_rt_mutex);
     drop_factor_graph(F);
     if (F->d_desc) (void)hipFree(F->d_desc);
     if (F->d_tiles) (void)hipFree(F->d_tiles);
@@ -994,7 +999,8 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     }
     F->nslots = nslot;
 
-    // ---- device upload
+    // ---- device upload (serialized with the other threads' runtime calls)
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
     std::vector<int> Ls32(S->ssize);
     for (int64_t k = 0; k < S->ssize; k++) Ls32[k] = (int)S->Ls[k];
     std::vector<long long> perm64(S->Perm, S->Perm + n);
@@ -1085,7 +1091,6 @@ extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const
         return MXC_INVALID;
     if (precision != MXC_FP64 && precision != MXC_FP32) return MXC_INVALID;
     *out = nullptr;
-    std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
     mxc_factor *F = new (std::nothrow) mxc_factor();
     if (!F) return MXC_OUT_OF_MEMORY;
     F->precision = precision;
@@ -1096,10 +1101,13 @@ extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const
             F->tune[k][b] = (mb <= 2 && nbk <= 2) ? 32 : 64;
         }
     hipError_t e = hipSetDevice(g_device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream2, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&F->ev_fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&F->ev_join, hipEventDisableTiming);
+    {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream2, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&F->ev_fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&F->ev_join, hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         int rc = set_hip_err(e, "hipStreamCreate", __LINE__);
         mxc_factor_free(F);
@@ -1383,6 +1391,7 @@ static int run_factor(mxc_factor *F) {
         }
         F->g_factor = g;
     }
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
     HIPTRY(hipGraphLaunch(F->x_factor, sm));
     return MXC_OK;
 }
@@ -1717,6 +1726,7 @@ static int run_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd) {
         }
         it = F->g_sweep.emplace(key, std::make_pair(g, x)).first;
     }
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
     HIPTRY(hipGraphLaunch(it->second.second, sm));
     return MXC_OK;
 }
@@ -2174,6 +2184,7 @@ static int autotune_impl(mxc_factor *F, int reps, int *table) {
 extern "C" int mxc_autotune(mxc_factor *F, int reps, int *table) {
     if (!F || reps < 1) return MXC_INVALID;
     HIPTRY(hipSetDevice(g_device));
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);   // times launches on scratch descriptors
     return F->precision == MXC_FP64 ? autotune_impl<double>(F, reps, table)
                                     : autotune_impl<float>(F, reps, table);
 }
@@ -2245,7 +2256,10 @@ extern "C" int mxc_rcond(mxc_factor *F, double *rcond) {
     HIPTRY(hipSetDevice(g_device));
     const int G = 256;
     double *d_out = nullptr;
-    HIPTRY(hipMalloc((void **)&d_out, sizeof(double) * 3 * G));
+    {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+        HIPTRY(hipMalloc((void **)&d_out, sizeof(double) * 3 * G));
+    }
     if (F->precision == MXC_FP64)
         hipLaunchKernelGGL(k_diag_minmax<double>, dim3(G), dim3(256), 0, F->stream, (const double *)F->W,
                            F->d_dpos, (long long)F->n, d_out);
@@ -2255,7 +2269,10 @@ extern "C" int mxc_rcond(mxc_factor *F, double *rcond) {
     std::vector<double> h(3 * G);
     hipError_t e = hipMemcpyAsync(h.data(), d_out, sizeof(double) * 3 * G, hipMemcpyDeviceToHost, F->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(F->stream);
-    (void)hipFree(d_out);
+    {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+        (void)hipFree(d_out);
+    }
     if (e != hipSuccess) return set_hip_err(e, "rcond", __LINE__);
     double lo = INFINITY, hi = -INFINITY;
     for (int b = 0; b < G; b++) {

@@ -16,7 +16,10 @@ Plans stay resident after a request finishes and are reused by later requests
 with the same pattern (a refactorization: new values, the captured hipGraph is
 replayed).  When a new plan does not fit, least-recently-used idle plans are
 evicted.  Two requests with the same pattern share one plan, so they run one
-after the other.
+after the other: the first one admitted registers a placeholder for its
+pattern, and later ones wait for it instead of building a second plan.
+Plans are freed explicitly by the server (eviction, a failed build, stop),
+never by a finalizer on an arbitrary thread.
 
 The ordering is AMD for every request (the product's own restatement,
 bit-exact with cholmod_l_analyze's AMD).
@@ -54,11 +57,14 @@ class Request:
 
 @dataclasses.dataclass
 class _Plan:
+    """A resident plan, or the placeholder of one being built.  While
+    ``state == "building"`` the plan holds the admission estimate in
+    ``nbytes`` and one user (the slot building it), so requests with the same
+    key wait for it instead of building a second plan."""
     key: str
-    L: cholmod.Factor
-    cm: cholmod.Common
-    nbytes: int
-    lock: threading.Lock
+    handle: object = None          # backend plan (DeviceBackend: cholmod.Factor + Common)
+    nbytes: int = 0
+    state: str = "building"        # "building" -> "ready"
     last_used: float = 0.0
     users: int = 0
 
@@ -71,27 +77,69 @@ def plan_bytes_estimate(xsize: int, nsuper_cols_sq: int, nnz: int, n: int, nrhs:
     return int(t * (xsize + nsuper_cols_sq) + 40 * nnz + 8 * n * (4 * nrhs + 2) + (1 << 20))
 
 
+class DeviceBackend:
+    """Plans on the MI355X: AMD analysis + device plan per pattern; each
+    request is a refactorization (graph replay after the first) and a solve
+    with FP64 iterative refinement."""
+
+    def __init__(self, precision: str = "fp32", tol: float = 1e-10, maxiter: int = 20):
+        self.precision, self.tol, self.maxiter = precision, tol, maxiter
+
+    def init(self):
+        cm = cholmod.Common()
+        if not cm.start():
+            raise RuntimeError(f"device init failed: {cm.status}")
+
+    def estimate(self, req: Request) -> int:
+        # before analysis the factor size is unknown: a conservative fill bound
+        # from nnz(A) (replaced by the plan's real size once it exists)
+        nnz = int(req.A.p[-1])
+        xguess = 40 * nnz + 64 * req.A.n
+        return plan_bytes_estimate(xguess, 64 * req.A.n, nnz, req.A.n, req.B.shape[1], self.precision)
+
+    def build(self, req: Request):
+        cm = cholmod.Common(ordering=cholmod.CHOLMOD_AMD, precision=self.precision)
+        L = cholmod.analyze(req.A, cm)
+        if L is None:
+            raise RuntimeError(f"analyze failed: {cm.status}")
+        if L._numeric(cm) is None:
+            L.free()
+            raise RuntimeError(f"plan failed: {cm.status}")
+        nbytes = L.info()["device_bytes"] + 8 * L.n * (4 * req.B.shape[1] + 2)
+        return (L, cm), nbytes
+
+    def run(self, req: Request, handle):
+        L, cm = handle
+        if not cholmod.factorize(req.A, L, cm) or cm.status != cholmod.CHOLMOD_OK:
+            req.status = cm.status
+            return
+        X, info = cholmod.solve_refine(L, req.B, cm, tol=self.tol, maxiter=self.maxiter)
+        req.X = X
+        req.info = dict(info or {}, reserved=(req.info or {}).get("reserved", 0))
+        req.status = cm.status
+
+    def free(self, handle):
+        handle[0].free()
+
+
 class Server:
     def __init__(self, slots: int = 8, hbm_budget: int = 64 << 30, precision: str = "fp32",
-                 tol: float = 1e-10, maxiter: int = 20, keep_results: bool = True):
+                 tol: float = 1e-10, maxiter: int = 20, keep_results: bool = True, backend=None):
         self.slots = slots
         self.keep_results = keep_results   # False: drop A, B and X once a request is done
         self.budget = int(hbm_budget)
-        self.precision = precision
-        self.tol = tol
-        self.maxiter = maxiter
+        self.backend = backend or DeviceBackend(precision, tol, maxiter)
         self._cv = threading.Condition()
         self._queue: collections.deque[Request] = collections.deque()
         self._plans: dict[str, _Plan] = {}
-        self._reserved = 0              # bytes of resident plans
+        self._reserved = 0              # bytes of resident plans and placeholders
         self._stop = False
         self._threads = []
         self._t0 = None
         self.completed: list[Request] = []
         self.evictions = 0
-        cm = cholmod.Common()
-        if not cm.start():
-            raise RuntimeError(f"device init failed: {cm.status}")
+        self.builds = 0
+        self.backend.init()
 
     # -------------------------------------------------------------- lifecycle
     def start(self):
@@ -121,33 +169,32 @@ class Server:
             self._cv.notify_all()
         for th in self._threads:
             th.join()
-        for p in self._plans.values():
-            p.L.free()
-        self._plans.clear()
+        with self._cv:
+            for p in list(self._plans.values()):
+                self._release(p)
+
+    def reserved_bytes(self) -> int:
+        with self._cv:
+            return self._reserved
 
     # -------------------------------------------------------------- admission
-    def _estimate(self, req: Request) -> int:
-        # before analysis the factor size is unknown: use a conservative fill
-        # bound from nnz(A) (refined once the plan exists)
-        p = self._plans.get(req.key)
-        if p is not None:
-            return 0
-        nnz = int(req.A.p[-1])
-        xguess = 40 * nnz + 64 * req.A.n
-        return plan_bytes_estimate(xguess, 64 * req.A.n, nnz, req.A.n, req.B.shape[1],
-                                   self.precision)
-
-    def _take(self) -> tuple[Request, _Plan | None] | None:
-        """Oldest waiting request whose plan is idle and whose memory fits."""
-        for idx, req in enumerate(self._queue):
+    # Every method below runs with self._cv held.
+    def _take(self) -> tuple[Request, _Plan] | None:
+        """Oldest waiting request whose plan is idle, or whose new plan fits
+        the budget (a placeholder is registered for it at once)."""
+        idx = 0
+        while idx < len(self._queue):
+            req = self._queue[idx]
             p = self._plans.get(req.key)
             if p is not None:
-                if p.users == 0:
+                if p.users == 0 and p.state == "ready":
                     del self._queue[idx]
                     p.users += 1
+                    req.reused_plan = True
                     return req, p
+                idx += 1          # its plan is busy or still being built: wait for it
                 continue
-            need = self._estimate(req)
+            need = self.backend.estimate(req)
             if need > self.budget:
                 # can never fit: fail it (CHOLMOD_OUT_OF_MEMORY) instead of waiting forever
                 del self._queue[idx]
@@ -156,27 +203,39 @@ class Server:
                 req.start = req.done = self.now()
                 self.completed.append(req)
                 self._cv.notify_all()
-                return self._take()
+                continue
             if self._reserved + need > self.budget:
                 self._evict(self._reserved + need - self.budget)
             if self._reserved + need <= self.budget:
                 del self._queue[idx]
                 self._reserved += need
                 req.info = {"reserved": need}
-                return req, None
+                p = _Plan(key=req.key, nbytes=need, users=1)
+                self._plans[req.key] = p
+                return req, p
             return None          # head-of-line: keep arrival order for new patterns
         return None
 
+    def _release(self, p: _Plan):
+        """Drop a plan from the table and free its device memory (explicitly,
+        here, never from a finalizer on some other thread)."""
+        if self._plans.get(p.key) is p:
+            del self._plans[p.key]
+        self._reserved -= p.nbytes
+        p.nbytes = 0
+        if p.handle is not None:
+            h, p.handle = p.handle, None
+            self.backend.free(h)
+
     def _evict(self, nbytes: int):
-        idle = sorted((p for p in self._plans.values() if p.users == 0), key=lambda p: p.last_used)
+        idle = sorted((p for p in self._plans.values() if p.users == 0 and p.state == "ready"),
+                      key=lambda p: p.last_used)
         freed = 0
         for p in idle:
             if freed >= nbytes:
                 break
-            del self._plans[p.key]
-            self._reserved -= p.nbytes
             freed += p.nbytes
-            p.L.free()
+            self._release(p)
             self.evictions += 1
 
     # -------------------------------------------------------------- worker
@@ -194,11 +253,9 @@ class Server:
             req, plan = got
             req.start = self.now()
             try:
-                if plan is None:
-                    plan = self._build_plan(req)
-                else:
-                    req.reused_plan = True
-                self._run(req, plan)
+                if plan.state == "building":
+                    self._build(req, plan)
+                self.backend.run(req, plan.handle)
             except Exception as e:      # report, never kill the slot
                 req.status = -5
                 req.info = {"error": repr(e)}
@@ -207,39 +264,22 @@ class Server:
             if not self.keep_results:
                 req.A = req.B = req.X = None
             with self._cv:
-                if plan is not None:
-                    plan.users -= 1
-                    plan.last_used = req.done
+                plan.users -= 1
+                plan.last_used = req.done
+                if plan.state == "building":
+                    self._release(plan)  # the build failed: give back its reservation
                 self.completed.append(req)
                 self._cv.notify_all()
 
-    def _build_plan(self, req: Request) -> _Plan:
-        cm = cholmod.Common(ordering=cholmod.CHOLMOD_AMD, precision=self.precision)
-        L = cholmod.analyze(req.A, cm)
-        if L is None:
-            raise RuntimeError(f"analyze failed: {cm.status}")
-        if L._numeric(cm) is None:
-            raise RuntimeError(f"plan failed: {cm.status}")
-        nbytes = L.info()["device_bytes"] + 8 * L.n * (4 * req.B.shape[1] + 2)
-        plan = _Plan(key=req.key, L=L, cm=cm, nbytes=nbytes, lock=threading.Lock(), users=1)
+    def _build(self, req: Request, plan: _Plan):
+        handle, nbytes = self.backend.build(req)
         with self._cv:
             # swap the admission estimate for the plan's real size
-            self._reserved += nbytes - req.info.get("reserved", 0)
-            self._plans[req.key] = plan
-        return plan
-
-    def _run(self, req: Request, plan: _Plan):
-        cm = plan.cm
-        if not cholmod.factorize(req.A, plan.L, cm):
-            req.status = cm.status
-            return
-        if cm.status != cholmod.CHOLMOD_OK:
-            req.status = cm.status
-            return
-        X, info = cholmod.solve_refine(plan.L, req.B, cm, tol=self.tol, maxiter=self.maxiter)
-        req.X = X
-        req.info = dict(info or {}, reserved=req.info.get("reserved", 0) if req.info else 0)
-        req.status = cm.status
+            self._reserved += nbytes - plan.nbytes
+            plan.nbytes = nbytes
+            plan.handle = handle
+            plan.state = "ready"
+            self.builds += 1
 
 
 # ----------------------------------------------------------------------------
@@ -348,7 +388,12 @@ def run_stream(server: Server, reqs, arrivals, bank: PatternBank | None = None,
     by a pool of producer threads, in arrival order, at most `lookahead` ahead.
     progress > 0 prints a line each time that many more requests were submitted."""
     import concurrent.futures
+    import faulthandler
     import queue
+    import sys
+    if not faulthandler.is_enabled():
+        # a crash in native code then prints every thread's Python stack
+        faulthandler.enable(file=sys.stderr, all_threads=True)
     ready: "queue.Queue" = queue.Queue(maxsize=lookahead)
     bank = bank or PatternBank()
 
