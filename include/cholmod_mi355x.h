@@ -182,6 +182,15 @@ int mxc_lsolve_pattern(const mxc_symbolic *S, int sys, const int64_t *Bset, int6
  * early (minor < n) or a diagonal entry is NaN; 1 when n = 0. */
 int mxc_rcond(mxc_factor *F, double *rcond);
 
+/* mxc_rcond for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops): no counterpart in the reference.  Same result, bit
+ * for bit, but it calls no hipMalloc / hipFree: the per-block partials
+ * (3 x 256 doubles, independent of n) live in a static device array, one per
+ * GPU, set up by mxc_device_init (which must have been called for the current
+ * device; else MXC_INVALID).  Calls on one GPU share that array and take turns
+ * (a process-wide lock; the call is synchronous). */
+int mxc_rcond_noalloc(mxc_factor *F, double *rcond);
+
 /* R = B - A X in FP64 with the values of the last mxc_factorize. */
 int mxc_residual(mxc_factor *F, const double *B, const double *X, int64_t nrhs, double *R,
                  int location);

@@ -53,6 +53,7 @@ _PROTOS = {
     "mxc_set_tiles": (i32, [vp, vp]),
     "mxc_get_tiles": (i32, [vp, vp]),
     "mxc_rcond": (i32, [vp, vp]),
+    "mxc_rcond_noalloc": (i32, [vp, vp]),
     "mxc_synchronize": (i32, [vp]),
     "mxc_set_option": (i32, [vp, i32, i64]),
     "mxc_set_timing": (i32, [vp, i32]),

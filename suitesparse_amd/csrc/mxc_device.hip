@@ -69,6 +69,18 @@ static int g_device = 0;
 // factor also releases its FP64 fallback.
 static std::recursive_mutex g_rt_mutex;
 
+// Scratch of mxc_rcond_noalloc: the per-block (min, max, has_nan) partials of
+// k_diag_minmax, which always runs RCOND_BLOCKS blocks (a grid-stride loop over
+// the n diagonal entries, so the size does not depend on the factor).  Each GPU
+// holds its own copy, from the first mxc_device_init on it until the process
+// ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for that device (g_scratch).
+static constexpr int RCOND_BLOCKS = 256;
+static constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[3 * RCOND_BLOCKS];
+static double *g_scratch[MAX_DEVICES];
+static std::mutex g_scratch_mutex;           // calls sharing one GPU's array take turns
+
 extern "C" int mxc_device_init(int device) {
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
@@ -79,6 +91,14 @@ extern "C" int mxc_device_init(int device) {
     if (device < 0 || device >= count) return MXC_INVALID;
     HIPTRY(hipSetDevice(device));
     g_device = device;
+    if (device < MAX_DEVICES) {
+        std::lock_guard<std::mutex> lk(g_scratch_mutex);
+        if (!g_scratch[device]) {
+            void *p = nullptr;
+            HIPTRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_scratch[device] = (double *)p;
+        }
+    }
     return MXC_OK;
 }
 
@@ -2246,36 +2266,24 @@ extern "C" int mxc_solve_refine_segments(mxc_factor *F, const double *B, int64_t
 
 // (min diag(L) / max diag(L))^2, 0 if any diagonal entry is NaN or the
 // factorization stopped early, 1 for n = 0 (CHOLMOD/Cholesky/cholmod_rcond.c:51-112,
-// t_cholmod_rcond_worker.c:53-143).
-extern "C" int mxc_rcond(mxc_factor *F, double *rcond) {
-    if (!F || !rcond) return MXC_INVALID;
-    F = active(F);
-    if (!F->values_loaded) return MXC_INVALID;
-    if (F->n == 0) { *rcond = 1; return MXC_OK; }
-    if (F->minor < F->n) { *rcond = 0; return MXC_OK; }
-    HIPTRY(hipSetDevice(g_device));
-    const int G = 256;
-    double *d_out = nullptr;
-    {
-        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
-        HIPTRY(hipMalloc((void **)&d_out, sizeof(double) * 3 * G));
-    }
+// t_cholmod_rcond_worker.c:53-143).  The diagonal's min / max / NaN flag are
+// reduced per block by k_diag_minmax into d_out (3 * RCOND_BLOCKS doubles) and
+// finished on the host.
+static int rcond_partials(mxc_factor *F, double *d_out, double *rcond) {
     if (F->precision == MXC_FP64)
-        hipLaunchKernelGGL(k_diag_minmax<double>, dim3(G), dim3(256), 0, F->stream, (const double *)F->W,
-                           F->d_dpos, (long long)F->n, d_out);
+        hipLaunchKernelGGL(k_diag_minmax<double>, dim3(RCOND_BLOCKS), dim3(256), 0, F->stream,
+                           (const double *)F->W, F->d_dpos, (long long)F->n, d_out);
     else
-        hipLaunchKernelGGL(k_diag_minmax<float>, dim3(G), dim3(256), 0, F->stream, (const float *)F->W,
-                           F->d_dpos, (long long)F->n, d_out);
-    std::vector<double> h(3 * G);
-    hipError_t e = hipMemcpyAsync(h.data(), d_out, sizeof(double) * 3 * G, hipMemcpyDeviceToHost, F->stream);
+        hipLaunchKernelGGL(k_diag_minmax<float>, dim3(RCOND_BLOCKS), dim3(256), 0, F->stream,
+                           (const float *)F->W, F->d_dpos, (long long)F->n, d_out);
+    double h[3 * RCOND_BLOCKS];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, F->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(F->stream);
-    {
-        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
-        (void)hipFree(d_out);
-    }
     if (e != hipSuccess) return set_hip_err(e, "rcond", __LINE__);
     double lo = INFINITY, hi = -INFINITY;
-    for (int b = 0; b < G; b++) {
+    for (int b = 0; b < RCOND_BLOCKS; b++) {
         if (h[3 * b + 2] != 0) { *rcond = 0; return MXC_OK; }
         lo = std::min(lo, h[3 * b]);
         hi = std::max(hi, h[3 * b + 1]);
@@ -2283,6 +2291,55 @@ extern "C" int mxc_rcond(mxc_factor *F, double *rcond) {
     double r = lo / hi;
     *rcond = r * r;
     return MXC_OK;
+}
+
+// The cases decided without the device: 1 for n = 0, 0 after a factorization
+// that stopped early.  Returns 1 when *rcond is set.
+static int rcond_trivial(const mxc_factor *F, double *rcond) {
+    if (F->n == 0) { *rcond = 1; return 1; }
+    if (F->minor < F->n) { *rcond = 0; return 1; }
+    return 0;
+}
+
+extern "C" int mxc_rcond(mxc_factor *F, double *rcond) {
+    if (!F || !rcond) return MXC_INVALID;
+    F = active(F);
+    if (!F->values_loaded) return MXC_INVALID;
+    if (rcond_trivial(F, rcond)) return MXC_OK;
+    HIPTRY(hipSetDevice(g_device));
+    double *d_out = nullptr;
+    {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+        HIPTRY(hipMalloc((void **)&d_out, sizeof(double) * 3 * RCOND_BLOCKS));
+    }
+    int rc = rcond_partials(F, d_out, rcond);
+    {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+        (void)hipFree(d_out);
+    }
+    return rc;
+}
+
+// mxc_rcond without any allocation: the partials go to this GPU's `scratch`
+// array (through its device address), so the call makes no hipMalloc / hipFree
+// and no host allocation.  Calls on one GPU share the array and are serialized
+// by g_scratch_mutex; the call is synchronous, so its contents are dead on
+// return (the next call overwrites all 3 * RCOND_BLOCKS entries before reading).
+extern "C" int mxc_rcond_noalloc(mxc_factor *F, double *rcond) {
+    if (!F || !rcond) return MXC_INVALID;
+    F = active(F);
+    if (!F->values_loaded) return MXC_INVALID;
+    if (rcond_trivial(F, rcond)) return MXC_OK;
+    static_assert(3 * RCOND_BLOCKS * sizeof(double) == sizeof(scratch), "scratch holds one partial per block");
+    if (g_device < 0 || g_device >= MAX_DEVICES || !g_scratch[g_device]) {
+        snprintf(g_err, sizeof(g_err),
+                 "mxc_rcond_noalloc: no scratch address for device %d (call mxc_device_init first; "
+                 "devices 0..%d are supported)", g_device, MAX_DEVICES - 1);
+        return MXC_INVALID;
+    }
+    HIPTRY(hipSetDevice(g_device));
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    return rcond_partials(F, g_scratch[g_device], rcond);
 }
 
 extern "C" int mxc_synchronize(mxc_factor *F) {
