@@ -70,6 +70,7 @@ extern "C" {
 
 typedef struct mxc_symbolic mxc_symbolic;   /* CPU symbolic factor */
 typedef struct mxc_factor mxc_factor;       /* device plan + numeric factor */
+typedef struct mxc_pool mxc_pool;           /* device memory pool (cholmod_common.dev_mempool) */
 
 /* ---- symbolic analysis (CPU) ------------------------------------------- */
 
@@ -107,12 +108,33 @@ int mxc_device_init(int device);
 /* info[0..3] = compute units, HBM bytes, gcn arch number (950), clock kHz */
 int mxc_device_info(int64_t *info);
 
+/* Free and total device memory of the current device (cholmod_l_gpu_memorysize,
+ * CHOLMOD/GPU/cholmod_gpu.c:63-110). */
+int mxc_memory_info(int64_t *free_bytes, int64_t *total_bytes);
+
+/* Device memory pool: one allocation of `bytes` on the current device, from
+ * which factors created with mxc_factor_create_in take all their device
+ * memory (first fit, coalescing frees).  Replaces the Common-owned
+ * dev_mempool of cholmod_l_gpu_allocate (CHOLMOD/GPU/cholmod_gpu.c:426-543) and
+ * its release in cholmod_l_gpu_deallocate (:234-260).  A factor that does not
+ * fit gets MXC_OUT_OF_MEMORY.  info[0..3] = capacity, bytes in use, peak bytes
+ * in use, live allocations.  mxc_pool_free returns MXC_INVALID (and frees
+ * nothing) while factors still hold memory from the pool. */
+int mxc_pool_create(size_t bytes, mxc_pool **P);
+int mxc_pool_info(const mxc_pool *P, int64_t *info);
+int mxc_pool_free(mxc_pool *P);
+
 /* ---- numeric factorization (device) ------------------------------------ */
 
 /* Build the device plan for S and the pattern of tril(A) (the pattern the
  * values passed to mxc_factorize follow).  precision: MXC_FP64 or MXC_FP32. */
 int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const int64_t *Ai,
                       int precision, mxc_factor **F);
+
+/* As mxc_factor_create, with every device buffer of the factor (and of its
+ * FP64 fallback, if one is ever built) taken from pool P (NULL: hipMalloc). */
+int mxc_factor_create_in(const mxc_symbolic *S, const int64_t *Ap, const int64_t *Ai,
+                         int precision, mxc_pool *P, mxc_factor **F);
 
 void mxc_factor_free(mxc_factor *F);
 
@@ -267,6 +289,15 @@ int mxc_synchronize(mxc_factor *F);
  * potrf, trsm, trailing, solve. */
 int mxc_set_timing(mxc_factor *F, int mask);
 int mxc_get_timing(const mxc_factor *F, double *t, int64_t *counts);
+
+/* Per-factor statistics for cholmod_common's GPU fields (gpuNumKernelLaunches,
+ * gpuKernelTime, cholmod_gpu_*_calls; CHOLMOD/Include/cholmod.h:921-986).
+ * counts[0..7] = launches per factorization of the descendant-update,
+ * diagonal-POTRF, TRSM and trailing-update classes, their sum, launches per
+ * solve (both sweeps), factorizations and solves done so far;
+ * ms[0..3] = device time (HIP events on the factor's stream) of the last
+ * factorization and the last solve call, and their sums so far. */
+int mxc_factor_stats(const mxc_factor *F, int64_t *counts, double *ms);
 
 /* Last HIP error text seen by the library (empty when none). */
 const char *mxc_last_error(void);

@@ -37,6 +37,12 @@ _PROTOS = {
     "mxc_device_init": (i32, [i32]),
     "mxc_device_info": (i32, [vp]),
     "mxc_factor_create": (i32, [vp, vp, vp, i32, C.POINTER(vp)]),
+    "mxc_factor_create_in": (i32, [vp, vp, vp, i32, vp, C.POINTER(vp)]),
+    "mxc_pool_create": (i32, [C.c_size_t, C.POINTER(vp)]),
+    "mxc_pool_info": (i32, [vp, vp]),
+    "mxc_pool_free": (i32, [vp]),
+    "mxc_memory_info": (i32, [vp, vp]),
+    "mxc_factor_stats": (i32, [vp, vp, vp]),
     "mxc_factor_free": (None, [vp]),
     "mxc_factor_info": (i32, [vp, vp, vp]),
     "mxc_factorize": (i32, [vp, vp, i32, f64, pi64]),

@@ -31,6 +31,34 @@ def _run(cmd, verbose):
     subprocess.run([str(c) for c in cmd], check=True)
 
 
+# The CHOLMOD drop-in (libcholmod_mi355x_dropin.so) compiles against the host
+# CHOLMOD's headers: cholmod.h fixes the struct layouts and prototypes it
+# implements.  SUITESPARSE_ROOT names a SuiteSparse source tree that holds them.
+SUITESPARSE_ROOT = pathlib.Path(os.environ.get("SUITESPARSE_ROOT", "/root/reference"))
+DROPIN = PKG / "libcholmod_mi355x_dropin.so"
+
+
+def cholmod_headers():
+    inc = [SUITESPARSE_ROOT / "CHOLMOD" / "Include", SUITESPARSE_ROOT / "SuiteSparse_config"]
+    return inc if all((d / h).exists() for d, h in zip(inc, ("cholmod.h", "SuiteSparse_config.h"))) else None
+
+
+def build_dropin(verbose: bool = False):
+    """CHOLMOD's cholmod_l_analyze / factorize / solve over this library, for
+    linking in front of a host CHOLMOD (skipped when its headers are absent)."""
+    inc = cholmod_headers()
+    if inc is None:
+        return None
+    BUILD.mkdir(exist_ok=True)
+    o = BUILD / "cholmod_dropin.c.o"
+    _run(["gcc", *C_FLAGS, *[f"-I{d}" for d in inc], "-c", CSRC / "cholmod_dropin.c", "-o", o], verbose)
+    tmp = DROPIN.with_suffix(".so.tmp")
+    _run(["gcc", "-shared", "-fPIC", o, "-o", tmp, f"-L{PKG}", "-lcholmod_mi355x", "-Wl,-rpath,$ORIGIN",
+          "-ldl", "-lpthread"], verbose)
+    os.replace(tmp, DROPIN)
+    return DROPIN
+
+
 def build(verbose: bool = False) -> pathlib.Path:
     BUILD.mkdir(exist_ok=True)
     objs = []
@@ -45,6 +73,7 @@ def build(verbose: bool = False) -> pathlib.Path:
     tmp = OUT.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     os.replace(tmp, OUT)
+    build_dropin(verbose)
     return OUT
 
 

@@ -153,6 +153,60 @@ enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5,
        L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11, L_FCHAIN = 12, L_BCHAIN = 13,
        L_TRAILD = 14, L_FTINY = 15, L_BTINY = 16, L_DIAG_TINY = 17, L_TRSM_TINY = 18 };
 
+// ---------------------------------------------------------------------------
+// device memory pool: one hipMalloc, sub-allocated first-fit with coalescing
+// (the Common-owned dev_mempool of CHOLMOD/GPU/cholmod_gpu.c:426-543, here
+// holding whole factors instead of per-supernode buffers)
+// ---------------------------------------------------------------------------
+struct mxc_pool {
+    int device = 0;
+    char *base = nullptr;
+    size_t cap = 0, used = 0, peak = 0;
+    std::map<size_t, size_t> holes;        // offset -> bytes, no two adjacent
+    std::map<size_t, size_t> live;         // offset -> bytes
+    std::mutex mu;
+};
+static constexpr size_t POOL_ALIGN = 256;
+
+static void *pool_take(mxc_pool *P, size_t bytes) {
+    bytes = (bytes + POOL_ALIGN - 1) / POOL_ALIGN * POOL_ALIGN;
+    std::lock_guard<std::mutex> lk(P->mu);
+    for (auto it = P->holes.begin(); it != P->holes.end(); ++it) {
+        if (it->second < bytes) continue;
+        const size_t off = it->first, rest = it->second - bytes;
+        P->holes.erase(it);
+        if (rest) P->holes[off + bytes] = rest;
+        P->live[off] = bytes;
+        P->used += bytes;
+        P->peak = std::max(P->peak, P->used);
+        return P->base + off;
+    }
+    return nullptr;
+}
+
+static void pool_give(mxc_pool *P, void *ptr) {
+    std::lock_guard<std::mutex> lk(P->mu);
+    const size_t off = (size_t)((char *)ptr - P->base);
+    auto lv = P->live.find(off);
+    if (lv == P->live.end()) return;
+    size_t a = off, b = off + lv->second;
+    P->used -= lv->second;
+    P->live.erase(lv);
+    auto nx = P->holes.lower_bound(a);
+    if (nx != P->holes.end() && nx->first == b) {            // merge with the hole after
+        b += nx->second;
+        nx = P->holes.erase(nx);
+    }
+    if (nx != P->holes.begin()) {                           // merge with the hole before
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == a) {
+            a = pv->first;
+            P->holes.erase(pv);
+        }
+    }
+    P->holes[a] = b - a;
+}
+
 struct StepBlocks {
     std::vector<DiagTask> diags;
     std::vector<UpdDesc> trsm, trail;
@@ -178,6 +232,15 @@ struct mxc_factor {
     bool diag_tails = true;              // trailing launches factor the next diagonal blocks
     bool merge_t2 = true;                // FP64: fold second-stream updates into later trailing launches
     size_t bytes = 0;
+    mxc_pool *pool = nullptr;            // device memory comes from this pool when set
+    // device time of the last factorization / solve (event pairs on the factor's
+    // stream around each call) and their running sums
+    hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
+    double ms_factor = 0, ms_solve = 0, ms_factor_sum = 0, ms_solve_sum = 0;
+    int64_t nfactor = 0, nsolve = 0;
+    long long *d_seg = nullptr;          // refinement over segments: offsets and per-segment norms,
+    double *d_segout = nullptr;          // grown on demand
+    int64_t seg_cap = 0, segout_cap = 0;
 
     // host copies
     std::vector<int64_t> Super, Lpi, Lpx, Perm;
@@ -252,14 +315,46 @@ struct mxc_factor {
     std::vector<std::pair<int, size_t>> pending;   // (class, index of the start event)
 };
 
-template <typename X>
-static int dalloc(mxc_factor *F, X **p, size_t n) {
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void **)p, n * sizeof(X));
+static inline mxc_factor *active(mxc_factor *F);
+static inline const mxc_factor *active(const mxc_factor *F);
+
+// Device memory of a factor: from its pool when it has one, else hipMalloc.
+static int falloc(mxc_factor *F, void **p, size_t bytes) {
+    if (F->pool) {
+        *p = pool_take(F->pool, bytes);
+        if (!*p) {
+            snprintf(g_err, sizeof(g_err), "device pool exhausted: %zu bytes requested, %zu of %zu in use",
+                     bytes, F->pool->used, F->pool->cap);
+            return MXC_OUT_OF_MEMORY;
+        }
+        return MXC_OK;
+    }
+    hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
         return set_hip_err(e, "hipMalloc", __LINE__);
     }
+    return MXC_OK;
+}
+
+// Release memory of a factor.  A pool region can be handed out again at once,
+// so the factor's streams are drained first (hipFree synchronizes by itself).
+static void ffree(mxc_factor *F, void *p) {
+    if (!p) return;
+    if (F->pool) {
+        if (F->stream) (void)hipStreamSynchronize(F->stream);
+        if (F->stream2) (void)hipStreamSynchronize(F->stream2);
+        pool_give(F->pool, p);
+    } else {
+        (void)hipFree(p);
+    }
+}
+
+template <typename X>
+static int dalloc(mxc_factor *F, X **p, size_t n) {
+    if (n == 0) n = 1;
+    int rc = falloc(F, (void **)p, n * sizeof(X));
+    if (rc) return rc;
     F->bytes += n * sizeof(X);
     return MXC_OK;
 }
@@ -535,9 +630,9 @@ static int flatten_upload(mxc_factor *F) {
     if (descs.size() >= (size_t)INT32_MAX || tiles.size() >= (size_t)INT32_MAX) return MXC_TOO_LARGE;
     std::lock_guard<std::recursive_mutex> lk0(g_rt_mutex);
     drop_factor_graph(F);
-    if (F->d_desc) (void)hipFree(F->d_desc);
-    if (F->d_tiles) (void)hipFree(F->d_tiles);
-    if (F->d_diag) (void)hipFree(F->d_diag);
+    ffree(F, F->d_desc);
+    ffree(F, F->d_tiles);
+    ffree(F, F->d_diag);
     F->d_desc = nullptr;
     F->d_tiles = nullptr;
     F->d_diag = nullptr;
@@ -1067,17 +1162,19 @@ static int ensure_work(mxc_factor *F, int64_t nrhs) {
     std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
     drop_sweep_graphs(F);
     size_t esz = F->precision == MXC_FP64 ? sizeof(double) : sizeof(float);
-    if (F->d_work) { (void)hipFree(F->d_work); F->d_work = nullptr; }
-    if (F->d_part) { (void)hipFree(F->d_part); F->d_part = nullptr; }
-    if (F->d_b) { (void)hipFree(F->d_b); F->d_b = nullptr; }
-    if (F->d_x) { (void)hipFree(F->d_x); F->d_x = nullptr; }
-    if (F->d_r) { (void)hipFree(F->d_r); F->d_r = nullptr; }
+    void **bufs[] = {&F->d_work, &F->d_part, (void **)&F->d_b, (void **)&F->d_x, (void **)&F->d_r};
+    for (void **b : bufs) {
+        ffree(F, *b);
+        *b = nullptr;
+    }
+    F->work_nrhs = 0;
     size_t len = (size_t)std::max<int64_t>(F->n, 1) * (size_t)nrhs;
-    HIPTRY(hipMalloc(&F->d_work, len * esz));
-    HIPTRY(hipMalloc(&F->d_part, (size_t)std::max<int64_t>(F->nslots, 1) * NB * (size_t)nrhs * esz));
-    HIPTRY(hipMalloc((void **)&F->d_b, len * sizeof(double)));
-    HIPTRY(hipMalloc((void **)&F->d_x, len * sizeof(double)));
-    HIPTRY(hipMalloc((void **)&F->d_r, len * sizeof(double)));
+    const size_t sizes[] = {len * esz, (size_t)std::max<int64_t>(F->nslots, 1) * NB * (size_t)nrhs * esz,
+                            len * sizeof(double), len * sizeof(double), len * sizeof(double)};
+    for (int k = 0; k < 5; k++) {
+        int rc = falloc(F, bufs[k], sizes[k]);
+        if (rc) return rc;
+    }
     F->work_nrhs = nrhs;
     return MXC_OK;
 }
@@ -1091,12 +1188,13 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
     void *ptrs[] = {F->W, F->d_Lh, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
                     F->d_diag, F->d_stask, F->d_fail, F->d_adiag, F->d_rowptr, F->d_src, F->d_col,
                     F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_part, F->d_b, F->d_x, F->d_r,
-                    F->d_cnt, F->d_abort};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+                    F->d_cnt, F->d_abort, F->d_seg, F->d_segout};
+    for (void *p : ptrs) ffree(F, p);
 
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
     F->ev.clear();
+    for (hipEvent_t e : {F->ev_f0, F->ev_f1, F->ev_s0, F->ev_s1})
+        if (e) (void)hipEventDestroy(e);
     if (F->ev_fork) (void)hipEventDestroy(F->ev_fork);
     if (F->ev_join) (void)hipEventDestroy(F->ev_join);
     if (F->stream2) (void)hipStreamDestroy(F->stream2);
@@ -1104,15 +1202,20 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
     delete F;
 }
 
-extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const int64_t *Ai,
-                                 int precision, mxc_factor **out) {
+extern "C" int mxc_factor_create_in(const mxc_symbolic *S, const int64_t *Ap, const int64_t *Ai,
+                                    int precision, mxc_pool *pool, mxc_factor **out) {
     if (!S || !out || (S->n > 0 && (!Ap || !Ai)) || !S->Super || !S->Lpi || !S->Lpx || !S->Ls ||
         !S->Perm)
         return MXC_INVALID;
     if (precision != MXC_FP64 && precision != MXC_FP32) return MXC_INVALID;
+    if (pool && pool->device != g_device) {
+        snprintf(g_err, sizeof(g_err), "pool is on device %d, current device is %d", pool->device, g_device);
+        return MXC_INVALID;
+    }
     *out = nullptr;
     mxc_factor *F = new (std::nothrow) mxc_factor();
     if (!F) return MXC_OUT_OF_MEMORY;
+    F->pool = pool;
     F->precision = precision;
     F->rho = precision == MXC_FP32 ? 1e-6 : 0.0;
     for (int k = 0; k < NKIND; k++)
@@ -1127,6 +1230,8 @@ extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&F->stream2, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&F->ev_fork, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&F->ev_join, hipEventDisableTiming);
+        for (hipEvent_t *ev : {&F->ev_f0, &F->ev_f1, &F->ev_s0, &F->ev_s1})
+            if (e == hipSuccess) e = hipEventCreate(ev);
     }
     if (e != hipSuccess) {
         int rc = set_hip_err(e, "hipStreamCreate", __LINE__);
@@ -1139,6 +1244,101 @@ extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const
         return rc;
     }
     *out = F;
+    return MXC_OK;
+}
+
+extern "C" int mxc_factor_create(const mxc_symbolic *S, const int64_t *Ap, const int64_t *Ai,
+                                 int precision, mxc_factor **out) {
+    return mxc_factor_create_in(S, Ap, Ai, precision, nullptr, out);
+}
+
+extern "C" int mxc_pool_create(size_t bytes, mxc_pool **out) {
+    if (!out || bytes == 0) return MXC_INVALID;
+    *out = nullptr;
+    HIPTRY(hipSetDevice(g_device));
+    mxc_pool *P = new (std::nothrow) mxc_pool();
+    if (!P) return MXC_OUT_OF_MEMORY;
+    P->device = g_device;
+    P->cap = bytes / POOL_ALIGN * POOL_ALIGN;
+    hipError_t e;
+    {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+        e = hipMalloc((void **)&P->base, P->cap);
+    }
+    if (e != hipSuccess) {
+        delete P;
+        return set_hip_err(e, "hipMalloc (pool)", __LINE__);
+    }
+    P->holes[0] = P->cap;
+    *out = P;
+    return MXC_OK;
+}
+
+extern "C" int mxc_pool_info(const mxc_pool *P, int64_t *info) {
+    if (!P || !info) return MXC_INVALID;
+    mxc_pool *Q = const_cast<mxc_pool *>(P);
+    std::lock_guard<std::mutex> lk(Q->mu);
+    info[0] = (int64_t)P->cap;
+    info[1] = (int64_t)P->used;
+    info[2] = (int64_t)P->peak;
+    info[3] = (int64_t)P->live.size();
+    return MXC_OK;
+}
+
+extern "C" int mxc_pool_free(mxc_pool *P) {
+    if (!P) return MXC_OK;
+    {
+        std::lock_guard<std::mutex> lk(P->mu);
+        if (!P->live.empty()) {
+            snprintf(g_err, sizeof(g_err), "pool still holds %zu allocations (free their factors first)",
+                     P->live.size());
+            return MXC_INVALID;
+        }
+    }
+    {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+        (void)hipSetDevice(P->device);
+        (void)hipFree(P->base);
+    }
+    delete P;
+    return MXC_OK;
+}
+
+extern "C" int mxc_memory_info(int64_t *free_bytes, int64_t *total_bytes) {
+    size_t f = 0, t = 0;
+    HIPTRY(hipSetDevice(g_device));
+    HIPTRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
+    return MXC_OK;
+}
+
+extern "C" int mxc_factor_stats(const mxc_factor *F, int64_t *counts, double *ms) {
+    if (!F) return MXC_INVALID;
+    F = active(F);
+    if (counts) {
+        for (int k = 0; k < 8; k++) counts[k] = 0;
+        for (const Launch &L : F->factor_launches) {
+            switch (L.kind) {
+                case L_UPD: counts[0]++; break;
+                case L_DIAG: case L_DIAG_SMALL: case L_DIAG_TINY: counts[1]++; break;
+                case L_TRSM: case L_TRSM_TINY: counts[2]++; break;
+                case L_TRAIL: case L_TRAIL2: case L_TRAILD: counts[3]++; break;
+                default: break;
+            }
+        }
+        counts[4] = counts[0] + counts[1] + counts[2] + counts[3];
+        counts[5] = (int64_t)(F->chain ? F->fwd_chain.size() + F->bwd_chain.size()
+                                       : F->fwd_launches.size() + F->bwd_launches.size());
+        counts[6] = F->nfactor;
+        counts[7] = F->nsolve;
+    }
+    if (ms) {
+        ms[0] = F->ms_factor;
+        ms[1] = F->ms_solve;
+        ms[2] = F->ms_factor_sum;
+        ms[3] = F->ms_solve_sum;
+    }
     return MXC_OK;
 }
 
@@ -1440,15 +1640,28 @@ extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, doub
     HIPTRY(hipSetDevice(g_device));
     hipStream_t sm = F->stream;
     F->escalated = false;
+    HIPTRY(hipEventRecord(F->ev_f0, sm));
     if (F->nnz_a > 0)
         HIPTRY(hipMemcpyAsync(F->d_Ax, Ax, F->nnz_a * sizeof(double),
                               location == MXC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, sm));
     HIPTRY(hipMemcpyAsync(F->d_beta, &beta, sizeof(double), hipMemcpyHostToDevice, sm));
     int rc = F->precision == MXC_FP64 ? run_factor<double>(F) : run_factor<float>(F);
-    if (rc) return rc;
+    if (rc) {
+        (void)hipStreamSynchronize(sm);       // the queued copies read host memory of the caller
+        return rc;
+    }
+    HIPTRY(hipEventRecord(F->ev_f1, sm));
     int flags[2] = {0, 0};
     HIPTRY(hipMemcpyAsync(flags, F->d_fail, sizeof(int) * 2, hipMemcpyDeviceToHost, sm));
     HIPTRY(hipStreamSynchronize(sm));
+    {
+        float t = 0;
+        if (hipEventElapsedTime(&t, F->ev_f0, F->ev_f1) == hipSuccess) {
+            F->ms_factor = t;
+            F->ms_factor_sum += t;
+        }
+        F->nfactor++;
+    }
     int fail = flags[0];
     F->nperturbed = flags[1];
     F->values_loaded = true;
@@ -1770,6 +1983,19 @@ static int solve_device(mxc_factor *F, const double *dB, int64_t nrhs, double *d
     return MXC_OK;
 }
 
+// Device time of a solve call: events around its work on the factor's stream,
+// read after the call's final synchronization.
+static void solve_t0(mxc_factor *F) { (void)hipEventRecord(F->ev_s0, F->stream); }
+static void solve_t1(mxc_factor *F) { (void)hipEventRecord(F->ev_s1, F->stream); }
+static void solve_done(mxc_factor *F) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, F->ev_s0, F->ev_s1) == hipSuccess) {
+        F->ms_solve = t;
+        F->ms_solve_sum += t;
+    }
+    F->nsolve++;
+}
+
 static int stage_in(mxc_factor *F, const double *B, int64_t nrhs, int location, const double **dB) {
     if (location == MXC_DEVICE) {
         *dB = B;
@@ -1789,14 +2015,20 @@ extern "C" int mxc_solve(mxc_factor *F, const double *B, int64_t nrhs, double *X
     int rc = ensure_work(F, nrhs);
     if (rc) return rc;
     const double *dB;
+    solve_t0(F);
     if ((rc = stage_in(F, B, nrhs, location, &dB))) return rc;
     double *dX = location == MXC_DEVICE ? X : F->d_x;
     rc = F->precision == MXC_FP64 ? solve_device<double>(F, dB, nrhs, dX)
                                   : solve_device<float>(F, dB, nrhs, dX);
-    if (rc) return rc;
+    if (rc) {
+        (void)hipStreamSynchronize(F->stream);
+        return rc;
+    }
     if (location != MXC_DEVICE)
         HIPTRY(hipMemcpyAsync(X, dX, F->n * nrhs * sizeof(double), hipMemcpyDeviceToHost, F->stream));
+    solve_t1(F);
     HIPTRY(hipStreamSynchronize(F->stream));
+    solve_done(F);
     return chain_status(F);
 }
 
@@ -1810,6 +2042,7 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
     if (rc) return rc;
     hipStream_t sm = F->stream;
     size_t len = (size_t)F->n * nrhs;
+    solve_t0(F);
     if (location != MXC_DEVICE)
         HIPTRY(hipMemcpyAsync(F->d_x, X, len * sizeof(double), hipMemcpyHostToDevice, sm));
     double *dX = location == MXC_DEVICE ? X : F->d_x;
@@ -1825,10 +2058,15 @@ static int super_sweep(mxc_factor *F, double *X, int64_t nrhs, int location, boo
         hipLaunchKernelGGL((k_cast<float, double>), g, dim3(256), 0, sm, (const float *)Xw, (long long)len, dX);
         HIPTRY(hipGetLastError());
     }
-    if (rc) return rc;
+    if (rc) {
+        (void)hipStreamSynchronize(sm);
+        return rc;
+    }
     if (location != MXC_DEVICE)
         HIPTRY(hipMemcpyAsync(X, dX, len * sizeof(double), hipMemcpyDeviceToHost, sm));
+    solve_t1(F);
     HIPTRY(hipStreamSynchronize(sm));
+    solve_done(F);
     return chain_status(F);
 }
 
@@ -1967,21 +2205,23 @@ static int refine_device(mxc_factor *F, const double *dB, int64_t nrhs, double *
     std::vector<double> bn, rn;
     long long *d_seg = nullptr;
     double *d_out = F->d_norm;
-    struct Free {
-        void *a, *b;
-        ~Free() {
-            if (!a && !b) return;
-            std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
-            if (a) (void)hipFree(a);
-            if (b) (void)hipFree(b);
-        }
-    } fr{nullptr, nullptr};
     if (nseg > 0) {
-        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
-        HIPTRY(hipMalloc((void **)&d_seg, sizeof(long long) * (nseg + 1)));
-        fr.a = d_seg;
-        HIPTRY(hipMalloc((void **)&d_out, sizeof(double) * nseg * nrhs));
-        fr.b = d_out;
+        // segment offsets and per-segment norms: buffers of the factor, grown on demand
+        if (F->seg_cap < nseg + 1 || F->segout_cap < nseg * nrhs) {
+            std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+            ffree(F, F->d_seg);
+            ffree(F, F->d_segout);
+            F->d_seg = nullptr;
+            F->d_segout = nullptr;
+            F->seg_cap = F->segout_cap = 0;
+            int rc = falloc(F, (void **)&F->d_seg, sizeof(long long) * (nseg + 1));
+            if (!rc) rc = falloc(F, (void **)&F->d_segout, sizeof(double) * nseg * std::max<int64_t>(nrhs, 4));
+            if (rc) return rc;
+            F->seg_cap = nseg + 1;
+            F->segout_cap = nseg * std::max<int64_t>(nrhs, 4);
+        }
+        d_seg = F->d_seg;
+        d_out = F->d_segout;
         HIPTRY(hipMemcpyAsync(d_seg, segoff, sizeof(long long) * (nseg + 1), hipMemcpyHostToDevice, sm));
     }
     int rc = colnorms(F, dB, nrhs, nseg, d_seg, d_out, bn);
@@ -2039,7 +2279,8 @@ static int fp64_fallback(mxc_factor *F) {
         S.Lpi = F->Lpi.data();
         S.Lpx = F->Lpx.data();
         S.Ls = F->Ls_host.data();
-        int rc = mxc_factor_create(&S, F->Ap_host.data(), F->Ai_host.data(), MXC_FP64, &F->fallback);
+        int rc = mxc_factor_create_in(&S, F->Ap_host.data(), F->Ai_host.data(), MXC_FP64, F->pool,
+                                      &F->fallback);
         if (rc) return rc;
     }
     double beta = 0;
@@ -2065,6 +2306,7 @@ static int solve_refine_impl(mxc_factor *F, const double *B, int64_t nrhs, doubl
     int rc = ensure_work(F, nrhs);
     if (rc) return rc;
     const double *dB;
+    solve_t0(F);
     if ((rc = stage_in(F, B, nrhs, location, &dB))) return rc;
     double *dX = location == MXC_DEVICE ? X : F->d_x;
     double o[3] = {0, 0, 0};
@@ -2094,7 +2336,9 @@ static int solve_refine_impl(mxc_factor *F, const double *B, int64_t nrhs, doubl
     }
     if (location != MXC_DEVICE)
         HIPTRY(hipMemcpyAsync(X, dX, F->n * nrhs * sizeof(double), hipMemcpyDeviceToHost, F->stream));
+    solve_t1(F);
     HIPTRY(hipStreamSynchronize(F->stream));
+    solve_done(F);
     return MXC_OK;
 }
 

@@ -1,0 +1,218 @@
+/* dropin_flow.c -- a CHOLMOD caller, written against cholmod.h only, that
+ * runs the sequence of CHOLMOD/Demo/cholmod_dl_demo.c:287-759 on one matrix
+ * file twice side by side: once through the MI355X drop-in (Common->useGPU =
+ * 1) and once on the host CHOLMOD (useGPU = 0), and prints what the test
+ * compares, one "key value" per line.
+ *
+ * Linked as: dropin_flow -> libcholmod_mi355x_dropin.so (analyze, factorize,
+ * solve, solve2, free_factor, finish, gpu_stats) -> the host CHOLMOD (here the
+ * reference compiled from its sources) for everything else.
+ *
+ *   dropin_flow <matrix file> [supernodal] [gpu|host]
+ *
+ * supernodal: Common->supernodal = CHOLMOD_SUPERNODAL (default: CHOLMOD_AUTO).
+ * gpu / host: run only that side (default: both, and compare).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "cholmod.h"
+
+typedef struct {
+    cholmod_common c;
+    cholmod_sparse *A;
+    cholmod_factor *L;
+    cholmod_dense *B, *X;
+    double Lx_copy_n;
+    double *Lx;            /* L->x right after the factorization (before any Bset solve) */
+    int is_super, status, minor, nsuper;
+    double rcond, resid, resid_ir, bset_err, solve2_err, fl, lnz;
+    int ordering, is_super_after_bset;
+} side;
+
+static double norm_inf_dense(cholmod_dense *X) {
+    double m = 0;
+    double *x = (double *)X->x;
+    for (size_t j = 0; j < X->ncol; j++)
+        for (size_t i = 0; i < X->nrow; i++) m = fmax(m, fabs(x[i + j * X->d]));
+    return m;
+}
+
+/* |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf), the demo's residual (cholmod_dl_demo.c:614-617) */
+static double resid_of(side *s, cholmod_dense *X) {
+    double one[2] = {1, 0}, minusone[2] = {-1, 0};
+    cholmod_dense *R = cholmod_l_copy_dense(s->B, &s->c);
+    cholmod_l_sdmult(s->A, 0, minusone, one, X, R, &s->c);
+    double r = cholmod_l_norm_dense(R, 0, &s->c);
+    double anorm = cholmod_l_norm_sparse(s->A, 0, &s->c);
+    double v = r / (anorm * norm_inf_dense(X) + norm_inf_dense(s->B) + (s->A->nrow == 0));
+    cholmod_l_free_dense(&R, &s->c);
+    return v;
+}
+
+static int run(side *s, const char *path, int gpu, int supernodal) {
+    cholmod_l_start(&s->c);
+    s->c.print = 1;
+    s->c.useGPU = gpu;
+    s->c.maxGpuMemBytes = (size_t)1 << 30;     /* a 1 GiB pool, not the whole device */
+    if (supernodal) s->c.supernodal = CHOLMOD_SUPERNODAL;
+    FILE *f = fopen(path, "r");
+    if (!f) return 1;
+    s->A = cholmod_l_read_sparse(f, &s->c);
+    fclose(f);
+    if (!s->A) return 2;
+    const size_t n = s->A->nrow;
+    s->B = cholmod_l_zeros(n, 1, CHOLMOD_REAL + CHOLMOD_DOUBLE, &s->c);
+    double *b = (double *)s->B->x;
+    for (size_t i = 0; i < n; i++) b[i] = 1 + (double)i / (double)n;
+
+    s->L = cholmod_l_analyze(s->A, &s->c);
+    if (!s->L) return 3;
+    s->fl = s->c.fl;
+    s->lnz = s->c.lnz;
+    s->ordering = s->L->ordering;
+    for (int t = 0; t < 3; t++) cholmod_l_factorize(s->A, s->L, &s->c);
+    s->status = s->c.status;
+    s->minor = (int)s->L->minor;
+    s->is_super = s->L->is_super;
+    s->nsuper = (int)s->L->nsuper;
+    if (s->L->is_super && s->L->x) {
+        s->Lx_copy_n = (double)s->L->xsize;
+        s->Lx = (double *)malloc(s->L->xsize * sizeof(double));
+        memcpy(s->Lx, s->L->x, s->L->xsize * sizeof(double));
+    }
+    s->rcond = cholmod_l_rcond(s->L, &s->c);
+    if (s->status != CHOLMOD_OK) return 0;
+
+    /* solve, and solve again with a reused X and workspaces (solve2) */
+    s->X = cholmod_l_solve(CHOLMOD_A, s->L, s->B, &s->c);
+    s->resid = resid_of(s, s->X);
+    cholmod_dense *X2 = NULL, *Y = NULL, *E = NULL;
+    for (int t = 0; t < 3; t++) cholmod_l_solve2(CHOLMOD_A, s->L, s->B, NULL, &X2, NULL, &Y, &E, &s->c);
+    s->solve2_err = 0;
+    for (size_t i = 0; i < n; i++)
+        s->solve2_err = fmax(s->solve2_err, fabs(((double *)X2->x)[i] - ((double *)s->X->x)[i]));
+
+    /* sparse right-hand sides: x(Xset) for b = e_i * (3.1 i + 0.9) */
+    cholmod_dense *B2 = cholmod_l_zeros(n, 1, CHOLMOD_REAL + CHOLMOD_DOUBLE, &s->c), *X3 = NULL, *Xfull = NULL;
+    cholmod_sparse *Bset = cholmod_l_allocate_sparse(n, 1, 1, 0, 1, 0, CHOLMOD_PATTERN + CHOLMOD_DOUBLE, &s->c);
+    cholmod_sparse *Xset = NULL;
+    ((int64_t *)Bset->p)[0] = 0;
+    ((int64_t *)Bset->p)[1] = 1;
+    s->bset_err = 0;
+    for (size_t i = 0; i < (n < 20 ? n : 20); i++) {
+        ((int64_t *)Bset->i)[0] = (int64_t)i;
+        ((double *)B2->x)[i] = 3.1 * (double)i + 0.9;
+        cholmod_l_solve2(CHOLMOD_A, s->L, B2, NULL, &Xfull, NULL, &Y, &E, &s->c);
+        cholmod_l_solve2(CHOLMOD_A, s->L, B2, Bset, &X3, &Xset, &Y, &E, &s->c);
+        const int64_t xlen = ((int64_t *)Xset->p)[1];
+        for (int64_t k = 0; k < xlen; k++) {
+            const int64_t j = ((int64_t *)Xset->i)[k];
+            s->bset_err = fmax(s->bset_err, fabs(((double *)X3->x)[j] - ((double *)Xfull->x)[j]));
+        }
+        ((double *)B2->x)[i] = 0;
+    }
+    s->is_super_after_bset = s->L->is_super;
+    cholmod_l_free_dense(&B2, &s->c);
+    cholmod_l_free_dense(&X2, &s->c);
+    cholmod_l_free_dense(&X3, &s->c);
+    cholmod_l_free_dense(&Xfull, &s->c);
+    cholmod_l_free_dense(&Y, &s->c);
+    cholmod_l_free_dense(&E, &s->c);
+    cholmod_l_free_sparse(&Bset, &s->c);
+    cholmod_l_free_sparse(&Xset, &s->c);
+
+    /* one step of iterative refinement (cholmod_dl_demo.c:627-663) */
+    double one[2] = {1, 0}, minusone[2] = {-1, 0};
+    cholmod_dense *R = cholmod_l_copy_dense(s->B, &s->c);
+    cholmod_l_sdmult(s->A, 0, minusone, one, s->X, R, &s->c);
+    cholmod_dense *R2 = cholmod_l_solve(CHOLMOD_A, s->L, R, &s->c);
+    cholmod_dense *Xir = cholmod_l_copy_dense(s->X, &s->c);
+    for (size_t i = 0; i < n; i++) ((double *)Xir->x)[i] += ((double *)R2->x)[i];
+    s->resid_ir = resid_of(s, Xir);
+    cholmod_l_free_dense(&R, &s->c);
+    cholmod_l_free_dense(&R2, &s->c);
+    cholmod_l_free_dense(&Xir, &s->c);
+    return 0;
+}
+
+static void report(const char *tag, side *s) {
+    printf("%s_status %d\n%s_minor %d\n%s_is_super %d\n%s_nsuper %d\n%s_ordering %d\n", tag, s->status, tag,
+           s->minor, tag, s->is_super, tag, s->nsuper, tag, s->ordering);
+    printf("%s_fl %.17g\n%s_lnz %.17g\n%s_rcond %.17g\n%s_resid %.6e\n%s_resid_ir %.6e\n", tag, s->fl, tag,
+           s->lnz, tag, s->rcond, tag, s->resid, tag, s->resid_ir);
+    printf("%s_bset_err %.6e\n%s_solve2_err %.6e\n%s_is_super_after_bset %d\n", tag, s->bset_err, tag,
+           s->solve2_err, tag, s->is_super_after_bset);
+    printf("%s_gpu_launches %d\n%s_gpu_time %.6e\n%s_pool_bytes %.0f\n", tag, s->c.gpuNumKernelLaunches, tag,
+           s->c.gpuKernelTime, tag, (double)s->c.dev_mempool_size);
+    printf("%s_gpu_calls %.0f %.0f %.0f %.0f\n", tag, (double)s->c.cholmod_gpu_syrk_calls,
+           (double)s->c.cholmod_gpu_gemm_calls, (double)s->c.cholmod_gpu_potrf_calls,
+           (double)s->c.cholmod_gpu_trsm_calls);
+}
+
+static void finish(side *s) {
+    cholmod_l_free_factor(&s->L, &s->c);
+    cholmod_l_free_dense(&s->X, &s->c);
+    cholmod_l_free_dense(&s->B, &s->c);
+    cholmod_l_free_sparse(&s->A, &s->c);
+    cholmod_l_finish(&s->c);
+    free(s->Lx);
+}
+
+int main(int argc, char **argv) {
+    if (argc < 2) {
+        fprintf(stderr, "usage: %s <matrix file> [supernodal] [gpu|host]\n", argv[0]);
+        return 2;
+    }
+    int supernodal = 0, do_gpu = 1, do_host = 1;
+    for (int k = 2; k < argc; k++) {
+        if (!strcmp(argv[k], "supernodal")) supernodal = 1;
+        if (!strcmp(argv[k], "gpu")) do_host = 0;
+        if (!strcmp(argv[k], "host")) do_gpu = 0;
+    }
+    static side g, h;
+    if (do_gpu) {
+        int rc = run(&g, argv[1], 1, supernodal);
+        printf("gpu_rc %d\n", rc);
+        if (rc) return 1;
+        report("gpu", &g);
+    }
+    if (do_host) {
+        int rc = run(&h, argv[1], 0, supernodal);
+        printf("host_rc %d\n", rc);
+        if (rc) return 1;
+        report("host", &h);
+    }
+    if (do_gpu && do_host) {
+        /* the two factors side by side (before either side's Bset solves) */
+        double d = 0, m = 0;
+        if (g.Lx && h.Lx && g.Lx_copy_n == h.Lx_copy_n) {
+            for (size_t k = 0; k < (size_t)g.Lx_copy_n; k++) {
+                d = fmax(d, fabs(g.Lx[k] - h.Lx[k]));
+                m = fmax(m, fabs(h.Lx[k]));
+            }
+            printf("L_rel_diff %.6e\n", m > 0 ? d / m : d);
+        } else {
+            printf("L_rel_diff -1\n");
+        }
+        if (g.X && h.X) {
+            double num = 0, den = 0;
+            for (size_t i = 0; i < g.X->nrow; i++) {
+                const double a = ((double *)g.X->x)[i], b = ((double *)h.X->x)[i];
+                num += (a - b) * (a - b);
+                den += b * b;
+            }
+            printf("x_rel_diff %.6e\n", den > 0 ? sqrt(num / den) : sqrt(num));
+        }
+        printf("perm_equal %d\n", g.L && h.L && g.L->n == h.L->n &&
+               !memcmp(g.L->Perm, h.L->Perm, g.L->n * sizeof(int64_t)));
+    }
+    if (do_gpu) {
+        cholmod_l_gpu_stats(&g.c);
+        finish(&g);
+    }
+    if (do_host) finish(&h);
+    return 0;
+}

@@ -164,6 +164,32 @@ def test_fp32_indefinite_reports_not_posdef(side, sigma, gpu, ref_lib):
     print(f"ref minor {R.minor}; fp32 status {cm.status} minor {L.minor} {L.info()}", flush=True)
     assert cm.status == cholmod.CHOLMOD_NOT_POSDEF and L.minor == R.minor
     assert L.info()["escalated_fp64"] == 1
-    err = np.abs(L.x - Lref).max() / np.abs(Lref).max()
-    assert err <= 1e-12, err
+    # The leading minor x minor block is positive definite but can be close to
+    # singular (sigma = 0.05 sits just past lambda_min = 0.0205 of the 30 x 30
+    # Laplacian), so entries of two correctly rounded factors differ by up to
+    # ~u * kappa of that block: the entrywise bound is scaled by the block's
+    # condition estimate (max/min pivot)^2, and the block's backward error
+    # ||L11 L11' - (P A P')11|| / ||A|| (independent of conditioning) is held
+    # to 1e-13.
+    from oracle import restate
+    Lx = L.x
+    m = R.minor
+    Ld = restate.dense_L(sym, Lx)
+    piv = np.diag(Ld)[:m]
+    kappa = float((piv.max() / piv.min()) ** 2)
+    err = np.abs(Lx - Lref).max() / np.abs(Lref).max()
+    print(f"  L max rel err {err:.3e}, leading-block condition estimate {kappa:.3e}", flush=True)
+    assert err <= 1e-13 * max(10.0, kappa), (err, kappa)
+    Ad = A.to_dense() if hasattr(A, "to_dense") else None
+    if Ad is None:
+        import scipy.sparse as sp
+        T = sp.csc_matrix((A.x, A.i, A.p), shape=(A.n, A.n)).toarray()
+        Ad = T + np.tril(T, -1).T
+    perm = sym["Perm"]
+    PAP = Ad[np.ix_(perm, perm)]
+    L11 = Ld[:m, :m]
+    berr = np.abs(L11 @ L11.T - PAP[:m, :m]).max() / np.abs(Ad).max()
+    print(f"  leading-block backward error {berr:.3e}", flush=True)
+    assert berr <= 1e-13, berr
+    assert not Ld[:, m:].any()      # everything from the failing column on is zero
     assert cholmod.rcond(L, cm) == 0.0
