@@ -139,9 +139,9 @@ int mxc_factor_create_in(const mxc_symbolic *S, const int64_t *Ap, const int64_t
 void mxc_factor_free(mxc_factor *F);
 
 /* info[0..9] = n, nsuper, xsize, levels, update pairs, update tiles,
- *              device bytes held, precision, pivots perturbed by the
- *              low-precision safeguard in the last factorization, 1 if the
- *              last factorization escalated to FP64 (see mxc_factorize);
+ *              device bytes held, precision, diagonal blocks the last
+ *              factorization refactored in FP64 (see MXC_OPT_PIVOT_RHO_E9), 1 if
+ *              the last factorization escalated to FP64 (see mxc_factorize);
  * dinfo[0..5] = fl, lnz, and the algorithmic flops of one factorization by
  *               kernel class: descendant updates (dsyrk + dgemm counts), diagonal
  *               POTRF, panel TRSM, in-supernode trailing updates */
@@ -231,12 +231,16 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * solve sweeps into hipGraphs on first use and replay them afterwards (the
  * launch sequence depends only on the pattern); 0 enqueues every launch. */
 #define MXC_OPT_GRAPHS 1
-/* MXC_OPT_PIVOT_RHO_E9: pivot-ratio threshold rho in units of 1e-9 (FP32
- * factors default to rho = 1e-6, FP64 to 0).  With rho > 0, a diagonal-block
- * pivot d_jj <= rho * a_jj (a_jj the assembled diagonal of A) is raised to
- * rho * a_jj and counted (mxc_factor_info info[8]); the refinement, or the
- * FP64 fallback factor, corrects the perturbation.  rho = 0 keeps CHOLMOD's
- * semantics: a pivot <= 0 stops with MXC_NOT_POSDEF. */
+/* MXC_OPT_PIVOT_RHO_E9: pivot-ratio threshold rho in units of 1e-9 for FP32
+ * factors (default rho = 1e-6; FP64 factors ignore it).  With rho > 0, a
+ * 64-column (or smaller) diagonal block in which a pivot falls to
+ * rho * a_jj or below (a_jj the assembled diagonal of A) is refactored in
+ * FP64 on the device before anything reads it: from A's FP64 values plus
+ * the descendant updates accumulated in FP32, FP64 POTRF and inverse, the
+ * result stored in FP32; the factorization then continues in FP32
+ * (mxc_factor_info info[8] counts such blocks).  A pivot <= 0 even in FP64
+ * escalates the factorization to FP64 (see mxc_factorize).  rho = 0 keeps
+ * CHOLMOD's semantics in FP32: a pivot <= 0 stops with MXC_NOT_POSDEF. */
 #define MXC_OPT_PIVOT_RHO_E9 2
 /* MXC_OPT_LOOKAHEAD (default 1): run the update of each finished 384-column
  * panel of a large supernode right of the next panel on a second stream, so

@@ -174,7 +174,7 @@ class Factor:
         dinfo = np.zeros(6)
         check("mxc_factor_info", lib().mxc_factor_info(self._num, _ptr(info), _ptr(dinfo)))
         keys = ("n", "nsuper", "xsize", "levels", "pairs", "update_tiles", "device_bytes", "precision",
-                "perturbed_pivots", "escalated_fp64")
+                "fp64_blocks", "escalated_fp64")
         return dict(zip(keys, (int(v) for v in info)))
 
     def set_timing(self, on, classes=None):
@@ -186,7 +186,8 @@ class Factor:
         check("mxc_set_timing", lib().mxc_set_timing(self._num, int(mask)))
 
     def set_pivot_rho(self, rho: float):
-        """Pivot-ratio threshold of the low-precision safeguard (0 = CHOLMOD semantics)."""
+        """Pivot-ratio threshold of an FP32 factor: a diagonal block with a pivot at
+        or below rho * a_jj is refactored in FP64 (0 = CHOLMOD semantics)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 2, int(round(rho * 1e9))))
 
     def set_lookahead(self, on: bool):

@@ -152,10 +152,24 @@ __device__ __forceinline__ unsigned split_pack(float v, int e) {
            ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
 }
 
+// Queue of the diagonal blocks of a low-precision factor whose pivot ratio
+// signalled breakdown, for k_potrf_redo64 (which refactors them in FP64):
+// counters [0] blocks queued since the last redo launch, [1] blocks
+// refactored in FP64 in this factorization, [2] redo workgroups finished.
+struct RedoQ {
+    int *cnt;
+    DiagTask *list;
+};
+
+__device__ __forceinline__ void redo_push(const RedoQ &rq, const DiagTask &t) {
+    const int k = atomicAdd(rq.cnt, 1);
+    rq.list[k] = t;
+}
+
 template <typename T, bool SAFE>
 __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W, T *S, T *X, int *sfail,
                                             int *__restrict__ fail, const double *__restrict__ adiag,
-                                            double rho, int *__restrict__ npert);
+                                            double rho, RedoQ rq);
 
 // Arguments of the diagonal-block tail of a trailing launch (PD): after its
 // epilogue, the workgroup of tile (0, 0) of a block with pd >= 0 has the final
@@ -164,20 +178,20 @@ struct PdArgs {
     const DiagTask *diags;
     int *fail;
     double rho;
-    int *npert;
+    RedoQ rq;
 };
 
 // The tail itself: S and X of potrf_block carved from the tile's LDS buffer.
 template <typename T>
 __device__ __forceinline__ void pd_tail(const PdArgs &pda, int pd, T *__restrict__ W, unsigned char *smem,
                                         const double *__restrict__ adiag) {
-    __shared__ int sfail;
+    __shared__ int sfail[2];
     __syncthreads();                        // the tile's stores and LDS reads are done
     T *S = reinterpret_cast<T *>(smem);
     if (pda.rho > 0)
-        potrf_block<T, true>(pda.diags[pd], W, S, S + NB * DS, &sfail, pda.fail, adiag, pda.rho, pda.npert);
+        potrf_block<T, true>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rq);
     else
-        potrf_block<T, false>(pda.diags[pd], W, S, S + NB * DS, &sfail, pda.fail, adiag, pda.rho, pda.npert);
+        potrf_block<T, false>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rq);
 }
 
 template <typename T, int MODE, int TS, bool PD = false>
@@ -603,30 +617,27 @@ __device__ __forceinline__ void store16(T *S, int r0, int c0, const typename Mfm
 // goes to X in the same layout.  Each pivot costs one reciprocal square root
 // (no divide on the serial path): L_jj = d_jj r_j and L_ij = a_ij r_j with
 // r_j = 1/sqrt(d_jj); the inverse reuses r_j = 1/L_jj.  Returns the first
-// non-positive pivot of the block (16 if none), wave-uniform.
+// non-positive pivot of the block (16 if none), wave-uniform.  SAFE: *brk is
+// set when a pivot falls to rho * a_jj or below (a_jj the assembled diagonal
+// of A): the block's result is then not used (k_potrf_redo64 refactors it).
 template <typename T, bool SAFE>
 __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, int lane,
                                             const double *__restrict__ adiag, int col0, double rho,
-                                            int *__restrict__ npert) {
+                                            bool *brk) {
     T a[16], rinv[16];
 #pragma unroll
     for (int c = 0; c < 16; c++) a[c] = (lane < 16) ? S[(R + lane) * ld + R + c] : (T)0;
     // pivot floors of the panel's columns (low-precision safeguard), lane j holds column j's
     T floor_l = 0;
     if (SAFE && lane < 16 && R + lane < nb) floor_l = (T)(rho * adiag[col0 + R + lane]);
-    // wave-uniform bookkeeping, written once after the panel
-    int npj = 0;        // pivots raised to their floor
-    int badj = 16;      // first non-positive pivot of the panel
+    // wave-uniform bookkeeping
+    bool low_any = false;   // a pivot at or below its floor
+    int badj = 16;          // first non-positive pivot of the panel
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         T djj = lane_bcast(a[j], j);
         const bool real = R + j < nb;
-        if (SAFE) {
-            const T floor_j = lane_bcast(floor_l, j);
-            const bool low = real && djj > (T)0 && djj <= floor_j && floor_j > (T)0;
-            djj = low ? floor_j : djj;
-            npj += low ? 1 : 0;
-        }
+        if (SAFE) low_any = low_any || (real && djj <= lane_bcast(floor_l, j));
         const bool bad = !(djj > (T)0);
         badj = (bad && real && badj == 16) ? j : badj;
         djj = bad ? (T)1 : djj;
@@ -637,7 +648,7 @@ __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, i
 #pragma unroll
         for (int c = j + 1; c < 16; c++) a[c] -= l * lane_bcast(l, c);
     }
-    if (SAFE && lane == 0 && npj > 0) atomicAdd(npert, npj);
+    if (SAFE && low_any) *brk = true;
     if (lane < 16) {
 #pragma unroll
         for (int c = 0; c < 16; c++)
@@ -662,20 +673,21 @@ __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, i
     return badj;
 }
 
-// SAFE (rho > 0, low-precision factor): a pivot d_jj with 0 < d_jj <= rho * a_jj
-// (a_jj the assembled diagonal of A) signals breakdown; it is replaced by
-// rho * a_jj and counted in *npert, and the FP64 refinement (or the FP64
-// fallback factor) repairs the perturbation.  A pivot <= 0 is never raised: it
-// is reported as a failing column in both modes (the host then lets an FP64
-// factorization decide whether the matrix is positive definite).
+// SAFE (rho > 0, low-precision factor): a pivot d_jj <= rho * a_jj (a_jj the
+// assembled diagonal of A) signals breakdown of the block.  Its low-precision
+// result is dropped: the block stays as assembled in W and goes to the redo
+// queue, and the next launch (k_potrf_redo64) refactors it in FP64 from A's
+// FP64 values plus the accumulated descendant updates, before anything reads
+// it.  Without SAFE a pivot <= 0 is reported as the failing column.
 //
 // potrf_block: the whole workgroup (256 threads) factors diagonal block t in
-// LDS (S: the block, X: its inverse, both NB x DS row-major; *sfail: a shared
-// int) and writes L_tt and the inverse back to W.
+// LDS (S: the block, X: its inverse, both NB x DS row-major; sfail: two shared
+// ints, the first failing column and the breakdown flag) and writes L_tt and
+// the inverse back to W.
 template <typename T, bool SAFE>
 __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W, T *S, T *X, int *sfail,
                                             int *__restrict__ fail, const double *__restrict__ adiag,
-                                            double rho, int *__restrict__ npert) {
+                                            double rho, RedoQ rq) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nb = t.nb;
@@ -696,7 +708,10 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
             X[i * DS + c] = 0;
         }
     }
-    if (tid == 0) *sfail = -1;
+    if (tid == 0) {
+        sfail[0] = -1;
+        sfail[1] = 0;
+    }
     __syncthreads();
 
     // Right-looking over 16-column panels, one panel of look-ahead: after the
@@ -706,8 +721,10 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
     //   L_ii, X_ii = potrf16_wave(S_ii);   L_ip = S_ip X_pp';
     //   S_ij -= L_ip L_jp' (p < j <= i);   X_pj = -X_pp sum_{k=j}^{p-1} L_pk X_kj (j < p).
     if (wave == 0) {
-        const int badj = potrf16_wave<T, SAFE>(S, X, DS, 0, nb, lane, adiag, t.col0, rho, npert);
-        if (lane == 0 && badj < 16 && *sfail < 0) *sfail = badj;
+        bool brk = false;
+        const int badj = potrf16_wave<T, SAFE>(S, X, DS, 0, nb, lane, adiag, t.col0, rho, &brk);
+        if (lane == 0 && badj < 16 && sfail[0] < 0) sfail[0] = badj;
+        if (SAFE && lane == 0 && brk) sfail[1] = 1;
     }
     __syncthreads();
     for (int p = 0; p < np; p++) {
@@ -734,8 +751,10 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                const int badj = potrf16_wave<T, SAFE>(S, X, DS, Q, nb, lane, adiag, t.col0, rho, npert);
-                if (lane == 0 && badj < 16 && *sfail < 0) *sfail = Q + badj;
+                bool brk = false;
+                const int badj = potrf16_wave<T, SAFE>(S, X, DS, Q, nb, lane, adiag, t.col0, rho, &brk);
+                if (lane == 0 && badj < 16 && sfail[0] < 0) sfail[0] = Q + badj;
+                if (SAFE && lane == 0 && brk) sfail[1] = 1;
             }
         } else {
             // (2b) waves 1..3, round robin: panel p's other updates, then block row p of the inverse
@@ -771,7 +790,12 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
         }
         __syncthreads();
     }
-    if (tid == 0 && *sfail >= 0) atomicMin(fail, t.col0 + *sfail);
+    if (SAFE && sfail[1]) {
+        // breakdown: leave the assembled block in W for the FP64 redo
+        if (tid == 0) redo_push(rq, t);
+        return;
+    }
+    if (tid == 0 && sfail[0] >= 0) atomicMin(fail, t.col0 + sfail[0]);
     {
         const int i = tid & (NB - 1), c0 = tid >> 6;
 #pragma unroll
@@ -788,12 +812,11 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
 template <typename T, bool SAFE>
 __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__ tasks,
                                                     T *__restrict__ W, int *__restrict__ fail,
-                                                    const double *__restrict__ adiag, double rho,
-                                                    int *__restrict__ npert) {
+                                                    const double *__restrict__ adiag, double rho, RedoQ rq) {
     __shared__ T S[NB * DS];       // the block, row-major
     __shared__ T X[NB * DS];       // its inverse
-    __shared__ int sfail;
-    potrf_block<T, SAFE>(tasks[blockIdx.x], W, S, X, &sfail, fail, adiag, rho, npert);
+    __shared__ int sfail[2];
+    potrf_block<T, SAFE>(tasks[blockIdx.x], W, S, X, sfail, fail, adiag, rho, rq);
 }
 
 
@@ -803,8 +826,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
 template <typename T, bool SAFE>
 __global__ __launch_bounds__(256) void k_potrf_small(const DiagTask *__restrict__ tasks, int ntask,
                                                      T *__restrict__ W, int *__restrict__ fail,
-                                                     const double *__restrict__ adiag, double rho,
-                                                     int *__restrict__ npert) {
+                                                     const double *__restrict__ adiag, double rho, RedoQ rq) {
     constexpr int SS = 17;
     __shared__ T S[4][16 * SS];
     __shared__ T X[4][16 * SS];
@@ -825,7 +847,13 @@ __global__ __launch_bounds__(256) void k_potrf_small(const DiagTask *__restrict_
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const int badj = potrf16_wave<T, SAFE>(Sw, Xw, SS, 0, nb, lane, adiag, t.col0, rho, npert);
+    bool brk = false;
+    const int badj = potrf16_wave<T, SAFE>(Sw, Xw, SS, 0, nb, lane, adiag, t.col0, rho, &brk);
+    if (SAFE && brk) {
+        // breakdown: leave the assembled block in W for the FP64 redo
+        if (lane == 0) redo_push(rq, t);
+        return;
+    }
     if (lane == 0 && badj < nb) atomicMin(fail, t.col0 + badj);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -843,17 +871,16 @@ __global__ __launch_bounds__(256) void k_potrf_small(const DiagTask *__restrict_
 
 // Diagonal blocks of at most TINY_DIAG columns (the bottom levels of a forest:
 // mostly one to five columns): one lane factors and inverts a whole block, with
-// potrf16_wave's arithmetic: pivot d_jj (raised to rho a_jj when SAFE and
-// below it), a non-positive pivot reported and replaced by 1, r_j =
-// 1/sqrt(d_jj), L_jj = d_jj r_j, L_ij = a_ij r_j, the inverse by forward
-// substitution with r_j.
+// potrf16_wave's arithmetic: pivot d_jj (when SAFE, a pivot at or below
+// rho a_jj sends the block to the FP64 redo queue), a non-positive pivot
+// reported and replaced by 1, r_j = 1/sqrt(d_jj), L_jj = d_jj r_j,
+// L_ij = a_ij r_j, the inverse by forward substitution with r_j.
 constexpr int TINY_DIAG = 6;
 
 template <typename T, bool SAFE>
 __global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__ tasks, int ntask,
                                                     T *__restrict__ W, int *__restrict__ fail,
-                                                    const double *__restrict__ adiag, double rho,
-                                                    int *__restrict__ npert) {
+                                                    const double *__restrict__ adiag, double rho, RedoQ rq) {
     const int ti = blockIdx.x * blockDim.x + threadIdx.x;
     if (ti >= ntask) return;
     const DiagTask t = tasks[ti];
@@ -864,17 +891,13 @@ __global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__
 #pragma unroll
         for (int c = 0; c <= i; c++)
             a[i][c] = i < nb ? W[t.off + i + (long long)c * t.ld] : (i == c ? (T)1 : (T)0);
-    int npj = 0, badj = TINY_DIAG;
+    bool low_any = false;
+    int badj = TINY_DIAG;
 #pragma unroll
     for (int j = 0; j < TINY_DIAG; j++) {
         const bool real = j < nb;
         T djj = a[j][j];
-        if (SAFE) {
-            const T floor_j = real ? (T)(rho * adiag[t.col0 + j]) : (T)0;
-            const bool low = real && djj > (T)0 && djj <= floor_j && floor_j > (T)0;
-            djj = low ? floor_j : djj;
-            npj += low ? 1 : 0;
-        }
+        if (SAFE) low_any = low_any || (real && djj <= (T)(rho * adiag[t.col0 + j]));
         const bool bad = !(djj > (T)0);
         badj = (bad && real && badj == TINY_DIAG) ? j : badj;
         djj = bad ? (T)1 : djj;
@@ -888,7 +911,10 @@ __global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__
 #pragma unroll
             for (int i = c; i < TINY_DIAG; i++) a[i][c] -= a[i][j] * a[c][j];
     }
-    if (SAFE && npj > 0) atomicAdd(npert, npj);
+    if (SAFE && low_any) {
+        redo_push(rq, t);            // breakdown: the FP64 redo refactors the assembled block
+        return;
+    }
     if (badj < nb) atomicMin(fail, t.col0 + badj);
 #pragma unroll
     for (int c = 0; c < TINY_DIAG; c++)
@@ -907,6 +933,114 @@ __global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__
                 if (c <= i) W[t.off + i + (long long)c * t.ld] = a[i][c];
                 W[t.winv + i + (long long)c * nb] = (c <= i) ? x[i][c] : (T)0;
             }
+}
+
+// FP64 redo of the diagonal blocks a low-precision factorization queued on
+// breakdown (a pivot at or below rho * a_jj): launched after every launch
+// that factors diagonal blocks, so a queued block is final before the next
+// launch (its TRSM, or the next step) reads it.  For each queued block:
+//   1. A64 = the block as assembled in W (A's values in T, minus the
+//      descendant and earlier-step updates accumulated in T), in FP64, with
+//      A's own entries of the block swapped for their FP64 values (+ beta on
+//      the diagonal): A64 = W + (A - fl_T(A)).  The block's A entries are
+//      found through the full CSR of A in the original order: column
+//      Perm[col0 + c] holds, among its neighbours, the rows whose permuted
+//      index falls in the block;
+//   2. right-looking Cholesky of A64 in FP64 (a pivot <= 0 is reported as the
+//      failing column, as CHOLMOD reports it);
+//   3. the inverse of L in FP64, by forward substitution;
+//   4. L and the inverse rounded to T into W.
+// Workgroups take queued blocks in turn; the last one to finish clears the
+// queue for the next launch.  With nothing queued (the usual case) each
+// workgroup only reads the counter.  LDS: two NB x (NB + 1) FP64 blocks.
+constexpr int REDO_WG = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_redo64(RedoQ rq, T *__restrict__ W, int *__restrict__ fail,
+                                                      const long long *__restrict__ rowptr,
+                                                      const int *__restrict__ col,
+                                                      const double *__restrict__ vals,
+                                                      const long long *__restrict__ perm,
+                                                      const int *__restrict__ pinv,
+                                                      const double *__restrict__ beta) {
+    constexpr int LD = NB + 1;
+    __shared__ double A[NB * LD];
+    __shared__ double X[NB * LD];
+    __shared__ int sbad;
+    const int tid = threadIdx.x;
+    const int cnt = rq.cnt[0];
+    for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
+        const DiagTask t = rq.list[e];
+        const int nb = t.nb;
+        const double b = *beta;
+        for (int q = tid; q < NB * NB; q += blockDim.x) {
+            const int i = q % NB, c = q / NB;
+            A[i * LD + c] = (i < nb && c <= i) ? (double)W[t.off + i + (long long)c * t.ld] : (i == c ? 1.0 : 0.0);
+        }
+        if (tid == 0) sbad = -1;
+        __syncthreads();
+        // A's own entries of the block in FP64: 4 threads per column, distinct rows each
+        for (int c = tid >> 2; c < nb; c += blockDim.x >> 2) {
+            const long long o = perm[t.col0 + c];
+            for (long long q = rowptr[o] + (tid & 3); q < rowptr[o + 1]; q += 4) {
+                const int i = pinv[col[q]] - t.col0;
+                if (i < c || i >= nb) continue;
+                const double a = vals[q];
+                T a_t = (T)a;                       // what the assembly stored (k_scatter_a, k_add_diag)
+                double a64 = a;
+                if (i == c && b != 0.0) {
+                    a_t += (T)b;
+                    a64 += b;
+                }
+                A[i * LD + c] += a64 - (double)a_t;
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < nb; j++) {
+            double d = A[j * LD + j];
+            if (!(d > 0.0)) {
+                if (tid == 0 && sbad < 0) sbad = j;
+                d = 1.0;
+            }
+            const double r = 1.0 / sqrt(d);
+            __syncthreads();                         // every thread has read the pivot
+            if (tid == 0) A[j * LD + j] = d * r;
+            for (int i = j + 1 + tid; i < nb; i += blockDim.x) A[i * LD + j] *= r;
+            __syncthreads();
+            const int m = nb - j - 1;
+            for (int q = tid; q < m * m; q += blockDim.x) {
+                const int i = j + 1 + q % m, c = j + 1 + q / m;
+                if (c <= i) A[i * LD + c] -= A[i * LD + j] * A[c * LD + j];
+            }
+            __syncthreads();
+        }
+        // X = inv(L): row i of X from rows 0..i-1, thread c forming column c
+        for (int i = 0; i < nb; i++) {
+            const int c = tid;
+            if (c <= i) {
+                double acc = (i == c) ? 1.0 : 0.0;
+                for (int k = c; k < i; k++) acc -= A[i * LD + k] * X[k * LD + c];
+                X[i * LD + c] = acc / A[i * LD + i];
+            }
+            __syncthreads();
+        }
+        for (int q = tid; q < nb * nb; q += blockDim.x) {
+            const int i = q % nb, c = q / nb;
+            if (c <= i) W[t.off + i + (long long)c * t.ld] = (T)A[i * LD + c];
+            W[t.winv + i + (long long)c * nb] = (c <= i) ? (T)X[i * LD + c] : (T)0;
+        }
+        if (tid == 0 && sbad >= 0) atomicMin(fail, t.col0 + sbad);
+        __syncthreads();
+    }
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        if (atomicAdd(rq.cnt + 2, 1) == (int)gridDim.x - 1) {     // the last workgroup: clear the queue
+            rq.cnt[1] += cnt;
+            rq.cnt[0] = 0;
+            rq.cnt[2] = 0;
+        }
+    }
 }
 
 // Panel TRSM of a tiny supernode (K <= TINY_DIAG columns, at most

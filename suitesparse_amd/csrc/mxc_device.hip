@@ -255,7 +255,10 @@ struct mxc_factor {
     Tile *d_tiles = nullptr;
     DiagTask *d_diag = nullptr;
     SweepTask *d_stask = nullptr;
-    int *d_fail = nullptr;            // [0] first failing column, [1] perturbed pivots
+    int *d_fail = nullptr;            // [0] first failing column, [1..3] FP64 redo queue counters
+                                      // (queued, refactored in FP64, finished redo workgroups)
+    int *d_pinv = nullptr;            // inverse permutation (n), for the FP64 redo's A entries
+    DiagTask *d_redo = nullptr;       // FP64 redo queue (one slot per diagonal task of the plan)
     double *d_adiag = nullptr;        // assembled diagonal of A (n)
     long long *d_rowptr = nullptr, *d_src = nullptr;
     int *d_col = nullptr;
@@ -277,7 +280,7 @@ struct mxc_factor {
     std::vector<Launch> fwd_chain, bwd_chain;   // chained sweeps: per level small + chain
     bool values_loaded = false;
     int64_t minor = 0;                     // L->minor of the last factorization
-    int64_t nperturbed = 0;                // pivots raised by the low-precision safeguard
+    int64_t nredo = 0;                     // diagonal blocks refactored in FP64 (last factorization)
     double rho = 0;                        // pivot-ratio threshold (0: exact semantics)
     // FP64 fallback factor of a low-precision factor whose refinement stalls,
     // or whose last factorization met a pivot <= 0 (escalated: the FP64
@@ -642,6 +645,12 @@ static int flatten_upload(mxc_factor *F) {
     if ((rc = dupload(F, &F->d_desc, descs, F->stream))) return rc;
     if ((rc = dupload(F, &F->d_tiles, tiles, F->stream))) return rc;
     if ((rc = dupload(F, &F->d_diag, diags, F->stream))) return rc;
+    if (F->precision != MXC_FP64) {
+        // FP32 factors: room in the FP64 redo queue for every diagonal task
+        ffree(F, F->d_redo);
+        F->d_redo = nullptr;
+        if ((rc = dalloc(F, &F->d_redo, diags.size()))) return rc;
+    }
     F->plan_bytes = F->bytes - before;
     HIPTRY(hipStreamSynchronize(F->stream));
     return MXC_OK;
@@ -1135,7 +1144,12 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     if ((rc = dupload(F, &F->d_src, src, sm))) return rc;
     if ((rc = dalloc(F, &F->d_vals, (size_t)std::max<int64_t>(F->nnz_full, 1)))) return rc;
     if ((rc = dalloc(F, &F->d_Ax, (size_t)std::max<int64_t>(nnz, 1)))) return rc;
-    if ((rc = dalloc(F, &F->d_fail, 2))) return rc;
+    if ((rc = dalloc(F, &F->d_fail, 4))) return rc;
+    {
+        std::vector<int> pinv32(n > 0 ? n : 1, 0);
+        for (int64_t k = 0; k < n; k++) pinv32[k] = (int)Pinv[k];
+        if ((rc = dupload(F, &F->d_pinv, pinv32, sm))) return rc;
+    }
     if ((rc = dalloc(F, &F->d_adiag, (size_t)std::max<int64_t>(n, 1)))) return rc;
     F->Ls_host.assign(S->Ls, S->Ls + S->ssize);
     F->Ap_host.assign(Ap, Ap + n + 1);
@@ -1188,7 +1202,7 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
     void *ptrs[] = {F->W, F->d_Lh, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
                     F->d_diag, F->d_stask, F->d_fail, F->d_adiag, F->d_rowptr, F->d_src, F->d_col,
                     F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_part, F->d_b, F->d_x, F->d_r,
-                    F->d_cnt, F->d_abort, F->d_seg, F->d_segout};
+                    F->d_cnt, F->d_abort, F->d_seg, F->d_segout, F->d_pinv, F->d_redo};
     for (void *p : ptrs) ffree(F, p);
 
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
@@ -1353,7 +1367,7 @@ extern "C" int mxc_factor_info(const mxc_factor *F, int64_t *info, double *dinfo
         info[5] = F->nupd_tiles;
         info[6] = (int64_t)F->bytes;
         info[7] = F->precision;
-        info[8] = F->nperturbed;
+        info[8] = F->nredo;
         info[9] = F->escalated ? 1 : 0;
     }
     if (dinfo) {
@@ -1433,12 +1447,19 @@ static inline unsigned grid_for(long long n, int bs, long long cap = 65536) {
     return (unsigned)g;
 }
 
+// Low-precision factors with the pivot-ratio safeguard on send the diagonal
+// blocks whose pivots break down to the FP64 redo (k_potrf_redo64).
+template <typename T>
+static inline bool redo_safe(const mxc_factor *F) {
+    return std::is_same<T, float>::value && F->rho > 0 && F->d_redo != nullptr;
+}
+
 template <typename T, int MODE, bool PD = false>
 static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = nullptr) {
     if (!st) st = F->stream;
     const bool h16 = std::is_same<T, float>::value && F->split16 && F->d_Lh;
     const double *adiag = F->d_adiag;
-    const PdArgs pda{F->d_diag, F->d_fail, F->rho, F->d_fail + 1};
+    const PdArgs pda{F->d_diag, F->d_fail, redo_safe<T>(F) ? F->rho : 0.0, RedoQ{F->d_fail + 1, F->d_redo}};
     const dim3 g((unsigned)L.count);
     if constexpr (std::is_same<T, float>::value && MODE != MODE_STORE) {
         // FP32 factor: split-FP16 products on the FP16 MFMA, operands from the
@@ -1483,7 +1504,16 @@ static int enqueue_factor(mxc_factor *F) {
     size_t t0 = 0, tt = 0;
     tick(F, 8, &t0);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_fail, INT_MAX);
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_fail + 1, 0);
+    for (int k = 1; k < 4; k++) hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_fail + k, 0);
+    const bool safe = redo_safe<T>(F);
+    const RedoQ rq{F->d_fail + 1, F->d_redo};
+    // after a launch that factors diagonal blocks: refactor in FP64 the blocks it queued
+    auto redo = [&]() {
+        if (safe)
+            hipLaunchKernelGGL(k_potrf_redo64<T>, dim3(REDO_WG), dim3(256), 0, sm, rq, W, F->d_fail,
+                               (const long long *)F->d_rowptr, (const int *)F->d_col, (const double *)F->d_vals,
+                               (const long long *)F->d_perm, (const int *)F->d_pinv, (const double *)F->d_beta);
+    };
     // assembly: L = 0, scatter A, + beta on the diagonal; also refresh the CSR values
     tick(F, 0, &tt);
     hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(F->xsize, 256)), dim3(256), 0, sm, W, (long long)F->xsize, (T)0);
@@ -1518,39 +1548,40 @@ static int enqueue_factor(mxc_factor *F) {
                 F->counts[0]++;
                 break;
             case L_DIAG:
-                if (F->rho > 0)
+                if (safe)
                     hipLaunchKernelGGL((k_potrf_diag<T, true>), dim3((unsigned)L.count), dim3(256), 0,
                                        sm, F->d_diag + L.begin, W, F->d_fail,
-                                       (const double *)F->d_adiag, F->rho, F->d_fail + 1);
+                                       (const double *)F->d_adiag, F->rho, rq);
                 else
                     hipLaunchKernelGGL((k_potrf_diag<T, false>), dim3((unsigned)L.count), dim3(256), 0,
                                        sm, F->d_diag + L.begin, W, F->d_fail,
-                                       (const double *)F->d_adiag, F->rho, F->d_fail + 1);
+                                       (const double *)F->d_adiag, 0.0, rq);
+                redo();
                 F->counts[1]++;
                 break;
             case L_DIAG_SMALL: {
                 const unsigned g = (unsigned)((L.count + 3) / 4);
-                if (F->rho > 0)
+                if (safe)
                     hipLaunchKernelGGL((k_potrf_small<T, true>), dim3(g), dim3(256), 0, sm,
                                        F->d_diag + L.begin, (int)L.count, W, F->d_fail,
-                                       (const double *)F->d_adiag, F->rho, F->d_fail + 1);
+                                       (const double *)F->d_adiag, F->rho, rq);
                 else
                     hipLaunchKernelGGL((k_potrf_small<T, false>), dim3(g), dim3(256), 0, sm,
                                        F->d_diag + L.begin, (int)L.count, W, F->d_fail,
-                                       (const double *)F->d_adiag, F->rho, F->d_fail + 1);
+                                       (const double *)F->d_adiag, 0.0, rq);
+                redo();
                 F->counts[1]++;
                 break;
             }
             case L_DIAG_TINY: {
                 const unsigned g = (unsigned)((L.count + 255) / 256);
-                if (F->rho > 0)
+                if (safe)
                     hipLaunchKernelGGL((k_potrf_tiny<T, true>), dim3(g), dim3(256), 0, sm, F->d_diag + L.begin,
-                                       (int)L.count, W, F->d_fail, (const double *)F->d_adiag, F->rho,
-                                       F->d_fail + 1);
+                                       (int)L.count, W, F->d_fail, (const double *)F->d_adiag, F->rho, rq);
                 else
                     hipLaunchKernelGGL((k_potrf_tiny<T, false>), dim3(g), dim3(256), 0, sm, F->d_diag + L.begin,
-                                       (int)L.count, W, F->d_fail, (const double *)F->d_adiag, F->rho,
-                                       F->d_fail + 1);
+                                       (int)L.count, W, F->d_fail, (const double *)F->d_adiag, 0.0, rq);
+                redo();
                 F->counts[1]++;
                 break;
             }
@@ -1572,6 +1603,7 @@ static int enqueue_factor(mxc_factor *F) {
                 break;
             case L_TRAILD:
                 launch_tile<T, MODE_TRAIL, true>(F, L, W);
+                redo();
                 F->counts[3]++;
                 break;
             case L_TRAIL2:
@@ -1651,8 +1683,8 @@ extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, doub
         return rc;
     }
     HIPTRY(hipEventRecord(F->ev_f1, sm));
-    int flags[2] = {0, 0};
-    HIPTRY(hipMemcpyAsync(flags, F->d_fail, sizeof(int) * 2, hipMemcpyDeviceToHost, sm));
+    int flags[3] = {0, 0, 0};
+    HIPTRY(hipMemcpyAsync(flags, F->d_fail, sizeof(int) * 3, hipMemcpyDeviceToHost, sm));
     HIPTRY(hipStreamSynchronize(sm));
     {
         float t = 0;
@@ -1663,7 +1695,7 @@ extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, doub
         F->nfactor++;
     }
     int fail = flags[0];
-    F->nperturbed = flags[1];
+    F->nredo = flags[2];
     F->values_loaded = true;
     if (fail == 0x7fffffff) {
         F->minor = F->n;
@@ -2471,6 +2503,7 @@ extern "C" int mxc_get_tiles(const mxc_factor *F, int *table) {
 
 extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
     if (!F) return MXC_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);     // graphs are destroyed below
     switch (option) {
         case MXC_OPT_GRAPHS:
             F->use_graphs = value != 0;

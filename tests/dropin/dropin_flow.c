@@ -28,7 +28,8 @@ typedef struct {
     double Lx_copy_n;
     double *Lx;            /* L->x right after the factorization (before any Bset solve) */
     int is_super, status, minor, nsuper;
-    double rcond, resid, resid_ir, bset_err, solve2_err, fl, lnz;
+    double rcond, resid, resid_ir, fl, lnz;
+    double bset_err, solve2_err;   /* max |difference| / max |x| */
     int ordering, is_super_after_bset;
 } side;
 
@@ -92,8 +93,9 @@ static int run(side *s, const char *path, int gpu, int supernodal) {
     cholmod_dense *X2 = NULL, *Y = NULL, *E = NULL;
     for (int t = 0; t < 3; t++) cholmod_l_solve2(CHOLMOD_A, s->L, s->B, NULL, &X2, NULL, &Y, &E, &s->c);
     s->solve2_err = 0;
+    const double xmax = norm_inf_dense(s->X) + (n == 0);
     for (size_t i = 0; i < n; i++)
-        s->solve2_err = fmax(s->solve2_err, fabs(((double *)X2->x)[i] - ((double *)s->X->x)[i]));
+        s->solve2_err = fmax(s->solve2_err, fabs(((double *)X2->x)[i] - ((double *)s->X->x)[i]) / xmax);
 
     /* sparse right-hand sides: x(Xset) for b = e_i * (3.1 i + 0.9) */
     cholmod_dense *B2 = cholmod_l_zeros(n, 1, CHOLMOD_REAL + CHOLMOD_DOUBLE, &s->c), *X3 = NULL, *Xfull = NULL;
@@ -108,9 +110,10 @@ static int run(side *s, const char *path, int gpu, int supernodal) {
         cholmod_l_solve2(CHOLMOD_A, s->L, B2, NULL, &Xfull, NULL, &Y, &E, &s->c);
         cholmod_l_solve2(CHOLMOD_A, s->L, B2, Bset, &X3, &Xset, &Y, &E, &s->c);
         const int64_t xlen = ((int64_t *)Xset->p)[1];
+        const double fmaxv = norm_inf_dense(Xfull) + (n == 0);
         for (int64_t k = 0; k < xlen; k++) {
             const int64_t j = ((int64_t *)Xset->i)[k];
-            s->bset_err = fmax(s->bset_err, fabs(((double *)X3->x)[j] - ((double *)Xfull->x)[j]));
+            s->bset_err = fmax(s->bset_err, fabs(((double *)X3->x)[j] - ((double *)Xfull->x)[j]) / fmaxv);
         }
         ((double *)B2->x)[i] = 0;
     }

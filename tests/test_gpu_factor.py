@@ -117,21 +117,23 @@ def _near_singular_pairs(m, delta, seed=0):
 
 def test_fp32_breakdown_falls_back_to_fp64(gpu):
     """FP32 pivots that break down (true pivot ratio ~1e-9 < rho = 1e-6; in FP32
-    the computed pivots scatter around 0) are raised and counted when positive,
-    and a pivot <= 0 escalates the factorization to FP64, which finds the
-    matrix positive definite (status OK).  Either way refinement through the FP32
-    factor cannot reach 1e-10, so the FP64 factor is used and does."""
+    the computed pivots scatter around 0) send their blocks to the FP64 redo,
+    which factors them accurately but stores L in FP32; a pivot <= 0 escalates
+    the factorization to FP64, which finds the matrix positive definite
+    (status OK).  At cond ~1e9 an FP32-stored factor cannot carry refinement
+    to 1e-10, so the FP64 factor is used and does."""
     A = _near_singular_pairs(200, 1e-9)
     cm = cholmod.Common(precision="fp32")
     L = cholmod.analyze_p(A, np.arange(A.n), cm)
     assert cholmod.factorize(A, L, cm) and cm.status == 0 and L.minor == A.n
     inf = L.info()
     print(inf, flush=True)
-    assert inf["perturbed_pivots"] > 0 or inf["escalated_fp64"] == 1
+    assert inf["fp64_blocks"] > 0 or inf["escalated_fp64"] == 1
     # B = A X_true with X_true = O(1): then ||x|| ~ ||b||, so FP64's backward error
     # (~eps ||A|| ||x|| / ||b||) is far below 1e-10, while b's components along the
     # ~2e-9 eigenvectors (relative size ~1e-9) stall refinement through the
-    # perturbed FP32 factor (contraction 1 - 2e-9/1e-6 per step).  A random B
+    # FP32-stored factor (its rounding alone is ~u32 / 2e-9 of the small
+    # eigenvalues).  A random B
     # instead has ||x|| ~ 1e9 ||b|| and no FP64 solve could reach 1e-10.
     Xt = np.random.default_rng(1).uniform(-1, 1, (A.n, 2))
     B = np.asfortranarray(A.matvec(Xt))
@@ -193,3 +195,55 @@ def test_fp32_indefinite_reports_not_posdef(side, sigma, gpu, ref_lib):
     assert berr <= 1e-13, berr
     assert not Ld[:, m:].any()      # everything from the failing column on is zero
     assert cholmod.rcond(L, cm) == 0.0
+
+
+def _ill_blocks(sizes, kappa, seed):
+    """Block-diagonal SPD matrix: dense blocks Q diag(logspace(0, -log10 kappa)) Q'
+    of the given sizes (Q random orthogonal), lower triangle, CSC."""
+    import scipy.linalg
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    blocks = []
+    for m in sizes:
+        Q = scipy.linalg.qr(rng.standard_normal((m, m)))[0]
+        lam = np.logspace(0.0, -np.log10(kappa), m)
+        blocks.append(Q @ np.diag(lam) @ Q.T)
+    T = sp.tril(sp.block_diag(blocks, format="csc")).tocsc()
+    T.sort_indices()
+    return problems.SymMatrix(n=T.shape[0], p=T.indptr.astype(np.int64), i=T.indices.astype(np.int64),
+                              x=T.data.astype(np.float64), name=f"ill{len(sizes)}")
+
+
+@pytest.mark.parametrize("case", ["dense64", "dense200", "small12", "tiny5"])
+def test_fp32_breakdown_refactors_block_in_fp64(gpu, case):
+    """A dense SPD block with cond 3e6: FP32 POTRF loses the small pivots
+    (relative error ~ n u32 cond >> 1), so a pivot ratio falls below rho =
+    1e-6 and the block is refactored in FP64 on the device (from A's FP64
+    values), then stored in FP32; the factorization continues in FP32 with no
+    whole-matrix FP64 factor, and refinement through it reaches 1e-10
+    (its contraction ~ u32 cond < 1).  Covered: a 64-column block
+    (k_potrf_diag), a 200-column supernode (steps 1..3 factored as tails of
+    the trailing launches), blocks of 12 columns (k_potrf_small) and of 5
+    (k_potrf_tiny), each against its own FP64 factorization."""
+    sizes = {"dense64": [64], "dense200": [200], "small12": [12] * 24, "tiny5": [5] * 40}[case]
+    A = _ill_blocks(sizes, 3e6, seed=len(sizes))
+    cm = cholmod.Common(precision="fp32")
+    L = cholmod.analyze_p(A, np.arange(A.n), cm)
+    assert cholmod.factorize(A, L, cm) and cm.status == 0 and L.minor == A.n
+    inf = L.info()
+    print(case, inf, flush=True)
+    assert inf["fp64_blocks"] > 0 and inf["escalated_fp64"] == 0
+    Xt = np.random.default_rng(3).uniform(-1, 1, (A.n, 2))
+    B = np.asfortranarray(A.matvec(Xt))
+    X, info = cholmod.solve_refine(L, B, cm, tol=1e-10, maxiter=40)
+    print(info, flush=True)
+    assert not info["fp64_fallback"]
+    assert info["relres"] <= 1e-10
+    R = B - A.matvec(X)
+    assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) <= 1e-10
+    # the FP32 factor is the FP64 one rounded: compare with the FP64 factor
+    cm64 = cholmod.Common(precision="fp64")
+    L64 = cholmod.analyze_p(A, np.arange(A.n), cm64)
+    assert cholmod.factorize(A, L64, cm64) and cm64.status == 0
+    err = np.abs(L.x - L64.x).max() / np.abs(L64.x).max()
+    print(f"  FP32 factor vs FP64 factor: {err:.3e}", flush=True)
