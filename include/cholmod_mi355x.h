@@ -259,6 +259,17 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * wait is bounded: a sweep whose wait gives up returns MXC_GPU_PROBLEM.
  * 0: one launch per 64-column step of a level. */
 #define MXC_OPT_SOLVE_CHAIN 5
+/* MXC_OPT_DETERMINISTIC (default 0): bitwise-repeatable factorization.  The
+ * descendant updates of a level are split into rounds, round r holding the
+ * r-th (d, s) pair (by descendant, ascending) of every target supernode s, one
+ * launch after the other: no two tiles of a launch add into the same entry of
+ * L, so every entry receives its updates in a fixed order, d ascending (the
+ * order in which the reference's left-looking loop applies them,
+ * t_cholmod_super_numeric_worker.c:896-930).  In-supernode steps already have
+ * one writer per entry.  Costs one launch per round: as many rounds per level
+ * as the most descendants of any of its supernodes.  The solve sweeps are not
+ * affected (the forward sweep adds ancestor contributions atomically). */
+#define MXC_OPT_DETERMINISTIC 6
 int mxc_set_option(mxc_factor *F, int option, int64_t value);
 
 /* As mxc_solve_refine for a block-diagonal batch of independent systems: rows

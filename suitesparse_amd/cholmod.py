@@ -203,6 +203,11 @@ class Factor:
         launch per 64-column step."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 5, int(bool(on))))
 
+    def set_deterministic(self, on: bool):
+        """Bitwise-repeatable factorization: descendant updates in rounds, one pair
+        per target supernode per launch, applied in descendant order (default off)."""
+        check("mxc_set_option", lib().mxc_set_option(self._num, 6, int(bool(on))))
+
     def set_graphs(self, on: bool):
         """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 1, int(bool(on))))

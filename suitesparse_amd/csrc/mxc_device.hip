@@ -231,6 +231,7 @@ struct mxc_factor {
     bool chain = true;                   // solves: one chained launch per level
     bool diag_tails = true;              // trailing launches factor the next diagonal blocks
     bool merge_t2 = true;                // FP64: fold second-stream updates into later trailing launches
+    bool deterministic = false;          // descendant updates in rounds: one pair per target per launch
     size_t bytes = 0;
     mxc_pool *pool = nullptr;            // device memory comes from this pool when set
     // device time of the last factorization / solve (event pairs on the factor's
@@ -494,7 +495,23 @@ static int flatten_upload(mxc_factor *F) {
     };
     for (size_t l = 0; l < F->blk_upd.size(); l++) {
         bool level_forked = false;
-        nupd += emit(L_UPD, F->blk_upd[l], true);
+        if (!F->deterministic) {
+            nupd += emit(L_UPD, F->blk_upd[l], true);
+        } else {
+            // Rounds: round r holds the r-th pair (by descendant, ascending) of every
+            // target supernode of the level, so no two tiles of a launch add into the
+            // same entry of L and the sum into each entry has a fixed order: d
+            // ascending, one launch after the other.  (Within one pair the relative
+            // map sends distinct C entries to distinct entries of the target.)
+            std::map<long long, int> rank;          // target (its L offset) -> pairs seen so far
+            std::vector<std::vector<UpdDesc>> rounds;
+            for (const UpdDesc &u : F->blk_upd[l]) {
+                const int r = rank[u.c_off]++;
+                if ((int)rounds.size() <= r) rounds.resize(r + 1);
+                rounds[r].push_back(u);
+            }
+            for (const auto &round : rounds) nupd += emit(L_UPD, round, true);
+        }
         std::vector<int64_t> fused_s;              // supernodes whose diagonal block of this step is
                                                    // factored by the previous step's trailing launch
         const size_t nst = F->blk_steps[l].size();
@@ -2525,6 +2542,10 @@ extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
             F->chain = value != 0;
             drop_sweep_graphs(F);
             return MXC_OK;
+        case MXC_OPT_DETERMINISTIC:
+            if (F->deterministic == (value != 0)) return MXC_OK;
+            F->deterministic = value != 0;
+            return flatten_upload(F);           // new launch list (drops the factor graph)
         default:
             return MXC_INVALID;
     }

@@ -157,3 +157,38 @@ def test_rcond_matches_reference_definition(gpu, ref_lib):
     L = cholmod.analyze_p(A2, sym["Perm"], cm)
     assert cholmod.factorize(A2, L, cm) and cm.status == cholmod.CHOLMOD_NOT_POSDEF
     assert cholmod.rcond(L, cm) == 0.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_deterministic_factor_is_bitwise_repeatable(gpu, ref_lib, precision):
+    """MXC_OPT_DETERMINISTIC: descendant updates in rounds (one pair per target
+    supernode per launch, in descendant order), so every entry of L receives
+    its updates in a fixed order.  Factorizations repeated from the captured
+    graph and eagerly give the same L bit for bit, and it is the reference's
+    factor to the usual tolerance."""
+    A = problems.grid3d_lognormal(22, np.random.default_rng(9))
+    R = ref_lib.RefFactor(A, ref_lib.AMD)
+    sym = R.symbolic()
+    assert R.factorize() == 0
+    Lref = R.Lx()
+    cm = cholmod.Common(precision=precision)
+    L = cholmod.analyze_p(A, sym["Perm"], cm)
+    L._numeric(cm)
+    L.set_deterministic(True)
+    xs = []
+    for graphs in (True, True, True, False):
+        L.set_graphs(graphs)
+        assert cholmod.factorize(A, L, cm) and cm.status == 0
+        xs.append(L.x)
+    for x in xs[1:]:
+        assert np.array_equal(x, xs[0])
+    err = np.abs(xs[0] - Lref).max() / np.abs(Lref).max()
+    print(f"{precision}: deterministic factor vs reference {err:.3e}", flush=True)
+    assert err <= (1e-12 if precision == "fp64" else 1e-4)
+    # the default (atomic) assembly gives the same factor to rounding
+    L.set_deterministic(False)
+    L.set_graphs(True)
+    assert cholmod.factorize(A, L, cm) and cm.status == 0
+    d = np.abs(L.x - xs[0]).max() / np.abs(xs[0]).max()
+    print(f"  atomic vs deterministic assembly: {d:.3e}", flush=True)
+    assert d <= (1e-13 if precision == "fp64" else 1e-5)
