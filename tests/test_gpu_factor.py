@@ -165,36 +165,41 @@ def test_fp32_without_redo_falls_back_to_fp64(gpu):
     assert info["fp64_fallback"] and info["relres"] <= 1e-10
 
 
-def _ill_blocks(sizes, kappa, seed):
-    """Block-diagonal SPD matrix: dense blocks Q diag(logspace(0, -log10 kappa)) Q'
-    of the given sizes (Q random orthogonal), lower triangle, CSC."""
-    import scipy.linalg
+def _pair_blocks(sizes, delta, seed):
+    """Block-diagonal SPD matrix of dense blocks of the given sizes, each built
+    from near-singular 2x2 pairs [[1, 1-delta], [1-delta, 1]] (an odd size ends
+    with a 1 on the diagonal) plus a dense symmetric coupling of norm ~1e-3 delta
+    inside the block, symmetrically scaled: true pivot ratios ~delta, and in
+    FP32 1 - delta rounds to 1, so FP32 pivots carry no information."""
     import scipy.sparse as sp
     rng = np.random.default_rng(seed)
     blocks = []
     for m in sizes:
-        Q = scipy.linalg.qr(rng.standard_normal((m, m)))[0]
-        lam = np.logspace(0.0, -np.log10(kappa), m)
-        blocks.append(Q @ np.diag(lam) @ Q.T)
+        B = np.eye(m)
+        for k in range(0, m - 1, 2):
+            B[k + 1, k] = B[k, k + 1] = 1.0 - delta
+        E = rng.standard_normal((m, m))
+        B += 1e-3 * delta / m * (E + E.T)
+        d = np.exp(rng.uniform(-1, 1, m))
+        blocks.append(d[:, None] * B * d[None, :])
     T = sp.tril(sp.block_diag(blocks, format="csc")).tocsc()
     T.sort_indices()
     return problems.SymMatrix(n=T.shape[0], p=T.indptr.astype(np.int64), i=T.indices.astype(np.int64),
-                              x=T.data.astype(np.float64), name=f"ill{len(sizes)}")
+                              x=T.data.astype(np.float64), name=f"pairs{len(sizes)}")
 
 
 @pytest.mark.parametrize("case", ["dense64", "dense200", "small12", "tiny5"])
 def test_fp32_breakdown_refactors_block_in_fp64(gpu, case):
-    """A dense SPD block with cond 3e6: FP32 POTRF loses the small pivots
-    (relative error ~ n u32 cond >> 1), so a pivot ratio falls below rho =
-    1e-6 and the block is refactored in FP64 on the device (from A's FP64
-    values), then stored in FP32; the factorization continues in FP32 with no
-    whole-matrix FP64 factor, and refinement through it reaches 1e-10
-    (its contraction ~ u32 cond < 1).  Covered: a 64-column block
-    (k_potrf_diag), a 200-column supernode (steps 1..3 factored as tails of
-    the trailing launches), blocks of 12 columns (k_potrf_small) and of 5
-    (k_potrf_tiny), each against its own FP64 factorization."""
+    """Dense blocks of near-singular pairs (pivot ratios ~1e-9 < rho): each
+    breaking diagonal block is refactored in FP64 on the device and stored in
+    FP32, the factorization continues in FP32 with no whole-matrix FP64 factor,
+    and refinement through it reaches 1e-10.  Covered: a 64-column block
+    (k_potrf_diag), a 200-column supernode (steps 1..3 factored as tails of the
+    trailing launches), blocks of 12 columns (k_potrf_small) and of 5
+    (k_potrf_tiny).  Against the FP64 factorization of the same matrix, the
+    FP32 factor agrees to FP32 rounding of L's entries."""
     sizes = {"dense64": [64], "dense200": [200], "small12": [12] * 24, "tiny5": [5] * 40}[case]
-    A = _ill_blocks(sizes, 3e6, seed=len(sizes))
+    A = _pair_blocks(sizes, 1e-9, seed=len(sizes))
     cm = cholmod.Common(precision="fp32")
     L = cholmod.analyze_p(A, np.arange(A.n), cm)
     assert cholmod.factorize(A, L, cm) and cm.status == 0 and L.minor == A.n
@@ -203,15 +208,15 @@ def test_fp32_breakdown_refactors_block_in_fp64(gpu, case):
     assert inf["fp64_blocks"] > 0 and inf["escalated_fp64"] == 0
     Xt = np.random.default_rng(3).uniform(-1, 1, (A.n, 2))
     B = np.asfortranarray(A.matvec(Xt))
-    X, info = cholmod.solve_refine(L, B, cm, tol=1e-10, maxiter=40)
+    X, info = cholmod.solve_refine(L, B, cm, tol=1e-10, maxiter=10)
     print(info, flush=True)
     assert not info["fp64_fallback"]
     assert info["relres"] <= 1e-10
     R = B - A.matvec(X)
     assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) <= 1e-10
-    # the FP32 factor is the FP64 one rounded: compare with the FP64 factor
     cm64 = cholmod.Common(precision="fp64")
     L64 = cholmod.analyze_p(A, np.arange(A.n), cm64)
     assert cholmod.factorize(A, L64, cm64) and cm64.status == 0
     err = np.abs(L.x - L64.x).max() / np.abs(L64.x).max()
     print(f"  FP32 factor vs FP64 factor: {err:.3e}", flush=True)
+    assert err <= 1e-5
