@@ -383,6 +383,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched workload (4,096 2D systems, FP64 and FP32+IR)")
     ap.add_argument("--only-batch", action="store_true", help="run only the batched workload")
+    ap.add_argument("--only-serving", action="store_true", help="run only the serving stream (--serving N)")
     ap.add_argument("--serving", type=int, default=0, metavar="N",
                     help="also run a serving stream of N requests (FP32+IR)")
     ap.add_argument("--serving-rate", type=float, default=100.0, help="Poisson arrival rate (1/s)")
@@ -413,6 +414,11 @@ def main():
     if args.only_batch:
         out = {"metric": "batched factor+solve (4,096 2D systems)", "value": None}
         batch_section(args, out)
+        print(json.dumps(out))
+        return
+    if args.only_serving:
+        out = {"metric": "serving: p50/p99 latency and solves/s (FP32+IR to 1e-10)", "value": None}
+        serving_section(args, out)
         print(json.dumps(out))
         return
     if args.only_64:
