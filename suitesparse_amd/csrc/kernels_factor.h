@@ -152,24 +152,120 @@ __device__ __forceinline__ unsigned split_pack(float v, int e) {
            ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
 }
 
-// Queue of the diagonal blocks of a low-precision factor whose pivot ratio
-// signalled breakdown, for k_potrf_redo64 (which refactors them in FP64):
-// counters [0] blocks queued since the last redo launch, [1] blocks
-// refactored in FP64 in this factorization, [2] redo workgroups finished.
-struct RedoQ {
+// FP64 redo of the diagonal blocks of a low-precision factor whose pivot ratio
+// signals breakdown (a pivot at or below rho * a_jj).  cnt: [0] blocks queued
+// since the last k_potrf_redo64 launch, [1] blocks refactored in FP64 in this
+// factorization, [2] redo workgroups finished (queue clearing).  The rest
+// finds A's own entries of a block: the full CSR of A in the original order,
+// its values (refreshed every factorization), Perm, the inverse permutation
+// and beta.
+struct Redo {
     int *cnt;
     DiagTask *list;
+    const long long *rowptr;
+    const int *col;
+    const double *vals;
+    const long long *perm;
+    const int *pinv;
+    const double *beta;
 };
 
-__device__ __forceinline__ void redo_push(const RedoQ &rq, const DiagTask &t) {
-    const int k = atomicAdd(rq.cnt, 1);
-    rq.list[k] = t;
+__device__ __forceinline__ void redo_push(const Redo &rd, const DiagTask &t) {
+    const int k = atomicAdd(rd.cnt, 1);
+    rd.list[k] = t;
+}
+
+// Refactor diagonal block t in FP64 with the nt threads tid < nt of the
+// caller (sync() is their barrier), in place of its low-precision result:
+//   1. A64 = the block as assembled in W (A's values in T minus the descendant
+//      and earlier-step updates accumulated in T), in FP64, with A's own
+//      entries swapped for their FP64 values (+ beta on the diagonal):
+//      A64 = W + (A - fl_T(A)).  Column Perm[col0 + c] of A's CSR holds, among
+//      its neighbours, the rows whose permuted index falls in the block;
+//   2. right-looking Cholesky of A64 in FP64 (a pivot <= 0 is reported as the
+//      failing column, as CHOLMOD reports it), L written to W in T;
+//   3. inv(L) in place, last column first (column j from the columns right
+//      of it: X_ij = -X_jj sum_{k=j+1..i} X_ik L_kj), written to W in T.
+// A: NB x (NB + 1) doubles, tmp: NB doubles, sbad: one int (all in LDS).
+template <typename T, typename Sync>
+__device__ __forceinline__ void redo_block64(const DiagTask &t, T *__restrict__ W, double *A, double *tmp,
+                                             int *sbad, int tid, int nt, const Redo &rd,
+                                             int *__restrict__ fail, Sync sync) {
+    constexpr int LD = NB + 1;
+    const int nb = t.nb;
+    for (int q = tid; q < nb * nb; q += nt) {
+        const int i = q % nb, c = q / nb;
+        A[i * LD + c] = (c <= i) ? (double)W[t.off + i + (long long)c * t.ld] : 0.0;
+    }
+    if (tid == 0) *sbad = -1;
+    sync();
+    const double b = *rd.beta;
+    const int per = nt >= 4 * nb ? 4 : 1;           // threads per column (distinct rows each)
+    for (int c = tid / per; c < nb; c += nt / per) {
+        const long long o = rd.perm[t.col0 + c];
+        for (long long q = rd.rowptr[o] + (tid % per); q < rd.rowptr[o + 1]; q += per) {
+            const int i = rd.pinv[rd.col[q]] - t.col0;
+            if (i < c || i >= nb) continue;
+            const double a = rd.vals[q];
+            T a_t = (T)a;                           // what the assembly stored (k_scatter_a, k_add_diag)
+            double a64 = a;
+            if (i == c && b != 0.0) {
+                a_t += (T)b;
+                a64 += b;
+            }
+            A[i * LD + c] += a64 - (double)a_t;
+        }
+    }
+    sync();
+    for (int j = 0; j < nb; j++) {
+        double d = A[j * LD + j];
+        if (!(d > 0.0)) {
+            if (tid == 0 && *sbad < 0) *sbad = j;
+            d = 1.0;
+        }
+        const double r = 1.0 / sqrt(d);
+        sync();                                     // every thread has read the pivot
+        if (tid == 0) A[j * LD + j] = d * r;
+        for (int i = j + 1 + tid; i < nb; i += nt) A[i * LD + j] *= r;
+        sync();
+        const int m = nb - j - 1;
+        for (int q = tid; q < m * m; q += nt) {
+            const int i = j + 1 + q % m, c = j + 1 + q / m;
+            if (c <= i) A[i * LD + c] -= A[i * LD + j] * A[c * LD + j];
+        }
+        sync();
+    }
+    for (int q = tid; q < nb * nb; q += nt) {
+        const int i = q % nb, c = q / nb;
+        if (c <= i) W[t.off + i + (long long)c * t.ld] = (T)A[i * LD + c];
+    }
+    sync();
+    for (int j = nb - 1; j >= 0; j--) {
+        if (tid == 0) A[j * LD + j] = 1.0 / A[j * LD + j];
+        for (int i = j + 1 + tid; i < nb; i += nt) tmp[i] = A[i * LD + j];
+        sync();
+        const double xjj = -A[j * LD + j];
+        for (int i = j + 1 + tid; i < nb; i += nt) {
+            double acc = 0.0;
+            for (int k = j + 1; k <= i; k++) acc += A[i * LD + k] * tmp[k];
+            A[i * LD + j] = xjj * acc;
+        }
+        sync();
+    }
+    for (int q = tid; q < nb * nb; q += nt) {
+        const int i = q % nb, c = q / nb;
+        W[t.winv + i + (long long)c * nb] = (c <= i) ? (T)A[i * LD + c] : (T)0;
+    }
+    if (tid == 0) {
+        if (*sbad >= 0) atomicMin(fail, t.col0 + *sbad);
+        atomicAdd(rd.cnt + 1, 1);
+    }
 }
 
 template <typename T, bool SAFE>
 __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W, T *S, T *X, int *sfail,
                                             int *__restrict__ fail, const double *__restrict__ adiag,
-                                            double rho, RedoQ rq);
+                                            double rho, const Redo &rd, double *tmp);
 
 // Arguments of the diagonal-block tail of a trailing launch (PD): after its
 // epilogue, the workgroup of tile (0, 0) of a block with pd >= 0 has the final
@@ -178,7 +274,7 @@ struct PdArgs {
     const DiagTask *diags;
     int *fail;
     double rho;
-    RedoQ rq;
+    Redo rd;
 };
 
 // The tail itself: S and X of potrf_block carved from the tile's LDS buffer.
@@ -188,10 +284,12 @@ __device__ __forceinline__ void pd_tail(const PdArgs &pda, int pd, T *__restrict
     __shared__ int sfail[2];
     __syncthreads();                        // the tile's stores and LDS reads are done
     T *S = reinterpret_cast<T *>(smem);
-    if (pda.rho > 0)
-        potrf_block<T, true>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rq);
-    else
-        potrf_block<T, false>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rq);
+    if (pda.rho > 0) {
+        __shared__ double tmp[NB];
+        potrf_block<T, true>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rd, tmp);
+    } else {
+        potrf_block<T, false>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rd, nullptr);
+    }
 }
 
 template <typename T, int MODE, int TS, bool PD = false>
@@ -675,19 +773,20 @@ __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, i
 
 // SAFE (rho > 0, low-precision factor): a pivot d_jj <= rho * a_jj (a_jj the
 // assembled diagonal of A) signals breakdown of the block.  Its low-precision
-// result is dropped: the block stays as assembled in W and goes to the redo
-// queue, and the next launch (k_potrf_redo64) refactors it in FP64 from A's
-// FP64 values plus the accumulated descendant updates, before anything reads
-// it.  Without SAFE a pivot <= 0 is reported as the failing column.
+// result is dropped and the same workgroup refactors the block in FP64 in
+// place (redo_block64, in the LDS of S and X, which hold NB x (NB + 1)
+// doubles for T = float), before anything reads it.  Without SAFE a pivot
+// <= 0 is reported as the failing column.
 //
 // potrf_block: the whole workgroup (256 threads) factors diagonal block t in
-// LDS (S: the block, X: its inverse, both NB x DS row-major; sfail: two shared
-// ints, the first failing column and the breakdown flag) and writes L_tt and
-// the inverse back to W.
+// LDS (S: the block, X: its inverse, both NB x DS row-major and contiguous;
+// sfail: two shared ints, the first failing column and the breakdown flag;
+// tmp: NB shared doubles for the redo) and writes L_tt and the inverse back
+// to W.
 template <typename T, bool SAFE>
 __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W, T *S, T *X, int *sfail,
                                             int *__restrict__ fail, const double *__restrict__ adiag,
-                                            double rho, RedoQ rq) {
+                                            double rho, const Redo &rd, double *tmp) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nb = t.nb;
@@ -791,8 +890,11 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
         __syncthreads();
     }
     if (SAFE && sfail[1]) {
-        // breakdown: leave the assembled block in W for the FP64 redo
-        if (tid == 0) redo_push(rq, t);
+        // breakdown: the block again from W, in FP64, in the LDS of S and X
+        static_assert(!SAFE || sizeof(T) * 2 * NB * DS >= sizeof(double) * NB * (NB + 1),
+                      "the FP64 redo needs the LDS of S and X");
+        redo_block64<T>(t, W, reinterpret_cast<double *>(S), tmp, sfail, tid, 256, rd, fail,
+                        [] { __syncthreads(); });
         return;
     }
     if (tid == 0 && sfail[0] >= 0) atomicMin(fail, t.col0 + sfail[0]);
@@ -812,11 +914,11 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
 template <typename T, bool SAFE>
 __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__ tasks,
                                                     T *__restrict__ W, int *__restrict__ fail,
-                                                    const double *__restrict__ adiag, double rho, RedoQ rq) {
-    __shared__ T S[NB * DS];       // the block, row-major
-    __shared__ T X[NB * DS];       // its inverse
+                                                    const double *__restrict__ adiag, double rho, Redo rd) {
+    __shared__ __attribute__((aligned(16))) T SX[2 * NB * DS];   // the block and its inverse, row-major
     __shared__ int sfail[2];
-    potrf_block<T, SAFE>(tasks[blockIdx.x], W, S, X, sfail, fail, adiag, rho, rq);
+    __shared__ double tmp[SAFE ? NB : 1];
+    potrf_block<T, SAFE>(tasks[blockIdx.x], W, SX, SX + NB * DS, sfail, fail, adiag, rho, rd, tmp);
 }
 
 
@@ -826,7 +928,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const DiagTask *__restrict__
 template <typename T, bool SAFE>
 __global__ __launch_bounds__(256) void k_potrf_small(const DiagTask *__restrict__ tasks, int ntask,
                                                      T *__restrict__ W, int *__restrict__ fail,
-                                                     const double *__restrict__ adiag, double rho, RedoQ rq) {
+                                                     const double *__restrict__ adiag, double rho, Redo rd) {
     constexpr int SS = 17;
     __shared__ T S[4][16 * SS];
     __shared__ T X[4][16 * SS];
@@ -851,7 +953,7 @@ __global__ __launch_bounds__(256) void k_potrf_small(const DiagTask *__restrict_
     const int badj = potrf16_wave<T, SAFE>(Sw, Xw, SS, 0, nb, lane, adiag, t.col0, rho, &brk);
     if (SAFE && brk) {
         // breakdown: leave the assembled block in W for the FP64 redo
-        if (lane == 0) redo_push(rq, t);
+        if (lane == 0) redo_push(rd, t);
         return;
     }
     if (lane == 0 && badj < nb) atomicMin(fail, t.col0 + badj);
@@ -880,7 +982,7 @@ constexpr int TINY_DIAG = 6;
 template <typename T, bool SAFE>
 __global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__ tasks, int ntask,
                                                     T *__restrict__ W, int *__restrict__ fail,
-                                                    const double *__restrict__ adiag, double rho, RedoQ rq) {
+                                                    const double *__restrict__ adiag, double rho, Redo rd) {
     const int ti = blockIdx.x * blockDim.x + threadIdx.x;
     if (ti >= ntask) return;
     const DiagTask t = tasks[ti];
@@ -912,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__
             for (int i = c; i < TINY_DIAG; i++) a[i][c] -= a[i][j] * a[c][j];
     }
     if (SAFE && low_any) {
-        redo_push(rq, t);            // breakdown: the FP64 redo refactors the assembled block
+        redo_push(rd, t);            // breakdown: the FP64 redo refactors the assembled block
         return;
     }
     if (badj < nb) atomicMin(fail, t.col0 + badj);
@@ -935,110 +1037,31 @@ __global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__
             }
 }
 
-// FP64 redo of the diagonal blocks a low-precision factorization queued on
-// breakdown (a pivot at or below rho * a_jj): launched after every launch
-// that factors diagonal blocks, so a queued block is final before the next
-// launch (its TRSM, or the next step) reads it.  For each queued block:
-//   1. A64 = the block as assembled in W (A's values in T, minus the
-//      descendant and earlier-step updates accumulated in T), in FP64, with
-//      A's own entries of the block swapped for their FP64 values (+ beta on
-//      the diagonal): A64 = W + (A - fl_T(A)).  The block's A entries are
-//      found through the full CSR of A in the original order: column
-//      Perm[col0 + c] holds, among its neighbours, the rows whose permuted
-//      index falls in the block;
-//   2. right-looking Cholesky of A64 in FP64 (a pivot <= 0 is reported as the
-//      failing column, as CHOLMOD reports it);
-//   3. the inverse of L in FP64, by forward substitution;
-//   4. L and the inverse rounded to T into W.
-// Workgroups take queued blocks in turn; the last one to finish clears the
-// queue for the next launch.  With nothing queued (the usual case) each
-// workgroup only reads the counter.  LDS: two NB x (NB + 1) FP64 blocks.
-constexpr int REDO_WG = 16;
+// The FP64 redo of the blocks that the one-wave and one-lane POTRF kernels
+// (k_potrf_small, k_potrf_tiny) queued on breakdown: launched after each such
+// launch, so a queued block is final before the next launch reads it.  A
+// workgroup takes queued blocks in turn (redo_block64); the last one to
+// finish clears the queue for the next launch.  With nothing queued each
+// workgroup only reads the counter.
+constexpr int REDO_WG = 4;
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_redo64(RedoQ rq, T *__restrict__ W, int *__restrict__ fail,
-                                                      const long long *__restrict__ rowptr,
-                                                      const int *__restrict__ col,
-                                                      const double *__restrict__ vals,
-                                                      const long long *__restrict__ perm,
-                                                      const int *__restrict__ pinv,
-                                                      const double *__restrict__ beta) {
-    constexpr int LD = NB + 1;
-    __shared__ double A[NB * LD];
-    __shared__ double X[NB * LD];
+__global__ __launch_bounds__(256) void k_potrf_redo64(Redo rd, T *__restrict__ W, int *__restrict__ fail) {
+    __shared__ double A[NB * (NB + 1)];
+    __shared__ double tmp[NB];
     __shared__ int sbad;
     const int tid = threadIdx.x;
-    const int cnt = rq.cnt[0];
+    const int cnt = rd.cnt[0];
     for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
-        const DiagTask t = rq.list[e];
-        const int nb = t.nb;
-        const double b = *beta;
-        for (int q = tid; q < NB * NB; q += blockDim.x) {
-            const int i = q % NB, c = q / NB;
-            A[i * LD + c] = (i < nb && c <= i) ? (double)W[t.off + i + (long long)c * t.ld] : (i == c ? 1.0 : 0.0);
-        }
-        if (tid == 0) sbad = -1;
-        __syncthreads();
-        // A's own entries of the block in FP64: 4 threads per column, distinct rows each
-        for (int c = tid >> 2; c < nb; c += blockDim.x >> 2) {
-            const long long o = perm[t.col0 + c];
-            for (long long q = rowptr[o] + (tid & 3); q < rowptr[o + 1]; q += 4) {
-                const int i = pinv[col[q]] - t.col0;
-                if (i < c || i >= nb) continue;
-                const double a = vals[q];
-                T a_t = (T)a;                       // what the assembly stored (k_scatter_a, k_add_diag)
-                double a64 = a;
-                if (i == c && b != 0.0) {
-                    a_t += (T)b;
-                    a64 += b;
-                }
-                A[i * LD + c] += a64 - (double)a_t;
-            }
-        }
-        __syncthreads();
-        for (int j = 0; j < nb; j++) {
-            double d = A[j * LD + j];
-            if (!(d > 0.0)) {
-                if (tid == 0 && sbad < 0) sbad = j;
-                d = 1.0;
-            }
-            const double r = 1.0 / sqrt(d);
-            __syncthreads();                         // every thread has read the pivot
-            if (tid == 0) A[j * LD + j] = d * r;
-            for (int i = j + 1 + tid; i < nb; i += blockDim.x) A[i * LD + j] *= r;
-            __syncthreads();
-            const int m = nb - j - 1;
-            for (int q = tid; q < m * m; q += blockDim.x) {
-                const int i = j + 1 + q % m, c = j + 1 + q / m;
-                if (c <= i) A[i * LD + c] -= A[i * LD + j] * A[c * LD + j];
-            }
-            __syncthreads();
-        }
-        // X = inv(L): row i of X from rows 0..i-1, thread c forming column c
-        for (int i = 0; i < nb; i++) {
-            const int c = tid;
-            if (c <= i) {
-                double acc = (i == c) ? 1.0 : 0.0;
-                for (int k = c; k < i; k++) acc -= A[i * LD + k] * X[k * LD + c];
-                X[i * LD + c] = acc / A[i * LD + i];
-            }
-            __syncthreads();
-        }
-        for (int q = tid; q < nb * nb; q += blockDim.x) {
-            const int i = q % nb, c = q / nb;
-            if (c <= i) W[t.off + i + (long long)c * t.ld] = (T)A[i * LD + c];
-            W[t.winv + i + (long long)c * nb] = (c <= i) ? (T)X[i * LD + c] : (T)0;
-        }
-        if (tid == 0 && sbad >= 0) atomicMin(fail, t.col0 + sbad);
+        redo_block64<T>(rd.list[e], W, A, tmp, &sbad, tid, (int)blockDim.x, rd, fail, [] { __syncthreads(); });
         __syncthreads();
     }
     __syncthreads();
     if (tid == 0) {
         __threadfence();
-        if (atomicAdd(rq.cnt + 2, 1) == (int)gridDim.x - 1) {     // the last workgroup: clear the queue
-            rq.cnt[1] += cnt;
-            rq.cnt[0] = 0;
-            rq.cnt[2] = 0;
+        if (atomicAdd(rd.cnt + 2, 1) == (int)gridDim.x - 1) {     // the last workgroup: clear the queue
+            rd.cnt[0] = 0;
+            rd.cnt[2] = 0;
         }
     }
 }

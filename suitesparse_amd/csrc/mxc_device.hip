@@ -1464,11 +1464,17 @@ static inline unsigned grid_for(long long n, int bs, long long cap = 65536) {
     return (unsigned)g;
 }
 
-// Low-precision factors with the pivot-ratio safeguard on send the diagonal
-// blocks whose pivots break down to the FP64 redo (k_potrf_redo64).
+// Low-precision factors with the pivot-ratio safeguard on refactor the
+// diagonal blocks whose pivots break down in FP64 (redo_block64).
 template <typename T>
 static inline bool redo_safe(const mxc_factor *F) {
     return std::is_same<T, float>::value && F->rho > 0 && F->d_redo != nullptr;
+}
+
+static inline Redo redo_args(const mxc_factor *F) {
+    return Redo{F->d_fail + 1, F->d_redo, (const long long *)F->d_rowptr, (const int *)F->d_col,
+                (const double *)F->d_vals, (const long long *)F->d_perm, (const int *)F->d_pinv,
+                (const double *)F->d_beta};
 }
 
 template <typename T, int MODE, bool PD = false>
@@ -1476,7 +1482,7 @@ static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = n
     if (!st) st = F->stream;
     const bool h16 = std::is_same<T, float>::value && F->split16 && F->d_Lh;
     const double *adiag = F->d_adiag;
-    const PdArgs pda{F->d_diag, F->d_fail, redo_safe<T>(F) ? F->rho : 0.0, RedoQ{F->d_fail + 1, F->d_redo}};
+    const PdArgs pda{F->d_diag, F->d_fail, redo_safe<T>(F) ? F->rho : 0.0, redo_args(F)};
     const dim3 g((unsigned)L.count);
     if constexpr (std::is_same<T, float>::value && MODE != MODE_STORE) {
         // FP32 factor: split-FP16 products on the FP16 MFMA, operands from the
@@ -1523,13 +1529,11 @@ static int enqueue_factor(mxc_factor *F) {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_fail, INT_MAX);
     for (int k = 1; k < 4; k++) hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_fail + k, 0);
     const bool safe = redo_safe<T>(F);
-    const RedoQ rq{F->d_fail + 1, F->d_redo};
-    // after a launch that factors diagonal blocks: refactor in FP64 the blocks it queued
+    const Redo rq = redo_args(F);
+    // after a one-wave or one-lane POTRF launch: refactor in FP64 the blocks it
+    // queued (the workgroup POTRF and the diagonal tails redo their own)
     auto redo = [&]() {
-        if (safe)
-            hipLaunchKernelGGL(k_potrf_redo64<T>, dim3(REDO_WG), dim3(256), 0, sm, rq, W, F->d_fail,
-                               (const long long *)F->d_rowptr, (const int *)F->d_col, (const double *)F->d_vals,
-                               (const long long *)F->d_perm, (const int *)F->d_pinv, (const double *)F->d_beta);
+        if (safe) hipLaunchKernelGGL(k_potrf_redo64<T>, dim3(REDO_WG), dim3(256), 0, sm, rq, W, F->d_fail);
     };
     // assembly: L = 0, scatter A, + beta on the diagonal; also refresh the CSR values
     tick(F, 0, &tt);
@@ -1573,7 +1577,6 @@ static int enqueue_factor(mxc_factor *F) {
                     hipLaunchKernelGGL((k_potrf_diag<T, false>), dim3((unsigned)L.count), dim3(256), 0,
                                        sm, F->d_diag + L.begin, W, F->d_fail,
                                        (const double *)F->d_adiag, 0.0, rq);
-                redo();
                 F->counts[1]++;
                 break;
             case L_DIAG_SMALL: {
@@ -1620,7 +1623,6 @@ static int enqueue_factor(mxc_factor *F) {
                 break;
             case L_TRAILD:
                 launch_tile<T, MODE_TRAIL, true>(F, L, W);
-                redo();
                 F->counts[3]++;
                 break;
             case L_TRAIL2:
