@@ -12,13 +12,14 @@ mixed form) of the same system, inputs resident in HBM.
     reference's FP64 x; roofline of the dominant kernel class from HIP events.
   * 64^3 7-point diffusion, FP64, 1 RHS (secondary record).
   * batched: 4,096 2D systems (side 16..128), FP64 and FP32+IR.
-  * serving (--serving N): a Poisson stream of N requests.
+  * serving: a Poisson stream of 20,000 requests (--serving N), 2D 5-point
+    side 32..512 and 3D 7-point side 16..64, FP32+IR, run in a child process.
   * cpu_baseline: the reference CHOLMOD (compiled from the reference sources,
     MKL BLAS, OpenMP) on the host cores this job may use, same matrix and
     ordering: FP64, and its single-precision factor + FP64 refinement.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
-                    [--no-64] [--serving N]
+                    [--no-64] [--serving N] [--only-headline] [--only-serving]
 
 Prints ONE JSON line.
 """
@@ -384,8 +385,8 @@ def main():
                     help="skip the batched workload (4,096 2D systems, FP64 and FP32+IR)")
     ap.add_argument("--only-batch", action="store_true", help="run only the batched workload")
     ap.add_argument("--only-serving", action="store_true", help="run only the serving stream (--serving N)")
-    ap.add_argument("--serving", type=int, default=0, metavar="N",
-                    help="also run a serving stream of N requests (FP32+IR)")
+    ap.add_argument("--serving", type=int, default=20000, metavar="N",
+                    help="serving stream of N requests (FP32+IR; 0: none), in a child process")
     ap.add_argument("--serving-rate", type=float, default=100.0, help="Poisson arrival rate (1/s)")
     ap.add_argument("--serving-slots", type=int, default=8)
     ap.add_argument("--serving-budget-gib", type=float, default=64.0)
@@ -434,7 +435,7 @@ def main():
         if not args.no_batch:
             batch_section(args, out)
         if args.serving:
-            serving_section(args, out)
+            serving_child(args, out)
     print(json.dumps(out))
 
 
@@ -635,6 +636,21 @@ def batch_section(args, out):
         gc.collect()
     res["speedup_fp32ir_vs_fp64"] = round(res["fp64"]["time_ms"] / res["fp32"]["time_ms"], 3)
     out["batched"] = res
+
+
+def serving_child(args, out):
+    """The serving stream in a child process (this script with --only-serving),
+    so that whatever happens there cannot take the bench line with it."""
+    import subprocess
+    cmd = [sys.executable, "-X", "faulthandler", str(ROOT / "bench.py"), "--only-serving",
+           "--serving", str(args.serving), "--serving-rate", str(args.serving_rate),
+           "--serving-slots", str(args.serving_slots), "--serving-budget-gib", str(args.serving_budget_gib)]
+    log(f"[bench] serving child: {' '.join(cmd[2:])}")
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True)
+    try:
+        out["serving"] = json.loads(p.stdout.strip().splitlines()[-1])["serving"]
+    except Exception as e:
+        out["serving"] = {"error": f"serving child exited {p.returncode}: {e!r}"}
 
 
 def serving_section(args, out):
