@@ -4,7 +4,8 @@ One "step" = one numeric factorization + one solve (+ FP64 refinement for the
 mixed form) of the same system, inputs resident in HBM.
 
   * headline (the metric's workload): 3D 27-point stencil on 96^3
-    (n = 884,736), METIS ordering (the reference's own permutation), 4 RHS:
+    (n = 884,736), METIS ordering (the product's own, identical to the
+    reference's), 4 RHS:
     FP32 factor + FP64 iterative refinement to a 1e-10 relative residual
     against plain FP64 factor + solve.  value = FP32+IR factor+solve GFLOP/s =
     (fl + 4 lnz nrhs) / time-to-solution; the FP64 figure, the speedup,
@@ -45,8 +46,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def load_perm(name: str) -> np.ndarray:
-    return np.load(ROOT / "tests" / "golden" / f"perm_{name}.npz")["perm"].astype(np.int64)
+def product_ordering(A, ordering: int) -> tuple[np.ndarray, float]:
+    """The product's own fill-reducing ordering and analysis of A (METIS for
+    the 3D workloads, as cholmod_l_analyze computes it): the final permutation,
+    which the later analyses of the same matrix take as given, and the time."""
+    from suitesparse_amd import cholmod
+    cm = cholmod.Common(nmethods=1, ordering=ordering)
+    t0 = time.perf_counter()
+    L = cholmod.analyze(A, cm)
+    assert L is not None and L.ordering == ordering, cm.status
+    perm = L.Perm.copy()
+    L.free()
+    return perm, time.perf_counter() - t0
 
 
 def relres(A, B, X) -> float:
@@ -173,7 +184,7 @@ def gpu_factor_solve(A, perm, B, precision, steps, warmup, refine=False, tol=1e-
     class.  Returns timings, the last X and the factor."""
     import torch
     from suitesparse_amd import cholmod
-    cm = cholmod.Common(precision=precision)
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision=precision)
     t0 = time.perf_counter()
     L = cholmod.analyze_p(A, perm, cm)
     assert L is not None, cm.status
@@ -449,7 +460,9 @@ def headline_section(args) -> dict:
     side = args.side
     log(f"[bench] generating {side}^3 27-point stencil")
     A3, B3, _ = problems.workload_stencil27_96(side)
-    perm3 = load_perm("stencil27_96_metis") if side == 96 else None
+    from suitesparse_amd import cholmod
+    log(f"[bench] {side}^3 METIS ordering + analysis (CPU)")
+    perm3, t_order = product_ordering(A3, cholmod.CHOLMOD_METIS)
     nrhs = B3.shape[1]
     res = {}
     for prec in ("fp64", "fp32"):
@@ -486,6 +499,7 @@ def headline_section(args) -> dict:
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64ir", "data": "synthetic",
         "config": {"workload": f"stencil3d_27pt_{side}^3_metis_4rhs_fp32ir_vs_fp64", "n": A3.n,
                    "nnz": A3.nnz_full, "fl": fl, "lnz": lnz, "nrhs": nrhs},
+        "t_order_metis_s": round(t_order, 2),
         "solves_per_s": f32["solves_per_s"], "latency_ms": f32["latency_ms"],
         "fp64_value": f64["factor_solve_gflops"], "fp64_ms_per_step": f64["time_to_solution_ms"],
         "fp64_solves_per_s": f64["solves_per_s"], "fp64_latency_ms": f64["latency_ms"],
@@ -527,7 +541,9 @@ def section_64(args) -> dict:
     from suitesparse_amd import problems
     log("[bench] generating 64^3 7-point diffusion")
     A, B, xt = problems.workload_diffusion3d_64()
-    perm = load_perm("diffusion3d_64_metis")
+    from suitesparse_amd import cholmod
+    log("[bench] 64^3 METIS ordering + analysis (CPU)")
+    perm, t_order = product_ordering(A, cholmod.CHOLMOD_METIS)
     log("[bench] FP64 factor+solve on GPU")
     g = gpu_factor_solve(A, perm, B, "fp64", args.steps, args.warmup,
                          tune=("diffusion3d_64", args.tiles), roof_steps=args.roof_steps)
@@ -538,7 +554,8 @@ def section_64(args) -> dict:
     lat = np.array(g["lat"]) * 1e3
     roof = class_roofline(g, "fp64")
     sec = {"workload": "diffusion3d_7pt_64^3_metis_fp64_1rhs", "n": A.n, "nnz": A.nnz_full,
-           "fl": fl, "lnz": lnz, "nrhs": nrhs, "value_gflops": round(value, 2), "ms_per_step": round(ms, 3),
+           "fl": fl, "lnz": lnz, "nrhs": nrhs, "t_order_metis_s": round(t_order, 2),
+           "value_gflops": round(value, 2), "ms_per_step": round(ms, 3),
            "solves_per_s": round(1e3 / ms, 3),
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                           "p99": round(float(np.percentile(lat, 99)), 3)},
@@ -666,7 +683,7 @@ def serving_section(args, out):
     srv = serving.Server(slots=args.serving_slots, hbm_budget=int(args.serving_budget_gib * (1 << 30)),
                          precision="fp32", tol=1e-10, keep_results=False)
     st = serving.run_stream(srv, reqs, arrivals, progress=1000)
-    st.update({"workload": f"serving_{n}req_poisson{args.serving_rate:g}_2d32..512_3d16..64_amd_fp32ir",
+    st.update({"workload": f"serving_{n}req_poisson{args.serving_rate:g}_2d32..512amd_3d16..64metis_fp32ir",
                "rate_per_s": args.serving_rate, "slots": args.serving_slots,
                "hbm_budget_gib": args.serving_budget_gib})
     log(f"[bench]   serving: {st}")

@@ -12,7 +12,11 @@
  *
  * Reference interfaces replaced (paths relative to the SuiteSparse root):
  *   mxc_analyze            cholmod_l_analyze_p   CHOLMOD/Include/cholmod.h:2734
- *                          (CHOLMOD/Cholesky/cholmod_analyze.c:384-909, GIVEN/NATURAL/AMD)
+ *                          (CHOLMOD/Cholesky/cholmod_analyze.c:384-909: GIVEN, NATURAL,
+ *                          AMD, METIS, and the default GIVEN -> AMD -> METIS strategy)
+ *   mxc_amd_order          cholmod_l_amd       CHOLMOD/Cholesky/cholmod_amd.c:40-194
+ *   mxc_metis_order        cholmod_l_metis     CHOLMOD/Partition/cholmod_metis.c:557-846
+ *                          (METIS_NodeND, SuiteSparse_metis/libmetis/ometis.c:43)
  *   mxc_symbolic_import    the supernodal fields of cholmod_factor, cholmod.h:1584-1594
  *   mxc_device_init        cholmod_l_gpu_start / cholmod_l_gpu_allocate  cholmod.h:4148-4180
  *                          (CHOLMOD/GPU/cholmod_gpu.c:234, :426)
@@ -55,10 +59,15 @@ extern "C" {
 #define MXC_GPU_PROBLEM   (-5)
 #define MXC_NOT_POSDEF      1
 
-/* orderings (values of CHOLMOD_NATURAL, CHOLMOD_GIVEN, CHOLMOD_AMD) */
+/* orderings (values of CHOLMOD_NATURAL, CHOLMOD_GIVEN, CHOLMOD_AMD, CHOLMOD_METIS) */
 #define MXC_ORDER_NATURAL 0
 #define MXC_ORDER_GIVEN   1
 #define MXC_ORDER_AMD     2
+#define MXC_ORDER_METIS   3
+/* CHOLMOD's default strategy (Common->nmethods = 0): GIVEN if a UserPerm is
+ * passed, then AMD, then METIS unless AMD already predicts fewer than 500
+ * flops per entry of L or fill below 5x nnz(A); the smallest nnz(L) wins */
+#define MXC_ORDER_DEFAULT (-1)
 
 /* working precision of the numeric factor */
 #define MXC_FP64 0
@@ -75,14 +84,24 @@ typedef struct mxc_pool mxc_pool;           /* device memory pool (cholmod_commo
 /* ---- symbolic analysis (CPU) ------------------------------------------- */
 
 /* Analyze tril(A) (n x n, CSC, Ap[n+1], Ai[Ap[n]], rows i >= j) under an
- * ordering.  Produces Perm, ColCount, Super, Lpi, Lpx, Ls exactly as
- * cholmod_l_analyze_p does with default parameters and supernodal = SUPERNODAL. */
+ * ordering (MXC_ORDER_*; UserPerm is used by GIVEN and, when not NULL, by
+ * DEFAULT).  Produces Perm, ColCount, Super, Lpi, Lpx, Ls exactly as
+ * cholmod_l_analyze_p does with default parameters and supernodal = SUPERNODAL
+ * (with nmethods = 1 and that ordering, or nmethods = 0 for DEFAULT). */
 int mxc_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, int ordering,
                 const int64_t *UserPerm, mxc_symbolic **S);
 
 /* info[0..6] = n, nsuper, ssize, xsize, maxcsize, maxesize, ordering;
  * dinfo[0..1] = fl (Common->fl), lnz (Common->lnz). */
 int mxc_symbolic_info(const mxc_symbolic *S, int64_t *info, double *dinfo);
+
+/* What the ordering strategy left in Common: selected = index of the chosen
+ * method (0 GIVEN, 1 AMD, 2 METIS under DEFAULT; 0 for a single ordering),
+ * method_fl[3] / method_lnz[3] = Common->method[0..2].fl / lnz (AMD's are its
+ * own predictions; -1 where a method was not tried), anz = Common->anz
+ * (-1 if no ordering computed it).  Any output pointer may be NULL. */
+int mxc_symbolic_strategy(const mxc_symbolic *S, int64_t *selected, double *method_fl,
+                          double *method_lnz, double *anz);
 
 /* Copy out Perm[n], ColCount[n], Super[nsuper+1], Lpi[nsuper+1], Lpx[nsuper+1],
  * Ls[ssize] (any pointer may be NULL). */
@@ -99,6 +118,19 @@ void mxc_symbolic_free(mxc_symbolic *S);
 /* Approximate minimum degree ordering of tril(A) as used by cholmod_l_analyze
  * with CHOLMOD_AMD: Perm[k] = original index of the k-th pivot. */
 int mxc_amd_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm);
+
+/* The same, plus the statistics cholmod_l_amd leaves in Common:
+ * stats[0] = fl, stats[1] = lnz (AMD's predictions), stats[2] = anz. */
+int mxc_amd_order_stats(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm,
+                        double *stats);
+
+/* Nested-dissection ordering of tril(A) exactly as cholmod_l_metis computes it
+ * inside cholmod_l_analyze (METIS_NodeND with default options on A + A',
+ * no postorder): Perm[k] = original index of the k-th pivot.  nswitch and
+ * dswitch are Common->metis_nswitch / metis_dswitch (defaults 3000, 0.66):
+ * a matrix with n > nswitch and density > dswitch gets the identity. */
+int mxc_metis_order(int64_t n, const int64_t *Ap, const int64_t *Ai, double nswitch, double dswitch,
+                    int64_t *Perm);
 
 /* ---- device ------------------------------------------------------------- */
 

@@ -18,7 +18,7 @@ import numpy as np
 _HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = _HERE / "_ref" / "libssref.so"
 
-GIVEN, AMD, METIS, NATURAL = 0, 1, 2, 3
+GIVEN, AMD, METIS, NATURAL, DEFAULT = 0, 1, 2, 3, 4
 _lib = None
 
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
@@ -53,6 +53,10 @@ def lib():
         L.ref_residual.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int64, _f64p]
         L.ref_solve2_bset.argtypes = [C.c_void_p, C.c_int, _f64p, _i64p, C.c_int64, _f64p, _i64p,
                                       C.POINTER(C.c_int64)]
+        L.ref_strategy.argtypes = [C.c_void_p, C.POINTER(C.c_int64), _f64p, _f64p,
+                                   C.POINTER(C.c_double)]
+        L.ref_order.argtypes = [C.c_int64, _i64p, _i64p, C.c_int, _i64p, _f64p]
+        L.ref_order.restype = C.c_int
         L.ref_free.argtypes = [C.c_void_p]
         L.ref_free.restype = None
         L.ref_read_sparse.argtypes = [C.c_char_p] + [C.c_void_p] * 4
@@ -84,6 +88,18 @@ def read_sparse(path) -> dict:
             "x": x[:nnz]}
 
 
+def order(n: int, p, i, which: int):
+    """The raw cholmod_l_amd (which = AMD) or cholmod_l_metis (which = METIS,
+    no postorder) permutation of tril(A), and Common->fl, lnz, anz after it."""
+    perm = np.zeros(n, dtype=np.int64)
+    stats = np.zeros(3)
+    rc = lib().ref_order(int(n), np.ascontiguousarray(p, dtype=np.int64),
+                         np.ascontiguousarray(i, dtype=np.int64), int(which), perm, stats)
+    if rc != 0:
+        raise RuntimeError(f"reference ordering failed ({rc})")
+    return perm, stats
+
+
 class RefFactor:
     """Reference analyze -> factorize -> solve on tril(A) (stype = -1)."""
 
@@ -105,6 +121,14 @@ class RefFactor:
         (self.n, self.nsuper, self.ssize, self.xsize, self.maxcsize, self.maxesize,
          self.is_super, self.ordering, self.status, self.minor) = (int(v) for v in info)
         self.fl, self.lnz = float(dinfo[0]), float(dinfo[1])
+
+    def strategy(self) -> dict:
+        """Common->selected, method[0..2].fl / lnz and anz left by the analysis."""
+        sel, anz = C.c_int64(0), C.c_double(0)
+        fl, lnz = np.zeros(3), np.zeros(3)
+        lib().ref_strategy(self._h, C.byref(sel), fl, lnz, C.byref(anz))
+        return {"selected": int(sel.value), "method_fl": tuple(fl), "method_lnz": tuple(lnz),
+                "anz": float(anz.value)}
 
     def symbolic(self) -> dict:
         n, ns = self.n, self.nsuper

@@ -54,7 +54,8 @@ int ref_set_threads(int nthreads) {
     return omp_get_max_threads();
 }
 
-// ordering: 0 = given (UserPerm), 1 = AMD, 2 = METIS, 3 = natural
+// ordering: 0 = given (UserPerm), 1 = AMD, 2 = METIS, 3 = natural,
+//           4 = CHOLMOD's default strategy (nmethods = 0; UserPerm used if not NULL)
 // dtype: 0 = double factor, 1 = single factor (values converted on input)
 void *ref_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax,
                   int ordering, const int64_t *UserPerm, int single, double *t_analyze) {
@@ -72,6 +73,7 @@ void *ref_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         case 0: cm->method[0].ordering = CHOLMOD_GIVEN; break;
         case 1: cm->method[0].ordering = CHOLMOD_AMD; break;
         case 2: cm->method[0].ordering = CHOLMOD_METIS; break;
+        case 4: cm->nmethods = 0; break;
         default: cm->method[0].ordering = CHOLMOD_NATURAL; break;
     }
     int64_t nnz = Ap[n];
@@ -86,7 +88,7 @@ void *ref_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         h->dtype = CHOLMOD_SINGLE;
     }
     double t0 = now_s();
-    if (ordering == 0)
+    if (ordering == 0 || (ordering == 4 && UserPerm))
         h->L = cholmod_l_analyze_p(h->A, (int64_t *)UserPerm, NULL, 0, cm);
     else
         h->L = cholmod_l_analyze(h->A, cm);
@@ -98,6 +100,18 @@ void *ref_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         return NULL;
     }
     return h;
+}
+
+// Common->selected, method[0..2].fl / lnz and anz after ref_analyze
+int ref_strategy(void *vh, int64_t *selected, double *method_fl, double *method_lnz, double *anz) {
+    ref_handle *h = (ref_handle *)vh;
+    *selected = h->cm.selected;
+    for (int m = 0; m < 3; m++) {
+        method_fl[m] = h->cm.method[m].fl;
+        method_lnz[m] = h->cm.method[m].lnz;
+    }
+    *anz = h->cm.anz;
+    return 0;
 }
 
 // info[0..9] = n, nsuper, ssize, xsize, maxcsize, maxesize, is_super, ordering, status, minor
@@ -329,4 +343,26 @@ int ref_read_sparse(const char *path, int64_t *dims, int64_t *Ap, int64_t *Ai, d
     cholmod_l_free_sparse(&A, &cm);
     cholmod_l_finish(&cm);
     return 0;
+}
+
+// The raw orderings CHOLMOD's analyze computes before its postorder:
+// which = 1: cholmod_l_amd, 2: cholmod_l_metis (postorder FALSE), on tril(A).
+// stats[0..2] = Common->fl, Common->lnz, Common->anz after the call (AMD
+// sets all three; METIS only anz).
+int ref_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int which, int64_t *Perm,
+              double *stats) {
+    cholmod_common cm;
+    cholmod_l_start(&cm);
+    int64_t nnz = Ap[n];
+    cholmod_sparse *A = cholmod_l_allocate_sparse(n, n, nnz, 1, 1, -1, CHOLMOD_PATTERN, &cm);
+    if (!A) { cholmod_l_finish(&cm); return -1; }
+    memcpy(A->p, Ap, sizeof(int64_t) * (n + 1));
+    memcpy(A->i, Ai, sizeof(int64_t) * nnz);
+    cm.fl = cm.lnz = cm.anz = -1;
+    int ok = which == 1 ? cholmod_l_amd(A, NULL, 0, Perm, &cm)
+                        : cholmod_l_metis(A, NULL, 0, 0, Perm, &cm);
+    if (stats) { stats[0] = cm.fl; stats[1] = cm.lnz; stats[2] = cm.anz; }
+    cholmod_l_free_sparse(&A, &cm);
+    cholmod_l_finish(&cm);
+    return ok ? 0 : -2;
 }

@@ -15,7 +15,7 @@ HEADER = PKG.parent / "include" / "cholmod_mi355x.h"
 
 MXC_OK, MXC_NOT_INSTALLED, MXC_OUT_OF_MEMORY, MXC_TOO_LARGE = 0, -1, -2, -3
 MXC_INVALID, MXC_GPU_PROBLEM, MXC_NOT_POSDEF = -4, -5, 1
-ORDER_NATURAL, ORDER_GIVEN, ORDER_AMD = 0, 1, 2
+ORDER_NATURAL, ORDER_GIVEN, ORDER_AMD, ORDER_METIS, ORDER_DEFAULT = 0, 1, 2, 3, -1
 FP64, FP32 = 0, 1
 HOST, DEVICE = 0, 1
 
@@ -34,6 +34,9 @@ _PROTOS = {
     "mxc_symbolic_import": (i32, [i64, i64, vp, vp, vp, vp, vp, C.POINTER(vp)]),
     "mxc_symbolic_free": (None, [vp]),
     "mxc_amd_order": (i32, [i64, vp, vp, vp]),
+    "mxc_amd_order_stats": (i32, [i64, vp, vp, vp, vp]),
+    "mxc_metis_order": (i32, [i64, vp, vp, C.c_double, C.c_double, vp]),
+    "mxc_symbolic_strategy": (i32, [vp, vp, vp, vp, vp]),
     "mxc_device_init": (i32, [i32]),
     "mxc_device_info": (i32, [vp]),
     "mxc_factor_create": (i32, [vp, vp, vp, i32, C.POINTER(vp)]),

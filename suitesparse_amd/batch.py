@@ -101,6 +101,7 @@ def analyze_batch(mats, cm: cholmod.Common, ordering: int = cholmod.CHOLMOD_AMD)
         perms.append(cache[key])
     A, off, noff = block_diagonal(mats)
     perm = np.concatenate([pm + off[j] for j, pm in enumerate(perms)]) if perms else np.zeros(0, np.int64)
+    cm.nmethods, cm.ordering = 1, cholmod.CHOLMOD_GIVEN      # the block permutation as given
     L = cholmod.analyze_p(A, perm, cm)
     if L is None:
         return None

@@ -19,9 +19,9 @@ ARCH = "gfx950"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
-C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wno-unused-function"]
+C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 
-C_SOURCES = ["symbolic.c", "amd.c"]
+C_SOURCES = ["symbolic.c", "amd.c", "metis_nd.c"]
 HIP_SOURCES = ["mxc_device.hip"]
 
 

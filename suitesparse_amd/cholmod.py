@@ -8,7 +8,8 @@ error behaviour -- on top of the C ABI (include/cholmod_mi355x.h):
   ``factorize(A, L, cm)``           cholmod_l_factorize   cholmod.h:2775
   ``factorize_p(A, beta, L, cm)``   cholmod_l_factorize_p cholmod.h:2794
   ``solve(sys, L, B, cm)``          cholmod_l_solve       cholmod.h:2823
-  ``Common``                        cholmod_common: status, fl, lnz, useGPU, print
+  ``Common``                        cholmod_common: status, fl, lnz, nmethods, method[0].ordering,
+                                    selected, method[0..2].fl/lnz, anz, useGPU, print
   ``Factor``                        cholmod_factor: n, minor, Perm, ColCount, super, pi,
                                     px, s, x, nsuper, ssize, xsize, maxcsize, maxesize
 
@@ -45,6 +46,8 @@ CHOLMOD_NOT_POSDEF = 1
 CHOLMOD_NATURAL = 0
 CHOLMOD_GIVEN = 1
 CHOLMOD_AMD = 2
+CHOLMOD_METIS = 3
+CHOLMOD_COLAMD = 5
 
 # solve systems (cholmod.h:2805-2816)
 CHOLMOD_A, CHOLMOD_LDLt, CHOLMOD_LD, CHOLMOD_DLt = 0, 1, 2, 3
@@ -59,7 +62,13 @@ class Common:
     status: int = CHOLMOD_OK
     fl: float = -1.0
     lnz: float = -1.0
-    ordering: int = CHOLMOD_AMD        # method[0].ordering with nmethods = 1
+    nmethods: int = 0                  # 0: default strategy (GIVEN if perm, AMD, then METIS)
+    ordering: int = CHOLMOD_AMD        # method[0].ordering, used when nmethods = 1
+    # left by analyze: Common->selected, method[0..2].fl / lnz, anz
+    selected: int = -1
+    method_fl: tuple = (-1.0, -1.0, -1.0)
+    method_lnz: tuple = (-1.0, -1.0, -1.0)
+    anz: float = -1.0
     useGPU: int = 1
     device: int = 0
     precision: str = "fp64"            # numeric factor: "fp64" or "fp32" (+ FP64 refinement)
@@ -273,7 +282,12 @@ class Factor:
 # ---------------------------------------------------------------------------
 
 def analyze_p(A, perm, cm: Common) -> Factor | None:
-    """Order (GIVEN when perm is not None, else cm.ordering) and analyze tril(A)."""
+    """Order and analyze tril(A) as cholmod_l_analyze_p does (cholmod_analyze.c:384-909).
+
+    nmethods = 0: the default strategy -- the given perm (if any), AMD, then
+    METIS unless AMD's prediction is already good; the smallest nnz(L) wins.
+    nmethods = 1: cm.ordering alone (GIVEN uses perm, and falls back to AMD
+    without one; COLAMD is AMD for a symmetric matrix)."""
     cm.status = CHOLMOD_OK
     h = C.c_void_p()
     p = np.ascontiguousarray(A.p, dtype=np.int64)
@@ -283,15 +297,33 @@ def analyze_p(A, perm, cm: Common) -> Factor | None:
         if perm.shape != (A.n,):
             cm.status = CHOLMOD_INVALID
             return None
-        ordering = CHOLMOD_GIVEN
-    else:
+    if cm.nmethods == 0:
+        ordering = _lib.ORDER_DEFAULT
+    elif cm.nmethods == 1:
         ordering = cm.ordering
-    st = lib().mxc_analyze(int(A.n), _ptr(p), _ptr(i), int(ordering), _ptr(perm), C.byref(h))
+        if ordering == CHOLMOD_COLAMD or (ordering == CHOLMOD_GIVEN and perm is None):
+            ordering = CHOLMOD_AMD
+        if ordering not in (CHOLMOD_NATURAL, CHOLMOD_GIVEN, CHOLMOD_AMD, CHOLMOD_METIS):
+            cm.status = CHOLMOD_NOT_INSTALLED
+            return None
+    else:
+        cm.status = CHOLMOD_INVALID
+        return None
+    st = lib().mxc_analyze(int(A.n), _ptr(p), _ptr(i), int(ordering),
+                           _ptr(perm) if ordering in (CHOLMOD_GIVEN, _lib.ORDER_DEFAULT) else None,
+                           C.byref(h))
     if st != 0:
         cm.status = st
         return None
     L = Factor(h, A, cm)
     cm.fl, cm.lnz = L.fl, L.lnz
+    sel = C.c_int64()
+    mfl, mlnz, anz = np.zeros(3), np.zeros(3), C.c_double()
+    check("mxc_symbolic_strategy", lib().mxc_symbolic_strategy(h, C.byref(sel), _ptr(mfl), _ptr(mlnz),
+                                                              C.byref(anz)))
+    cm.selected = int(sel.value)
+    cm.method_fl, cm.method_lnz = tuple(float(v) for v in mfl), tuple(float(v) for v in mlnz)
+    cm.anz = float(anz.value)
     return L
 
 

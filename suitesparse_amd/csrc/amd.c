@@ -146,8 +146,14 @@ static Int build_element(amd_state *S, Int me, Int elenme, Int *pme1_out, Int *p
     return degme;
 }
 
-int mxc_amd_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm) {
+// stats (optional): the fill and work AMD predicts, as cholmod_l_amd stores
+// them in Common (CHOLMOD/Cholesky/cholmod_amd.c:145-180, from the Info
+// counts of AMD/Source/amd_2.c:1515-1590): stats[0] = fl = divisions + 2 *
+// LDL' multiply-subtracts + n, stats[1] = lnz = n + nnz(L) below the
+// diagonal, stats[2] = anz = n + nnz(strictly lower A).
+static int amd_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm, double *stats) {
     if (n < 0 || !Perm || (n > 0 && (!Ap || !Ai))) return MXC_INVALID;
+    if (stats) stats[0] = stats[1] = stats[2] = 0;
     if (n == 0) return MXC_OK;
 
     // ---- pattern of A + A' without the diagonal (column layout of cholmod_l_copy)
@@ -159,6 +165,7 @@ int mxc_amd_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm
             Int i = Ai[p];
             if (i > j) { cnt[j]++; cnt[i]++; cnz += 2; }
         }
+    if (stats) stats[2] = (double)(cnz / 2 + n);
     Int iwlen = cnz + cnz / 2 + n;
     amd_state S;
     memset(&S, 0, sizeof(S));
@@ -203,7 +210,8 @@ int mxc_amd_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm
     S.nel = 0;
     S.lemax = 0;
     Int ndense = 0;
-    (void)ndense;
+    // predicted factor statistics, accumulated in the order amd_2 does
+    double st_lnz = 0, st_ndiv = 0, st_nms_ldl = 0;
 
     // empty rows are eliminated at once; dense rows are set aside (ordered last)
     for (Int i = 0; i < n; i++) {
@@ -385,12 +393,32 @@ int mxc_amd_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm
             iw[p++] = i;
         }
         nv[me] = nvpiv;
+        {
+            const double f = (double)nvpiv, r = (double)(degme + ndense);
+            const double lnzme = f * r + (f - 1) * f / 2;
+            st_lnz += lnzme;
+            st_ndiv += lnzme;
+            const double s = f * r * r + r * (f - 1) * f + (f - 1) * f * (2 * f - 1) / 6;
+            st_nms_ldl += (s + lnzme) / 2;
+        }
         len[me] = p - pme1;
         if (len[me] == 0) {
             pe[me] = NONE;
             W[me] = 0;
         }
         if (elenme != 0) S.pfree = p;
+    }
+
+    if (stats) {
+        // the dense rows form one final dense block
+        const double f = (double)ndense;
+        const double lnzme = (f - 1) * f / 2;
+        st_lnz += lnzme;
+        st_ndiv += lnzme;
+        const double s = (f - 1) * f * (2 * f - 1) / 6;
+        st_nms_ldl += (s + lnzme) / 2;
+        stats[0] = st_ndiv + 2 * st_nms_ldl + (double)n;
+        stats[1] = (double)n + st_lnz;
     }
 
     // ---- assembly tree: parents in pe, front sizes in elen
@@ -482,4 +510,12 @@ int mxc_amd_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm
     for (Int i = 0; i < n; i++) Perm[pos[i]] = i;
     free(mem);
     return MXC_OK;
+}
+
+int mxc_amd_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm) {
+    return amd_order(n, Ap, Ai, Perm, NULL);
+}
+
+int mxc_amd_order_stats(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *Perm, double *stats) {
+    return amd_order(n, Ap, Ai, Perm, stats);
 }

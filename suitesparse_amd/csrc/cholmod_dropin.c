@@ -416,13 +416,23 @@ cholmod_factor *cholmod_l_analyze_p(cholmod_sparse *A, int64_t *UserPerm, int64_
         return NULL;
     }
     const int nm = cm->nmethods < 0 ? 0 : cm->nmethods > CHOLMOD_MAXMETHODS ? CHOLMOD_MAXMETHODS : cm->nmethods;
-    int custom = !cm->postorder;
-    for (int m = 0; m < nm; m++) {
-        const int o = cm->method[m].ordering;
+    /* anything the restated orderings do not cover goes to the host library:
+     * NESDIS, other methods, non-default AMD controls (cholmod_amd.c:160-164),
+     * no postorder */
+    int custom = !cm->postorder || (nm == 0 && cm->default_nesdis);
+    for (int m = 0; m < (nm == 0 ? 3 : nm); m++) {
+        const int o = nm == 0 ? (m == 1 ? CHOLMOD_AMD : CHOLMOD_GIVEN) : cm->method[m].ordering;
         if (o != CHOLMOD_NATURAL && o != CHOLMOD_GIVEN && o != CHOLMOD_AMD && o != CHOLMOD_COLAMD &&
-            o != CHOLMOD_METIS && o != CHOLMOD_NESDIS)
+            o != CHOLMOD_METIS)
+            custom = 1;
+        if ((o == CHOLMOD_AMD || o == CHOLMOD_COLAMD) &&
+            (cm->method[m].prune_dense != 10.0 || !cm->method[m].aggressive))
             custom = 1;
     }
+    /* the AMD backup after failed methods runs with the controls of the slot after the last */
+    if (nm > 0 && nm <= CHOLMOD_MAXMETHODS &&
+        (cm->method[nm].prune_dense != 10.0 || !cm->method[nm].aggressive))
+        custom = 1;
     if (!use_gpu(cm) || !device_matrix(A) || custom)
         return NEXT(analyze_p_fn, "cholmod_l_analyze_p")(A, UserPerm, fset, fsize, cm);
 
@@ -438,20 +448,18 @@ cholmod_factor *cholmod_l_analyze_p(cholmod_sparse *A, int64_t *UserPerm, int64_
                rc == -3 ? "duplicate entries in A" : "invalid A");
         return NULL;
     }
-    int64_t anz = n;
-    for (int64_t j = 0; j < n; j++)
-        for (int64_t p = Cp[j]; p < Cp[j + 1]; p++) anz += Ci[p] > j;
-    cm->anz = (double)anz;
+    cm->anz = -1;                        /* set by the AMD and METIS orderings */
 
-    /* the methods to try (cholmod_analyze.c:451-489) */
+    /* the methods to try (cholmod_analyze.c:444-490) */
     int methods[CHOLMOD_MAXMETHODS + 1], nmeth, amd_backup;
-    if (nm == 0) {
-        methods[0] = CHOLMOD_GIVEN;
+    const int default_strategy = nm == 0;
+    if (default_strategy) {
+        methods[0] = CHOLMOD_GIVEN;      /* skipped without a UserPerm */
         methods[1] = CHOLMOD_AMD;
-        nmeth = 2;                       /* no Partition module: METIS is not in the default list */
+        methods[2] = CHOLMOD_METIS;      /* default_nesdis goes to the host library */
+        nmeth = 3;
         amd_backup = 0;
-        cm->method[0].ordering = CHOLMOD_GIVEN;
-        cm->method[1].ordering = CHOLMOD_AMD;
+        for (int m = 0; m < 3; m++) cm->method[m].ordering = methods[m];
     } else {
         for (int m = 0; m < nm; m++) methods[m] = cm->method[m].ordering;
         nmeth = nm;
@@ -459,7 +467,18 @@ cholmod_factor *cholmod_l_analyze_p(cholmod_sparse *A, int64_t *UserPerm, int64_
                                             (methods[0] == CHOLMOD_GIVEN && UserPerm == NULL)));
     }
 
+    /* Each method yields a permutation and (fl, lnz): AMD its own prediction
+     * (cholmod_amd.c:177-180), the others their exact analysis.  The first
+     * strictly smallest lnz wins; AMD's symbolic factor is only built if it
+     * wins (cholmod_analyze.c:808-822). */
     mxc_symbolic *best = NULL;
+    int64_t *best_perm = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    int64_t *cand = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    if (!best_perm || !cand) {
+        free(best_perm); free(cand); free(Cp); free(Ci);
+        report(cm, CHOLMOD_OUT_OF_MEMORY, "out of memory");
+        return NULL;
+    }
     int best_order = 0, status = CHOLMOD_OK;
     double best_lnz = -1;
     for (int m = 0; m <= nmeth; m++) {
@@ -471,41 +490,75 @@ cholmod_factor *cholmod_l_analyze_p(cholmod_sparse *A, int64_t *UserPerm, int64_
             ordering = methods[m];
         }
         cm->current = m;
-        int mo;
-        if (ordering == CHOLMOD_NATURAL) mo = MXC_ORDER_NATURAL;
-        else if (ordering == CHOLMOD_GIVEN) {
+        cm->fl = cm->lnz = -1;
+        mxc_symbolic *S = NULL;
+        int st = MXC_OK, amd = 0;
+        double mfl = -1, mlnz = -1;
+        if (ordering == CHOLMOD_NATURAL) {
+            st = mxc_analyze(n, Cp, Ci, MXC_ORDER_NATURAL, NULL, &S);
+        } else if (ordering == CHOLMOD_GIVEN) {
             if (!UserPerm) continue;
-            mo = MXC_ORDER_GIVEN;
+            st = mxc_analyze(n, Cp, Ci, MXC_ORDER_GIVEN, UserPerm, &S);
         } else if (ordering == CHOLMOD_AMD || ordering == CHOLMOD_COLAMD) {
-            mo = MXC_ORDER_AMD;
+            double stats[3];
             amd_backup = 0;
+            amd = 1;
+            st = mxc_amd_order_stats(n, Cp, Ci, cand, stats);
+            if (st == MXC_OK) {
+                mfl = stats[0];
+                mlnz = stats[1];
+                cm->anz = stats[2];
+            }
+        } else if (ordering == CHOLMOD_METIS) {
+            cm->called_nd = 1;
+            st = mxc_metis_order(n, Cp, Ci, cm->metis_nswitch, cm->metis_dswitch, cand);
+            if (st == MXC_OK) {
+                int64_t nz = 0;
+                for (int64_t j = 0; j < n; j++)
+                    for (int64_t p = Cp[j]; p < Cp[j + 1]; p++) nz += Ci[p] > j;
+                cm->anz = (double)(nz + n);
+                st = mxc_analyze(n, Cp, Ci, MXC_ORDER_GIVEN, cand, &S);
+            }
         } else {
             status = status < CHOLMOD_NOT_INSTALLED ? status : CHOLMOD_NOT_INSTALLED;
-            continue;                     /* METIS / NESDIS: Partition module not installed */
+            continue;                     /* NESDIS: not on this path */
         }
-        mxc_symbolic *S = NULL;
-        int st = mxc_analyze(n, Cp, Ci, mo, mo == MXC_ORDER_GIVEN ? UserPerm : NULL, &S);
         if (st != MXC_OK) {
             status = status < st ? status : st;
             continue;
         }
-        int64_t info[7];
-        double dinfo[2];
-        mxc_symbolic_info(S, info, dinfo);
-        if (m < CHOLMOD_MAXMETHODS) {
-            cm->method[m].fl = dinfo[0];
-            cm->method[m].lnz = dinfo[1];
+        if (S) {
+            int64_t info[7];
+            double dinfo[2];
+            mxc_symbolic_info(S, info, dinfo);
+            mfl = dinfo[0];
+            mlnz = dinfo[1];
         }
-        if (cm->selected == -1 || dinfo[1] < best_lnz) {
+        cm->fl = mfl;
+        cm->lnz = mlnz;
+        if (m < CHOLMOD_MAXMETHODS) {
+            cm->method[m].fl = mfl;
+            cm->method[m].lnz = mlnz;
+        }
+        if (cm->selected == -1 || mlnz < best_lnz) {
             if (best) mxc_symbolic_free(best);
-            best = S;
-            best_lnz = dinfo[1];
+            best = S;                     /* NULL for AMD: built below from best_perm */
+            S = NULL;
+            if (amd) memcpy(best_perm, cand, (size_t)n * sizeof(int64_t));
+            best_lnz = mlnz;
             best_order = ordering == CHOLMOD_COLAMD ? CHOLMOD_AMD : ordering;
             cm->selected = m;
-        } else {
-            mxc_symbolic_free(S);
         }
+        if (S) mxc_symbolic_free(S);
+        /* AMD already good: skip METIS (cholmod_analyze.c:767-781) */
+        if (default_strategy && amd && (cm->fl < 500 * cm->lnz || cm->lnz < 5 * cm->anz)) break;
     }
+    free(cand);
+    if (cm->selected != -1 && !best) {
+        int st = mxc_analyze(n, Cp, Ci, MXC_ORDER_GIVEN, best_perm, &best);
+        if (st != MXC_OK) status = status < st ? status : st;
+    }
+    free(best_perm);
     if (!best) {
         free(Cp);
         free(Ci);

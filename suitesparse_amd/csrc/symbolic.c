@@ -484,8 +484,103 @@ void mxc_symbolic_free(mxc_symbolic *L) {
     free(L);
 }
 
-// Analyze tril(A) under the given ordering.  ordering: MXC_ORDER_GIVEN (UserPerm),
-// MXC_ORDER_NATURAL or MXC_ORDER_AMD.
+// lnz of the factor under Perm (Common->lnz after cholmod_l_analyze_ordering)
+static int ordering_lnz(Int n, const Int *Ap, const Int *Ai, const Int *Perm, double *lnz) {
+    Int *Pinv = (Int *)xmalloc((size_t)n, sizeof(Int));
+    Int *Parent = (Int *)xmalloc((size_t)n, sizeof(Int));
+    Int *Post = (Int *)xmalloc((size_t)n, sizeof(Int));
+    Int *ColCount = (Int *)xmalloc((size_t)n, sizeof(Int));
+    Int *Wk = (Int *)xmalloc((size_t)n, sizeof(Int));
+    Int *Sp = NULL, *Si = NULL, *Fp = NULL, *Fi = NULL;
+    int rc = MXC_OUT_OF_MEMORY;
+    if (!Pinv || !Parent || !Post || !ColCount || !Wk) goto done;
+    for (Int k = 0; k < n; k++) Pinv[Perm[k]] = k;
+    rc = permute_patterns(n, Ap, Ai, Pinv, &Sp, &Si, &Fp, &Fi);
+    if (rc != MXC_OK) goto done;
+    etree_upper(n, Sp, Si, Parent, Wk);
+    if (postorder(n, Parent, NULL, Post) != n) { rc = MXC_INVALID; goto done; }
+    rc = column_counts(n, Fp, Fi, Parent, Post, ColCount);
+    if (rc != MXC_OK) goto done;
+    double s = 0;
+    for (Int j = 0; j < n; j++) s += (double)ColCount[j];
+    *lnz = s;
+done:
+    free(Pinv); free(Parent); free(Post); free(ColCount); free(Wk);
+    free(Sp); free(Si); free(Fp); free(Fi);
+    return rc;
+}
+
+// Perm from UserPerm, rejecting anything that is not a permutation of 0..n-1
+static int given_perm(Int n, const Int *UserPerm, Int *Perm, Int *Pinv) {
+    if (!UserPerm) return MXC_INVALID;
+    for (Int k = 0; k < n; k++) Pinv[k] = EMPTY;
+    for (Int k = 0; k < n; k++) {
+        Int j = UserPerm[k];
+        if (j < 0 || j >= n || Pinv[j] != EMPTY) return MXC_INVALID;
+        Pinv[j] = k;
+        Perm[k] = j;
+    }
+    return MXC_OK;
+}
+
+// CHOLMOD's default strategy (CHOLMOD/Cholesky/cholmod_analyze.c:444-462,
+// :554-782): GIVEN when UserPerm is passed (lnz of its analysis), AMD (lnz
+// as AMD predicts it), then METIS (lnz of its analysis) unless AMD predicted
+// fewer than 500 flops per entry of L or nnz(L) < 5 nnz(A); the first
+// strictly smaller lnz wins.  Leaves the winner in Perm and its ordering in
+// L->ordering.
+static int default_strategy(Int n, const Int *Ap, const Int *Ai, const Int *UserPerm, Int *Perm,
+                            Int *Wk, mxc_symbolic *L) {
+    int rc;
+    double best = -1;
+    L->selected = -1;
+    Int *cand = (Int *)xmalloc((size_t)n, sizeof(Int));
+    if (!cand) return MXC_OUT_OF_MEMORY;
+    if (UserPerm) {
+        rc = given_perm(n, UserPerm, cand, Wk);
+        if (rc != MXC_OK) goto done;
+        double lnz;
+        rc = ordering_lnz(n, Ap, Ai, cand, &lnz);
+        if (rc != MXC_OK) goto done;
+        L->method_lnz[0] = lnz;
+        best = lnz;
+        L->selected = 0;
+        L->ordering = MXC_ORDER_GIVEN;
+        memcpy(Perm, cand, sizeof(Int) * (size_t)n);
+    }
+    double st[3];
+    rc = mxc_amd_order_stats(n, Ap, Ai, cand, st);
+    if (rc != MXC_OK) goto done;
+    L->method_fl[1] = st[0];
+    L->method_lnz[1] = st[1];
+    L->anz = st[2];
+    if (L->selected == -1 || st[1] < best) {
+        best = st[1];
+        L->selected = 1;
+        L->ordering = MXC_ORDER_AMD;
+        memcpy(Perm, cand, sizeof(Int) * (size_t)n);
+    }
+    if (!(st[0] < 500 * st[1] || st[1] < 5 * st[2])) {
+        rc = mxc_metis_order(n, Ap, Ai, 3000, 0.66, cand);
+        if (rc != MXC_OK) goto done;
+        double lnz;
+        rc = ordering_lnz(n, Ap, Ai, cand, &lnz);
+        if (rc != MXC_OK) goto done;
+        L->method_lnz[2] = lnz;
+        if (lnz < best) {
+            L->selected = 2;
+            L->ordering = MXC_ORDER_METIS;
+            memcpy(Perm, cand, sizeof(Int) * (size_t)n);
+        }
+    }
+    rc = MXC_OK;
+done:
+    free(cand);
+    return rc;
+}
+
+// Analyze tril(A) under an ordering: MXC_ORDER_GIVEN (UserPerm),
+// MXC_ORDER_NATURAL, MXC_ORDER_AMD, MXC_ORDER_METIS or MXC_ORDER_DEFAULT.
 int mxc_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, int ordering,
                 const int64_t *UserPerm, mxc_symbolic **out) {
     if (!out || n < 0 || (n > 0 && (!Ap || !Ai))) return MXC_INVALID;
@@ -494,6 +589,9 @@ int mxc_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, int ordering,
     if (!L) return MXC_OUT_OF_MEMORY;
     L->n = n;
     L->ordering = ordering;
+    L->selected = 0;
+    for (int m = 0; m < 3; m++) L->method_fl[m] = L->method_lnz[m] = -1;
+    L->anz = -1;
     Int *Perm = (Int *)xmalloc((size_t)n, sizeof(Int));
     Int *Pinv = (Int *)xmalloc((size_t)n, sizeof(Int));
     Int *Parent = (Int *)xmalloc((size_t)n, sizeof(Int));
@@ -505,18 +603,20 @@ int mxc_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, int ordering,
     if (!Perm || !Pinv || !Parent || !Post || !ColCount || !Wk) goto fail;
 
     if (ordering == MXC_ORDER_GIVEN) {
-        if (!UserPerm) { rc = MXC_INVALID; goto fail; }
-        for (Int k = 0; k < n; k++) Pinv[k] = EMPTY;
-        for (Int k = 0; k < n; k++) {
-            Int j = UserPerm[k];
-            if (j < 0 || j >= n || Pinv[j] != EMPTY) { rc = MXC_INVALID; goto fail; }
-            Pinv[j] = k;
-            Perm[k] = j;
-        }
+        rc = given_perm(n, UserPerm, Perm, Pinv);
+        if (rc != MXC_OK) goto fail;
     } else if (ordering == MXC_ORDER_NATURAL) {
         for (Int k = 0; k < n; k++) Perm[k] = k;
     } else if (ordering == MXC_ORDER_AMD) {
-        rc = mxc_amd_order(n, Ap, Ai, Perm);
+        double st[3];
+        rc = mxc_amd_order_stats(n, Ap, Ai, Perm, st);
+        if (rc != MXC_OK) goto fail;
+        L->anz = st[2];
+    } else if (ordering == MXC_ORDER_METIS) {
+        rc = mxc_metis_order(n, Ap, Ai, 3000, 0.66, Perm);
+        if (rc != MXC_OK) goto fail;
+    } else if (ordering == MXC_ORDER_DEFAULT) {
+        rc = default_strategy(n, Ap, Ai, UserPerm, Perm, Wk, L);
         if (rc != MXC_OK) goto fail;
     } else {
         rc = MXC_INVALID;
@@ -591,6 +691,18 @@ int mxc_symbolic_info(const mxc_symbolic *L, int64_t *info, double *dinfo) {
     info[5] = L->maxesize;
     info[6] = L->ordering;
     if (dinfo) { dinfo[0] = L->fl; dinfo[1] = L->lnz; }
+    return MXC_OK;
+}
+
+int mxc_symbolic_strategy(const mxc_symbolic *L, int64_t *selected, double *method_fl,
+                          double *method_lnz, double *anz) {
+    if (!L) return MXC_INVALID;
+    if (selected) *selected = L->selected;
+    for (int m = 0; m < 3; m++) {
+        if (method_fl) method_fl[m] = L->method_fl[m];
+        if (method_lnz) method_lnz[m] = L->method_lnz[m];
+    }
+    if (anz) *anz = L->anz;
     return MXC_OK;
 }
 
@@ -692,6 +804,8 @@ int mxc_symbolic_import(int64_t n, int64_t nsuper, const int64_t *Perm, const in
     L->n = n;
     L->nsuper = nsuper;
     L->ordering = MXC_ORDER_GIVEN;
+    for (int m = 0; m < 3; m++) L->method_fl[m] = L->method_lnz[m] = -1;
+    L->anz = -1;
     L->ssize = Lpi[nsuper] > 1 ? Lpi[nsuper] : 1;
     L->xsize = Lpx[nsuper] > 1 ? Lpx[nsuper] : 1;
     L->Perm = (Int *)xmalloc((size_t)n, sizeof(Int));

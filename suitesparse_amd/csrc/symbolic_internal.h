@@ -19,6 +19,10 @@ struct mxc_symbolic {
     int ordering;
     double fl;
     double lnz;
+    // ordering strategy record (Common->selected, method[0..2].fl/lnz, anz):
+    // slot 0 GIVEN, 1 AMD (AMD's own predictions), 2 METIS; -1 = not tried
+    int selected;
+    double method_fl[3], method_lnz[3], anz;
     int64_t *Perm;      // n: Perm[k] = original index of pivot k
     int64_t *ColCount;  // n (NULL after import)
     int64_t *Parent;    // n, etree of the permuted matrix (NULL after import)

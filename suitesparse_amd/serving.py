@@ -21,8 +21,9 @@ pattern, and later ones wait for it instead of building a second plan.
 Plans are freed explicitly by the server (eviction, a failed build, stop),
 never by a finalizer on an arbitrary thread.
 
-The ordering is AMD for every request (the product's own restatement,
-bit-exact with cholmod_l_analyze's AMD).
+The ordering is AMD for 2D grids and METIS for 3D grids (the product's own
+restatements, bit-exact with cholmod_l_analyze's AMD and METIS); the request's
+pattern key names its kind.
 """
 from __future__ import annotations
 
@@ -78,9 +79,9 @@ def plan_bytes_estimate(xsize: int, nsuper_cols_sq: int, nnz: int, n: int, nrhs:
 
 
 class DeviceBackend:
-    """Plans on the MI355X: AMD analysis + device plan per pattern; each
-    request is a refactorization (graph replay after the first) and a solve
-    with FP64 iterative refinement."""
+    """Plans on the MI355X: analysis (AMD for 2D patterns, METIS for 3D) +
+    device plan per pattern; each request is a refactorization (graph replay
+    after the first) and a solve with FP64 iterative refinement."""
 
     def __init__(self, precision: str = "fp32", tol: float = 1e-10, maxiter: int = 20):
         self.precision, self.tol, self.maxiter = precision, tol, maxiter
@@ -98,7 +99,8 @@ class DeviceBackend:
         return plan_bytes_estimate(xguess, 64 * req.A.n, nnz, req.A.n, req.B.shape[1], self.precision)
 
     def build(self, req: Request):
-        cm = cholmod.Common(ordering=cholmod.CHOLMOD_AMD, precision=self.precision)
+        ordering = cholmod.CHOLMOD_METIS if req.key.startswith("3d") else cholmod.CHOLMOD_AMD
+        cm = cholmod.Common(nmethods=1, ordering=ordering, precision=self.precision)
         L = cholmod.analyze(req.A, cm)
         if L is None:
             raise RuntimeError(f"analyze failed: {cm.status}")

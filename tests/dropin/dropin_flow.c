@@ -31,6 +31,8 @@ typedef struct {
     double rcond, resid, resid_ir, fl, lnz;
     double bset_err, solve2_err;   /* max |difference| / max |x| */
     int ordering, is_super_after_bset;
+    int selected, called_nd;             /* Common->selected, called_nd after analyze */
+    double anz, method_lnz[3];           /* Common->anz, method[0..2].lnz */
 } side;
 
 static double norm_inf_dense(cholmod_dense *X) {
@@ -74,6 +76,10 @@ static int run(side *s, const char *path, int gpu, int supernodal) {
     s->fl = s->c.fl;
     s->lnz = s->c.lnz;
     s->ordering = s->L->ordering;
+    s->selected = s->c.selected;
+    s->called_nd = s->c.called_nd;
+    s->anz = s->c.anz;
+    for (int m = 0; m < 3; m++) s->method_lnz[m] = s->c.method[m].lnz;
     for (int t = 0; t < 3; t++) cholmod_l_factorize(s->A, s->L, &s->c);
     s->status = s->c.status;
     s->minor = (int)s->L->minor;
@@ -148,6 +154,8 @@ static void report(const char *tag, side *s) {
            s->lnz, tag, s->rcond, tag, s->resid, tag, s->resid_ir);
     printf("%s_bset_err %.6e\n%s_solve2_err %.6e\n%s_is_super_after_bset %d\n", tag, s->bset_err, tag,
            s->solve2_err, tag, s->is_super_after_bset);
+    printf("%s_selected %d\n%s_called_nd %d\n%s_anz %.17g\n", tag, s->selected, tag, s->called_nd, tag, s->anz);
+    printf("%s_method_lnz %.17g %.17g %.17g\n", tag, s->method_lnz[0], s->method_lnz[1], s->method_lnz[2]);
     printf("%s_gpu_launches %d\n%s_gpu_time %.6e\n%s_pool_bytes %.0f\n", tag, s->c.gpuNumKernelLaunches, tag,
            s->c.gpuKernelTime, tag, (double)s->c.dev_mempool_size);
     printf("%s_gpu_calls %.0f %.0f %.0f %.0f\n", tag, (double)s->c.cholmod_gpu_syrk_calls,

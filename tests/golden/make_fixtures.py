@@ -2,10 +2,10 @@
 
 Runs in the build container (needs oracle/_ref/libssref.so, built from the
 reference sources by oracle/ref.mk).  For each benchmark workload it records
-the reference's final fill-reducing permutation (L->Perm after the weighted
-postorder, so analyzing with it as a GIVEN ordering reproduces the reference's
-symbolic factor) and the symbolic scalars and SHA-256 digests of
-Super/Lpi/Lpx/Ls.  Small cases also keep the reference's factor and solution.
+the symbolic scalars and the SHA-256 digests of Perm/ColCount/Super/Lpi/Lpx/Ls
+of the reference's analysis (AMD or METIS), which the product's own ordering
+and analysis must reproduce.  Small cases also keep the reference's factor and
+solution.
 
     python tests/golden/make_fixtures.py
 """
@@ -53,7 +53,6 @@ def main():
         rec, s = symbolic_record(F)
         rec["ordering"] = {ref.AMD: "amd", ref.METIS: "metis"}[ordering]
         rec["t_analyze_s"] = F.t_analyze
-        np.savez_compressed(HERE / f"perm_{name}.npz", perm=s["Perm"].astype(np.int32))
         out[name] = rec
         del F
     (HERE / "symbolic_golden.json").write_text(json.dumps(out, indent=1))

@@ -20,7 +20,7 @@ def test_autotune_then_factor_matches_reference(gpu, ref_lib, precision):
     sym = R.symbolic()
     assert R.factorize() == 0
     Lref = R.Lx()
-    cm = cholmod.Common(precision=precision)
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision=precision)
     L = cholmod.analyze_p(A, sym["Perm"], cm)
     L._numeric(cm)
     table = L.autotune(reps=1)

@@ -30,7 +30,7 @@ def test_reference_matrix_fp64_and_fp32ir(gpu, ref_lib, name):
     R = ref_lib.RefFactor(A, ref_lib.AMD)
     assert R.factorize() == 0
     sym = R.symbolic()
-    cm = cholmod.Common(ordering=cholmod.CHOLMOD_AMD)
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_AMD)
     L = cholmod.analyze(A, cm)
     for k, mine in (("Perm", L.Perm), ("Super", L.super), ("Lpi", L.pi), ("Lpx", L.px),
                     ("Ls", L.s)):
@@ -47,7 +47,7 @@ def test_reference_matrix_fp64_and_fp32ir(gpu, ref_lib, name):
     resid = np.abs(Af @ X - b).max() / (anorm * np.abs(X).max() + np.abs(b).max())
     assert resid < 1e-7          # the demo's pass criterion
     assert resid < 1e-13
-    cm32 = cholmod.Common(ordering=cholmod.CHOLMOD_AMD, precision="fp32")
+    cm32 = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_AMD, precision="fp32")
     L32 = cholmod.analyze(A, cm32)
     assert cholmod.factorize(A, L32, cm32) and cm32.status == cholmod.CHOLMOD_OK
     X32, info = cholmod.solve_refine(L32, b, cm32, tol=1e-10, maxiter=20)
@@ -63,7 +63,7 @@ def test_reference_matrix_not_posdef(gpu, ref_lib, name):
     A = _load(name)
     R = ref_lib.RefFactor(A, ref_lib.AMD)
     st = R.factorize()
-    cm = cholmod.Common(ordering=cholmod.CHOLMOD_AMD)
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_AMD)
     L = cholmod.analyze(A, cm)
     assert cholmod.factorize(A, L, cm)
     assert cm.status == st == cholmod.CHOLMOD_NOT_POSDEF

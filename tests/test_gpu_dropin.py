@@ -5,7 +5,8 @@ tests/dropin/dropin_flow.c runs the reference demo's sequence
 rcond, solve, solve2 with reused workspaces, solve2 with a sparse right-hand
 side, one step of refinement) once with Common->useGPU = 1 and once with 0.
 
-Checked per matrix: the same ordering (Perm), status and L->minor; the
+Checked per matrix: the same ordering (Perm, the method CHOLMOD's default
+strategy selected, its fl / lnz and the per-method lnz), status and L->minor; the
 factors within 1e-12 * max|L| of each other; x within 1e-10 (relative); the
 demo's residual below 1e-13 (its own pass bar is 1e-7); the Bset solutions
 equal to the full solve on Xset (relative to max |x|); L left simplicial after a Bset solve, as the
@@ -28,6 +29,10 @@ def _check(v, supernodal_expected=None):
     assert v["gpu_status"] == v["host_status"], (v["gpu_status"], v["host_status"])
     assert v["gpu_minor"] == v["host_minor"]
     assert v["perm_equal"] == "1"
+    # the same ordering strategy: method chosen, fill, work and what the
+    # orderings left in Common (METIS included, cholmod_analyze.c:554-782)
+    for k in ("ordering", "selected", "called_nd", "anz", "method_lnz", "fl", "lnz"):
+        assert v[f"gpu_{k}"] == v[f"host_{k}"], (k, v[f"gpu_{k}"], v[f"host_{k}"])
     assert v["gpu_is_super"] == v["host_is_super"]
     assert v["gpu_nsuper"] == v["host_nsuper"]
     if supernodal_expected is not None:
@@ -79,11 +84,18 @@ def _write_mm(path, A, upper=False):
         np.savetxt(f, np.column_stack([r + 1, c + 1, A.x]), fmt="%d %d %.17g")
 
 
-@pytest.mark.parametrize("kind", ["grid3d_22_lower", "grid2d_150_upper"])
+@pytest.mark.parametrize("kind", ["grid3d_22_lower", "grid2d_150_upper", "stencil27_22_lower"])
 def test_larger_matrix_through_dropin(gpu, tmp_path, kind):
+    """The default strategy stops after AMD on the first two; on the 27-point
+    grid AMD predicts more than 500 flops per entry of L, so METIS runs too."""
     from suitesparse_amd import problems
     rng = np.random.default_rng(17)
-    A = problems.grid3d_lognormal(22, rng) if kind.startswith("grid3d") else problems.grid2d_lognormal(150, rng)
+    if kind.startswith("grid3d"):
+        A = problems.grid3d_lognormal(22, rng)
+    elif kind.startswith("stencil27"):
+        A = problems.stencil3d_27pt(22, seed=18)
+    else:
+        A = problems.grid2d_lognormal(150, rng)
     path = tmp_path / f"{kind}.mtx"
     _write_mm(path, A, upper=kind.endswith("upper"))
     print(f"case {kind} n={A.n}", flush=True)
@@ -91,3 +103,5 @@ def test_larger_matrix_through_dropin(gpu, tmp_path, kind):
     print(out.stdout[-1500:], flush=True)
     assert out.returncode == 0, out.stderr[-2000:]
     _check(v, supernodal_expected=True)
+    if kind.startswith("stencil27"):
+        assert v["host_called_nd"] == "1"

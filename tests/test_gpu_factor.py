@@ -51,7 +51,7 @@ def test_factor_solve_fp64(name, make, ordering, gpu, ref_lib):
     B = rng.uniform(-1, 1, (A.n, 2))
     Xref = R.solve(B)
 
-    cm = cholmod.Common(precision="fp64")
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision="fp64")
     L = cholmod.analyze_p(A, sym["Perm"], cm)
     assert L is not None and cm.status == 0
     assert np.array_equal(L.s, sym["Ls"]) and np.array_equal(L.px, sym["Lpx"])
@@ -76,7 +76,7 @@ def test_factor_fp32_refine(name, make, ordering, gpu, ref_lib):
     sym = R.symbolic()
     R.factorize()
     Lref = R.Lx()
-    cm = cholmod.Common(precision="fp32")
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision="fp32")
     L = cholmod.analyze_p(A, sym["Perm"], cm)
     assert cholmod.factorize(A, L, cm), cm.status
     err = np.abs(L.x - Lref).max() / np.abs(Lref).max()
@@ -122,7 +122,7 @@ def test_fp32_breakdown_refactored_in_fp64(gpu):
     values, and refinement through the FP32 factor then reaches 1e-10 with no
     whole-matrix FP64 factor."""
     A = _near_singular_pairs(200, 1e-9)
-    cm = cholmod.Common(precision="fp32")
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision="fp32")
     L = cholmod.analyze_p(A, np.arange(A.n), cm)
     assert cholmod.factorize(A, L, cm) and cm.status == 0 and L.minor == A.n
     inf = L.info()
@@ -147,7 +147,7 @@ def test_fp32_without_redo_falls_back_to_fp64(gpu):
     stalls, and mxc_solve_refine builds, factors and refines with an FP64
     factor of the same values."""
     A = _near_singular_pairs(200, 1e-9)
-    cm = cholmod.Common(precision="fp32")
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision="fp32")
     L = cholmod.analyze_p(A, np.arange(A.n), cm)
     L._numeric(cm)
     L.set_pivot_rho(0.0)
@@ -200,7 +200,7 @@ def test_fp32_breakdown_refactors_block_in_fp64(gpu, case):
     FP32 factor agrees to FP32 rounding of L's entries."""
     sizes = {"dense64": [64], "dense200": [200], "small12": [12] * 24, "tiny5": [5] * 40}[case]
     A = _pair_blocks(sizes, 1e-9, seed=len(sizes))
-    cm = cholmod.Common(precision="fp32")
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision="fp32")
     L = cholmod.analyze_p(A, np.arange(A.n), cm)
     assert cholmod.factorize(A, L, cm) and cm.status == 0 and L.minor == A.n
     inf = L.info()
@@ -214,7 +214,7 @@ def test_fp32_breakdown_refactors_block_in_fp64(gpu, case):
     assert info["relres"] <= 1e-10
     R = B - A.matvec(X)
     assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) <= 1e-10
-    cm64 = cholmod.Common(precision="fp64")
+    cm64 = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision="fp64")
     L64 = cholmod.analyze_p(A, np.arange(A.n), cm64)
     assert cholmod.factorize(A, L64, cm64) and cm64.status == 0
     err = np.abs(L.x - L64.x).max() / np.abs(L64.x).max()

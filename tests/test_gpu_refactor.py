@@ -18,8 +18,8 @@ def test_graph_replay_matches_eager(gpu, ref_lib):
     A0 = problems.grid3d_lognormal(14, rng)
     R = ref_lib.RefFactor(A0, ref_lib.METIS)
     perm = R.symbolic()["Perm"]
-    cmg = cholmod.Common(precision="fp64")
-    cme = cholmod.Common(precision="fp64")
+    cmg = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision="fp64")
+    cme = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision="fp64")
     Lg = cholmod.analyze_p(A0, perm, cmg)
     Le = cholmod.analyze_p(A0, perm, cme)
     B = rng.uniform(-1, 1, (A0.n, 2))
@@ -53,7 +53,7 @@ def test_not_posdef_matches_reference(gpu, ref_lib):
         Lref = R.Lx()
         print(f"case side={side} bad={bad} ref minor={R.minor}", flush=True)
         for prec in ("fp64",):
-            cm = cholmod.Common(precision=prec)
+            cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision=prec)
             L = cholmod.analyze_p(A2, sym["Perm"], cm)
             assert cholmod.factorize(A2, L, cm)
             assert cm.status == cholmod.CHOLMOD_NOT_POSDEF
@@ -86,7 +86,7 @@ def test_not_posdef_inside_a_supernode(gpu, ref_lib):
         Lref = R.Lx()
         print(f"case bad={bad} ref minor={R.minor}", flush=True)
         for prec in ("fp64", "fp32"):
-            cm = cholmod.Common(precision=prec)
+            cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision=prec)
             L = cholmod.analyze_p(As, sym["Perm"], cm)
             L._numeric(cm)
             if prec == "fp32":
@@ -104,7 +104,7 @@ def test_device_resident_inputs(gpu, ref_lib):
     R = ref_lib.RefFactor(A, ref_lib.METIS)
     sym = R.symbolic()
     R.factorize()
-    cm = cholmod.Common(precision="fp64")
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision="fp64")
     L = cholmod.analyze_p(A, sym["Perm"], cm)
     ax = torch.as_tensor(A.x, device="cuda:0")
     assert cholmod.factorize(A, L, cm, values=ax)
@@ -140,7 +140,7 @@ def test_rcond_matches_reference_definition(gpu, ref_lib):
     # diagonal, the value against the reference to FP32-factor accuracy (1e-4,
     # the tolerance of every FP32 factor entry in these tests)
     for prec, tol in (("fp64", 1e-12), ("fp32", 1e-4)):
-        cm = cholmod.Common(precision=prec)
+        cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision=prec)
         L = cholmod.analyze_p(A, sym["Perm"], cm)
         assert cholmod.factorize(A, L, cm)
         rc = cholmod.rcond(L, cm)
@@ -153,7 +153,7 @@ def test_rcond_matches_reference_definition(gpu, ref_lib):
     x = A.x.copy()
     x[A.p[:-1][10]] = -1.0
     A2 = problems.SymMatrix(n=A.n, p=A.p, i=A.i, x=x)
-    cm = cholmod.Common()
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN)
     L = cholmod.analyze_p(A2, sym["Perm"], cm)
     assert cholmod.factorize(A2, L, cm) and cm.status == cholmod.CHOLMOD_NOT_POSDEF
     assert cholmod.rcond(L, cm) == 0.0
@@ -171,7 +171,7 @@ def test_deterministic_factor_is_bitwise_repeatable(gpu, ref_lib, precision):
     sym = R.symbolic()
     assert R.factorize() == 0
     Lref = R.Lx()
-    cm = cholmod.Common(precision=precision)
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision=precision)
     L = cholmod.analyze_p(A, sym["Perm"], cm)
     L._numeric(cm)
     L.set_deterministic(True)

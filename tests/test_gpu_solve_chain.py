@@ -53,7 +53,7 @@ def test_chain_matches_steps(name, make, ordering, prec, nrhs, gpu, ref_lib):
     B = rng.uniform(-1, 1, (A.n, nrhs))
     Xref = R.solve(B)
 
-    cm = cholmod.Common(precision=prec)
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN, precision=prec)
     L = cholmod.analyze_p(A, sym["Perm"], cm)
     assert L is not None and cholmod.factorize(A, L, cm), cm.status
     L.set_solve_chain(False)
@@ -78,7 +78,7 @@ def test_chain_matches_steps(name, make, ordering, prec, nrhs, gpu, ref_lib):
 def test_chain_refine_fp32(gpu):
     """FP32 factor + FP64 refinement through the chained sweeps reaches 1e-10."""
     A = problems.stencil3d_27pt(20, seed=9)
-    cm = cholmod.Common(precision="fp32")
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_AMD, precision="fp32")
     L = cholmod.analyze(A, cm)
     assert cholmod.factorize(A, L, cm)
     rng = np.random.default_rng(4)

@@ -30,7 +30,7 @@ def _mat(dense_lower, name="m"):
 def _natural_sym(A):
     """Symbolic factor of A in the natural order (product analysis, CPU only)."""
     from suitesparse_amd import cholmod
-    cm = cholmod.Common()
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN)
     L = cholmod.analyze_p(A, np.arange(A.n), cm)
     assert L is not None
     return {"Perm": L.Perm, "Super": L.super, "Lpi": L.pi, "Lpx": L.px, "Ls": L.s}
@@ -71,7 +71,7 @@ def test_uniqueness_dense_lapack():
         A = problems.grid2d_lognormal(side, np.random.default_rng(seed))
         perm = np.random.default_rng(seed).permutation(A.n)
         from suitesparse_amd import cholmod
-        cm = cholmod.Common()
+        cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN)
         L = cholmod.analyze_p(A, perm, cm)
         sym = {"Perm": L.Perm, "Super": L.super, "Lpi": L.pi, "Lpx": L.px, "Ls": L.s}
         Lx, st, _ = restate.super_numeric(A, sym)
@@ -127,7 +127,7 @@ def test_residual_and_beta():
     A = problems.grid3d_lognormal(8, np.random.default_rng(8))
     perm = np.random.default_rng(8).permutation(A.n)
     from suitesparse_amd import cholmod
-    cm = cholmod.Common()
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN)
     L = cholmod.analyze_p(A, perm, cm)
     sym = {"Perm": L.Perm, "Super": L.super, "Lpi": L.pi, "Lpx": L.px, "Ls": L.s}
     beta = 0.75
@@ -161,7 +161,7 @@ def test_planted_mistakes_are_caught():
     A = problems.grid2d_lognormal(8, np.random.default_rng(5))
     perm = np.random.default_rng(5).permutation(A.n)
     from suitesparse_amd import cholmod
-    cm = cholmod.Common()
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN)
     L = cholmod.analyze_p(A, perm, cm)
     sym = {"Perm": L.Perm, "Super": L.super, "Lpi": L.pi, "Lpx": L.px, "Ls": L.s}
     Lx, _, _ = restate.super_numeric(A, sym)

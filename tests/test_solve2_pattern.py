@@ -44,7 +44,7 @@ def test_pattern_matches_reference(name, make, ordering, ref_lib):
     R = ref_lib.RefFactor(A, ordering)
     sym = R.symbolic()
     assert R.factorize() == 0
-    cm = cholmod.Common()
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_GIVEN)
     L = cholmod.analyze_p(A, sym["Perm"], cm)
     assert L is not None
     rng = np.random.default_rng(11)
@@ -61,7 +61,7 @@ def test_pattern_invariants():
     column tree of L (j + 1 inside a supernode, else the first row below it);
     the union over Bset entries holds each path once; CHOLMOD_D keeps Bset."""
     A = problems.laplace2d_5pt(20)
-    cm = cholmod.Common()
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_AMD)
     L = cholmod.analyze(A, cm)
     sup, pi, s = L.super, L.pi, L.s
     col_super = np.repeat(np.arange(L.nsuper), np.diff(sup))

@@ -21,7 +21,7 @@ def _call(fn, L):
 
 
 def _factor(A, precision):
-    cm = cholmod.Common(precision=precision)
+    cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_AMD, precision=precision)
     L = cholmod.analyze(A, cm)
     assert L is not None and cholmod.factorize(A, L, cm), cm.status
     return L, cm
