@@ -83,6 +83,13 @@ def lib():
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is not built: run python -m suitesparse_amd.build "
                                "(there is no CPU fallback)")
+        # torch ships its own HIP runtime, which shares the process's HSA
+        # runtime with this library's: loaded after this library, it finds
+        # ROCm's HSA already in place and sees no GPU.  Load torch first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in _PROTOS.items():
             fn = getattr(L, name)
