@@ -217,6 +217,11 @@ class Factor:
         per target supernode per launch, applied in descendant order (default off)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 6, int(bool(on))))
 
+    def set_fold_trsm(self, on: bool):
+        """FP64: the trailing launch that factors the next diagonal block also
+        applies its TRSM (default on); off: one TRSM launch per step."""
+        check("mxc_set_option", lib().mxc_set_option(self._num, 7, int(bool(on))))
+
     def set_graphs(self, on: bool):
         """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 1, int(bool(on))))

@@ -270,17 +270,31 @@ __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W
 // Arguments of the diagonal-block tail of a trailing launch (PD): after its
 // epilogue, the workgroup of tile (0, 0) of a block with pd >= 0 has the final
 // values of the next step's diagonal block and factors it (potrf_block).
+//
+// With a folded TRSM (UpdDesc::fold) the tail publishes the factored block:
+// ready[pd] is set once its L and inverse are in W (a device-scope release
+// after the workgroup's stores), and the tiles of column 0 below it wait for
+// that flag (device-scope acquire) before applying the inverse.  Producer
+// tiles come first in the launch (the first nfirst workgroups map to tiles
+// 0 .. nfirst-1, which hold them) and never wait, and workgroups start in
+// index order, so every wait ends; a wait is still bounded (FOLD_SPIN polls),
+// past which it raises *abort and the host reports a device error.
 struct PdArgs {
     const DiagTask *diags;
     int *fail;
     double rho;
     Redo rd;
+    int *ready;        // per diagonal task: 1 once a folding tail has published it
+    int *abort;        // set by a wait that gave up
+    int nfirst;        // producer tiles at the head of the launch (no XCD remap)
 };
+
+constexpr int FOLD_SPIN = 1 << 22;
 
 // The tail itself: S and X of potrf_block carved from the tile's LDS buffer.
 template <typename T>
 __device__ __forceinline__ void pd_tail(const PdArgs &pda, int pd, T *__restrict__ W, unsigned char *smem,
-                                        const double *__restrict__ adiag) {
+                                        const double *__restrict__ adiag, bool publish) {
     __shared__ int sfail[2];
     __syncthreads();                        // the tile's stores and LDS reads are done
     T *S = reinterpret_cast<T *>(smem);
@@ -290,6 +304,26 @@ __device__ __forceinline__ void pd_tail(const PdArgs &pda, int pd, T *__restrict
     } else {
         potrf_block<T, false>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rd, nullptr);
     }
+    if (publish) {
+        __syncthreads();                    // every wave's stores of L and the inverse are issued
+        if (threadIdx.x == 0) __hip_atomic_store(pda.ready + pd, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Wait (whole workgroup) until *flag != 0 or the wait gives up.
+__device__ __forceinline__ void fold_wait(const int *flag, int *abort) {
+    if ((threadIdx.x & 63) == 0) {
+        for (int it = 1; __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0; it++) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((it & 63) == 0 &&
+                (it >= FOLD_SPIN || __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the inverse's loads come after the flag
+    __syncthreads();
 }
 
 template <typename T, int MODE, int TS, bool PD = false>
@@ -315,14 +349,21 @@ __global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
     constexpr int NBUF = TS == 128 ? 2 : 1;
     constexpr size_t XY_BYTES = sizeof(T) * NBUF * BK * (TS + LDS_PAD);
     constexpr size_t PD_BYTES = PD ? sizeof(T) * 2 * NB * DS : 0;
-    constexpr size_t SMEM = 2 * XY_BYTES > PD_BYTES ? 2 * XY_BYTES : PD_BYTES;
+    // folded TRSM (FP64, 64-wide trailing tiles): the updated block and the
+    // next diagonal block's inverse, both k-major like the staged chunks
+    constexpr bool CAN_FOLD = PD && MODE == MODE_TRAIL && TS == NB && std::is_same<T, double>::value;
+    constexpr size_t FOLD_HALF = sizeof(T) * NB * (TS + LDS_PAD);
+    constexpr size_t FOLD_BYTES = CAN_FOLD ? 2 * FOLD_HALF : 0;
+    constexpr size_t SMEM0 = 2 * XY_BYTES > PD_BYTES ? 2 * XY_BYTES : PD_BYTES;
+    constexpr size_t SMEM = SMEM0 > FOLD_BYTES ? SMEM0 : FOLD_BYTES;
     __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
     T (*Xs)[BK][TS + LDS_PAD] = reinterpret_cast<T (*)[BK][TS + LDS_PAD]>(smem);
     T (*Ys)[BK][TS + LDS_PAD] = reinterpret_cast<T (*)[BK][TS + LDS_PAD]>(smem + XY_BYTES);
     __shared__ int relr[TS];
     __shared__ int relc[TS];
 
-    const Tile tl = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+    const int bx = (int)blockIdx.x, nf = PD ? pda.nfirst : 0;
+    const Tile tl = tiles[bx < nf ? bx : nf + xcd_tile(bx - nf, (int)gridDim.x - nf)];
     const UpdDesc d = descs[tl.desc];
     const int i0 = tl.ti * TS, j0 = tl.tj * TS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -403,6 +444,10 @@ __global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
         if constexpr (NBUF == 2) buf ^= 1;
     }
     if (MODE == MODE_STORE) __syncthreads();   // in-place TRSM: every read of the panel is done
+    const bool fold = CAN_FOLD && d.fold && d.pd >= 0 && tl.tj == 0 && tl.ti >= 1;
+    T (*Af)[TS + LDS_PAD] = reinterpret_cast<T (*)[TS + LDS_PAD]>(smem);               // Af[k][i]
+    T (*If)[TS + LDS_PAD] = reinterpret_cast<T (*)[TS + LDS_PAD]>(smem + FOLD_HALF);   // If[k][j]
+    if (CAN_FOLD && fold) __syncthreads();     // every wave is done with the staged chunks
 
     // FP32 TRSM with Lh: the panel is also stored in split-FP16 form, each row
     // scaled by its own exponent, for the update and trailing tiles that read it
@@ -436,7 +481,9 @@ __global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
                     atomic_sub(W + d.c_off + (long long)relc[jl] * d.ldc + relr[il], c);
                 } else if (MODE == MODE_TRAIL) {
                     if (i < j) continue;
-                    W[d.c_off + (long long)j * d.ldc + i] -= c;
+                    T *w = W + d.c_off + (long long)j * d.ldc + i;
+                    if (CAN_FOLD && fold) Af[jl][il] = *w - c;   // the next TRSM's input, kept here
+                    else *w -= c;
                 } else {
                     W[d.c_off + (long long)j * d.ldc + i] = c;
                     if (std::is_same<T, float>::value && Lh)
@@ -445,8 +492,47 @@ __global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
             }
         }
     }
+    if constexpr (CAN_FOLD) {
+        if (fold) {
+            // the next step's TRSM of these rows: L(i, j) = sum_k A(i, k) inv(j, k),
+            // with the MFMA sequence of the TRSM tile (k ascending)
+            const DiagTask dt = pda.diags[d.pd];
+            fold_wait(pda.ready + d.pd, pda.abort);
+            for (int e = tid; e < NB * NB; e += NT) If[e / NB][e % NB] = W[dt.winv + e];
+            __syncthreads();
+            acc_t a2[MB][MB];
+#pragma unroll
+            for (int a = 0; a < MB; a++)
+#pragma unroll
+                for (int b = 0; b < MB; b++) a2[a][b] = acc_t{0, 0, 0, 0};
+#pragma unroll 4
+            for (int kk = 0; kk < NB; kk += 4) {
+                T xa[MB], yb[MB];
+#pragma unroll
+                for (int m = 0; m < MB; m++) xa[m] = If[kk + (lane >> 4)][wc * WB + m * 16 + (lane & 15)];
+#pragma unroll
+                for (int n = 0; n < MB; n++) yb[n] = Af[kk + (lane >> 4)][wr * WB + n * 16 + (lane & 15)];
+#pragma unroll
+                for (int m = 0; m < MB; m++)
+#pragma unroll
+                    for (int n = 0; n < MB; n++) a2[m][n] = MfmaOp<T>::mma(xa[m], yb[n], a2[m][n]);
+            }
+#pragma unroll
+            for (int m = 0; m < MB; m++) {
+#pragma unroll
+                for (int n = 0; n < MB; n++) {
+                    const int i = i0 + wr * WB + n * 16 + (lane & 15);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int j = wc * WB + m * 16 + MfmaOp<T>::row(lane, v);
+                        if (i < d.M) W[d.c_off + (long long)j * d.ldc + i] = a2[m][n][v];
+                    }
+                }
+            }
+        }
+    }
     if constexpr (PD) {
-        if (d.pd >= 0 && tl.ti == 0 && tl.tj == 0) pd_tail<T>(pda, d.pd, W, smem, adiag);
+        if (d.pd >= 0 && tl.ti == 0 && tl.tj == 0) pd_tail<T>(pda, d.pd, W, smem, adiag, d.fold != 0);
     }
 }
 
@@ -630,7 +716,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_h(const UpdDesc *__restrict__ d
         }
     }
     if constexpr (PD) {
-        if (d.pd >= 0 && tl.ti == 0 && tl.tj == 0) pd_tail<float>(pda, d.pd, W, smem, adiag);
+        if (d.pd >= 0 && tl.ti == 0 && tl.tj == 0) pd_tail<float>(pda, d.pd, W, smem, adiag, false);
     }
 }
 

@@ -126,6 +126,7 @@ struct Launch {
     long long begin;   // first task
     long long count;   // number of tasks (grid size)
     int tile;          // k_tile: tile edge; sweep steps: number of diag workgroups
+    int nfirst = 0;    // trailing launches with folded TRSMs: producer tiles at the head
 };
 
 // Tile-size buckets over (rows M, columns N, inner dimension K) of a block:
@@ -214,6 +215,7 @@ struct StepBlocks {
     bool panel_end = false;        // trail holds the update of the next panel's columns
     std::vector<int64_t> diag_s, trail_s;   // supernode of each diag task / trail block
     std::vector<int64_t> trail2_s;          // supernode of each trail2 block
+    std::vector<int64_t> trsm_s;            // supernode of each TRSM block
 };
 
 struct mxc_factor {
@@ -230,6 +232,9 @@ struct mxc_factor {
     bool split16 = true;                 // FP32 factors: split-FP16 update/trailing tiles
     bool chain = true;                   // solves: one chained launch per level
     bool diag_tails = true;              // trailing launches factor the next diagonal blocks
+    bool fold_trsm = true;               // FP64: ... and apply the next step's TRSM (UpdDesc::fold)
+    int *d_ready = nullptr;              // per diagonal task: published by a folding tail; + abort flag
+    int64_t nready = 0;                  // diagonal tasks covered by d_ready
     bool merge_t2 = true;                // FP64: fold second-stream updates into later trailing launches
     bool deterministic = false;          // descendant updates in rounds: one pair per target per launch
     size_t bytes = 0;
@@ -436,9 +441,13 @@ static int flatten_upload(mxc_factor *F) {
     // fused[i] records that, so the next step's diagonal launch skips it.
     // force[i] > 0 sets block i's tile edge (merged second-stream slices take the
     // edge of their supernode's own trailing block, so they share its launch).
+    // foldable[i] (trailing blocks, FP64): the next step's TRSM of block i's
+    // supernode may be applied by block i's tiles (see UpdDesc::fold); folded[i]
+    // records that it was (64-wide tiles with a tail), so that TRSM is skipped.
     auto emit = [&](int kind, const std::vector<UpdDesc> &blocks, bool lower_only,
                     const std::vector<const DiagTask *> *follow = nullptr, std::vector<char> *fused = nullptr,
-                    const std::vector<int> *force = nullptr) {
+                    const std::vector<int> *force = nullptr, const std::vector<char> *foldable = nullptr,
+                    std::vector<char> *folded = nullptr) {
         // FP64 descendant updates of at most 16 rows and columns take 16-wide tiles when a
         // level holds very many of them (the bottom levels of a forest of small systems:
         // millions); a 3D grid's levels (at most ~30,000 at 64^3) keep 32-wide tiles, which
@@ -468,27 +477,41 @@ static int flatten_upload(mxc_factor *F) {
                 allow = F->precision != MXC_FP64 || cnt <= PD_MAX_TILES_FP64;
             }
             bool tails = false;
+            std::vector<Tile> producers;      // tile (0, 0) of each folding block: dispatched first
             for (size_t bi = 0; bi < blocks.size(); bi++) {
                 const UpdDesc &u = blocks[bi];
                 if (edge(bi) != ts) continue;
                 int di = (int)descs.size();
                 descs.push_back(u);
                 descs.back().pd = -1;
+                descs.back().fold = 0;
                 if (allow && bi < follow->size() && (*follow)[bi]) {
                     descs.back().pd = (int)diags.size();
                     diags.push_back(*(*follow)[bi]);
                     (*fused)[bi] = 1;
                     tails = true;
+                    if (ts == NB && foldable && bi < foldable->size() && (*foldable)[bi]) {
+                        descs.back().fold = 1;
+                        (*folded)[bi] = 1;
+                    }
                 }
+                const bool prod_first = descs.back().fold != 0;
+                if (prod_first) producers.push_back(Tile{di, 0, 0});
                 int64_t mt = ceil_div(u.M, ts), nt = ceil_div(u.N, ts);
                 for (int64_t ti = 0; ti < mt; ti++)
                     for (int64_t tj = 0; tj < nt; tj++) {
                         if (lower_only && ti < tj) continue;   // tile wholly above the diagonal
+                        if (prod_first && ti == 0 && tj == 0) continue;
                         tiles.push_back(Tile{di, (int)ti, (int)tj});
                     }
             }
+            if (!producers.empty()) tiles.insert(tiles.begin() + b, producers.begin(), producers.end());
             long long c = (long long)tiles.size() - b;
-            if (c > 0) F->factor_launches.push_back(Launch{tails ? (int)L_TRAILD : kind, b, c, ts});
+            if (c > 0) {
+                Launch L{tails ? (int)L_TRAILD : kind, b, c, ts};
+                L.nfirst = (int)producers.size();
+                F->factor_launches.push_back(L);
+            }
             total += c;
         }
         return total;
@@ -514,6 +537,7 @@ static int flatten_upload(mxc_factor *F) {
         }
         std::vector<int64_t> fused_s;              // supernodes whose diagonal block of this step is
                                                    // factored by the previous step's trailing launch
+        std::vector<int64_t> folded_s;             // ... and whose TRSM of this step it applied
         const size_t nst = F->blk_steps[l].size();
         // second-stream updates merged into later trailing launches: per step, the
         // column slices and their supernodes
@@ -541,7 +565,9 @@ static int flatten_upload(mxc_factor *F) {
                 // TRSM of supernodes with at most TINY_DIAG columns and TINY_TRSM_ROWS rows below: one lane each
                 std::vector<UpdDesc> big;
                 long long b = (long long)tiles.size();
-                for (const UpdDesc &u : sb.trsm) {
+                for (size_t q = 0; q < sb.trsm.size(); q++) {
+                    const UpdDesc &u = sb.trsm[q];
+                    if (std::binary_search(folded_s.begin(), folded_s.end(), sb.trsm_s[q])) continue;
                     if (u.K <= TINY_DIAG && u.M <= TINY_TRSM_ROWS) {
                         descs.push_back(u);
                         descs.back().pd = -1;
@@ -611,20 +637,28 @@ static int flatten_upload(mxc_factor *F) {
                     level_forked = true;
                 }
             }
-            // the next step's diagonal blocks, one per trailing block
+            // the next step's diagonal blocks, one per trailing block, and (FP64)
+            // whether its TRSM can be folded in: a full 64-column diagonal block
+            // with rows below it, all inside this trailing block
             std::vector<const DiagTask *> follow(sb.trail.size(), nullptr);
-            std::vector<char> fused(sb.trail.size(), 0);
+            std::vector<char> fused(sb.trail.size(), 0), foldable(sb.trail.size(), 0),
+                folded(sb.trail.size(), 0);
             if (F->diag_tails && t + 1 < F->blk_steps[l].size()) {
-                // both lists ascend by supernode
+                // the lists ascend by supernode
                 const StepBlocks &nx = F->blk_steps[l][t + 1];
-                size_t r = 0;
+                size_t r = 0, rt = 0;
                 for (size_t q = 0; q < sb.trail.size(); q++) {
                     while (r < nx.diags.size() && nx.diag_s[r] < sb.trail_s[q]) r++;
                     if (r < nx.diags.size() && nx.diag_s[r] == sb.trail_s[q]) follow[q] = &nx.diags[r];
+                    while (rt < nx.trsm.size() && nx.trsm_s[rt] < sb.trail_s[q]) rt++;
+                    const bool has_trsm = rt < nx.trsm.size() && nx.trsm_s[rt] == sb.trail_s[q];
+                    foldable[q] = F->fold_trsm && F->precision == MXC_FP64 && follow[q] &&
+                                  follow[q]->nb == NB && has_trsm && nx.trsm[rt].K == NB &&
+                                  sb.trail[q].N >= NB && sb.trail[q].M == nx.trsm[rt].M + NB;
                 }
             }
             if (merged[t].empty()) {
-                emit(L_TRAIL, sb.trail, true, &follow, &fused);
+                emit(L_TRAIL, sb.trail, true, &follow, &fused, nullptr, &foldable, &folded);
             } else {
                 // this step's trailing blocks, then the merged slices at their
                 // supernodes' edges
@@ -637,11 +671,14 @@ static int flatten_upload(mxc_factor *F) {
                     blocks.push_back(merged[t][q]);
                     force.push_back(e);
                 }
-                emit(L_TRAIL, blocks, true, &follow, &fused, &force);
+                emit(L_TRAIL, blocks, true, &follow, &fused, &force, &foldable, &folded);
             }
             fused_s.clear();
-            for (size_t q = 0; q < sb.trail.size(); q++)
+            folded_s.clear();
+            for (size_t q = 0; q < sb.trail.size(); q++) {
                 if (fused[q]) fused_s.push_back(sb.trail_s[q]);
+                if (folded[q]) folded_s.push_back(sb.trail_s[q]);
+            }
         }
         // the next level reads this level's columns: join the second stream
         if (level_forked) F->factor_launches.push_back(Launch{L_JOIN, 0, 0, 0});
@@ -662,6 +699,12 @@ static int flatten_upload(mxc_factor *F) {
     if ((rc = dupload(F, &F->d_desc, descs, F->stream))) return rc;
     if ((rc = dupload(F, &F->d_tiles, tiles, F->stream))) return rc;
     if ((rc = dupload(F, &F->d_diag, diags, F->stream))) return rc;
+    // publication flags of the diagonal tasks (folded TRSMs) + the wait-abort flag
+    ffree(F, F->d_ready);
+    F->d_ready = nullptr;
+    F->nready = 0;
+    if ((rc = dalloc(F, &F->d_ready, diags.size() + 1))) return rc;
+    F->nready = (int64_t)diags.size() + 1;
     if (F->precision != MXC_FP64) {
         // FP32 factors: room in the FP64 redo queue for every diagonal task
         ffree(F, F->d_redo);
@@ -869,6 +912,7 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
                     u.ldc = (int)nsrow;
                     u.ls_off = (int)(S->Lpi[s] + r0);   // global rows (split-FP16 copy)
                     sb.trsm.push_back(u);
+                    sb.trsm_s.push_back(s);
                     F->flops_trsm += (double)u.M * nb * nb;
                 }
                 // two-level blocking: a 64-column step updates the rest of its
@@ -1219,7 +1263,7 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
     void *ptrs[] = {F->W, F->d_Lh, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
                     F->d_diag, F->d_stask, F->d_fail, F->d_adiag, F->d_rowptr, F->d_src, F->d_col,
                     F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_part, F->d_b, F->d_x, F->d_r,
-                    F->d_cnt, F->d_abort, F->d_seg, F->d_segout, F->d_pinv, F->d_redo};
+                    F->d_cnt, F->d_abort, F->d_seg, F->d_segout, F->d_pinv, F->d_redo, F->d_ready};
     for (void *p : ptrs) ffree(F, p);
 
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
@@ -1482,7 +1526,8 @@ static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = n
     if (!st) st = F->stream;
     const bool h16 = std::is_same<T, float>::value && F->split16 && F->d_Lh;
     const double *adiag = F->d_adiag;
-    const PdArgs pda{F->d_diag, F->d_fail, redo_safe<T>(F) ? F->rho : 0.0, redo_args(F)};
+    const PdArgs pda{F->d_diag, F->d_fail, redo_safe<T>(F) ? F->rho : 0.0, redo_args(F), F->d_ready,
+                     F->d_ready ? F->d_ready + (F->nready - 1) : nullptr, L.nfirst};
     const dim3 g((unsigned)L.count);
     if constexpr (std::is_same<T, float>::value && MODE != MODE_STORE) {
         // FP32 factor: split-FP16 products on the FP16 MFMA, operands from the
@@ -1538,6 +1583,9 @@ static int enqueue_factor(mxc_factor *F) {
     // assembly: L = 0, scatter A, + beta on the diagonal; also refresh the CSR values
     tick(F, 0, &tt);
     hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(F->xsize, 256)), dim3(256), 0, sm, W, (long long)F->xsize, (T)0);
+    if (F->d_ready)
+        hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(F->nready, 256)), dim3(256), 0, sm, F->d_ready,
+                           (long long)F->nready, 0);
     hipLaunchKernelGGL(k_scatter_a<T>, dim3(grid_for(F->nnz_a, 256)), dim3(256), 0, sm, F->d_Ax,
                        F->d_pos, (long long)F->nnz_a, W);
     hipLaunchKernelGGL(k_add_diag<T>, dim3(grid_for(F->n, 256, 1LL << 30)), dim3(256), 0, sm,
@@ -1702,9 +1750,15 @@ extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, doub
         return rc;
     }
     HIPTRY(hipEventRecord(F->ev_f1, sm));
-    int flags[3] = {0, 0, 0};
+    int flags[3] = {0, 0, 0}, wait_abort = 0;
     HIPTRY(hipMemcpyAsync(flags, F->d_fail, sizeof(int) * 3, hipMemcpyDeviceToHost, sm));
+    if (F->d_ready)
+        HIPTRY(hipMemcpyAsync(&wait_abort, F->d_ready + (F->nready - 1), sizeof(int), hipMemcpyDeviceToHost, sm));
     HIPTRY(hipStreamSynchronize(sm));
+    if (wait_abort) {
+        snprintf(g_err, sizeof(g_err), "factorization: a folded TRSM gave up waiting for its diagonal block");
+        return MXC_GPU_PROBLEM;
+    }
     {
         float t = 0;
         if (hipEventElapsedTime(&t, F->ev_f0, F->ev_f1) == hipSuccess) {
@@ -2548,6 +2602,10 @@ extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
             if (F->deterministic == (value != 0)) return MXC_OK;
             F->deterministic = value != 0;
             return flatten_upload(F);           // new launch list (drops the factor graph)
+        case MXC_OPT_FOLD_TRSM:
+            if (F->fold_trsm == (value != 0)) return MXC_OK;
+            F->fold_trsm = value != 0;
+            return flatten_upload(F);
         default:
             return MXC_INVALID;
     }

@@ -36,6 +36,9 @@ struct UpdDesc {
     int ls_off, base;
     int t_psi, t_nscol, t_nsrow;
     int pd;            // trailing blocks: diagonal task factored after tile (0, 0), or -1
+    int fold;          // trailing blocks with pd >= 0 (FP64, 64-wide tiles): the tiles of
+                       // column 0 below row 64 also apply the next step's TRSM (by the
+                       // inverse of diagonal task pd, once tile (0, 0) has published it)
 };
 
 struct Tile {

@@ -192,3 +192,54 @@ def test_deterministic_factor_is_bitwise_repeatable(gpu, ref_lib, precision):
     d = np.abs(L.x - xs[0]).max() / np.abs(xs[0]).max()
     print(f"  atomic vs deterministic assembly: {d:.3e}", flush=True)
     assert d <= (1e-13 if precision == "fp64" else 1e-5)
+
+
+@pytest.mark.parametrize("kind", ["diffusion3d_30_metis", "stencil27_18_amd"])
+def test_folded_trsm_is_bitwise_the_separate_trsm(gpu, ref_lib, kind):
+    """MXC_OPT_FOLD_TRSM (FP64): the trailing launch whose tile (0, 0) factors
+    the next diagonal block also applies that block's TRSM in its first tile
+    column, after waiting for the block's inverse.  Its MFMA sequence is the
+    TRSM tile's, so with the deterministic assembly the factor is bit for bit
+    the one with separate TRSM launches, replayed or eager; the fold takes
+    launches away; and the factor is the reference's to 1e-12."""
+    from suitesparse_amd import _lib
+    if kind.startswith("diffusion"):
+        A = problems.diffusion3d_7pt(30, seed=12)
+        ordering = cholmod.CHOLMOD_METIS
+    else:
+        A = problems.stencil3d_27pt(18, seed=13)
+        ordering = cholmod.CHOLMOD_AMD
+    print(f"case {kind} n={A.n}", flush=True)
+    cm = cholmod.Common(nmethods=1, ordering=ordering, precision="fp64")
+    L = cholmod.analyze(A, cm)
+    assert L is not None
+    L._numeric(cm)
+    L.set_deterministic(True)
+
+    def trsm_launches():
+        counts, ms = np.zeros(8, dtype=np.int64), np.zeros(4)
+        _lib.check("mxc_factor_stats", _lib.lib().mxc_factor_stats(L._num, cholmod._ptr(counts),
+                                                                    cholmod._ptr(ms)))
+        return int(counts[2])
+
+    xs, launches = {}, {}
+    for fold in (False, True):
+        L.set_fold_trsm(fold)
+        launches[fold] = trsm_launches()
+        runs = []
+        for graphs in (True, True, False):
+            L.set_graphs(graphs)
+            assert cholmod.factorize(A, L, cm) and cm.status == 0
+            runs.append(L.x)
+        for x in runs[1:]:
+            assert np.array_equal(x, runs[0])
+        xs[fold] = runs[0]
+    print(f"  TRSM launches: separate {launches[False]}, folded {launches[True]}", flush=True)
+    assert launches[True] < launches[False]
+    assert np.array_equal(xs[True], xs[False])
+    R = ref_lib.RefFactor(A, ref_lib.GIVEN, perm=L.Perm)
+    assert R.factorize() == 0
+    Lref = R.Lx()
+    err = np.abs(xs[True] - Lref).max() / np.abs(Lref).max()
+    print(f"  folded factor vs reference {err:.3e}", flush=True)
+    assert err <= 1e-12
