@@ -232,7 +232,7 @@ struct mxc_factor {
     bool split16 = true;                 // FP32 factors: split-FP16 update/trailing tiles
     bool chain = true;                   // solves: one chained launch per level
     bool diag_tails = true;              // trailing launches factor the next diagonal blocks
-    bool fold_trsm = true;               // FP64: ... and apply the next step's TRSM (UpdDesc::fold)
+    bool fold_trsm = false;              // FP64: ... and apply the next step's TRSM (UpdDesc::fold)
     int *d_ready = nullptr;              // per diagonal task: published by a folding tail; + abort flag
     int64_t nready = 0;                  // diagonal tasks covered by d_ready
     bool merge_t2 = true;                // FP64: fold second-stream updates into later trailing launches
