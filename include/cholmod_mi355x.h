@@ -302,10 +302,12 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * as the most descendants of any of its supernodes.  The solve sweeps are not
  * affected (the forward sweep adds ancestor contributions atomically). */
 #define MXC_OPT_DETERMINISTIC 6
-/* MXC_OPT_FOLD_TRSM (default 1, FP64 factors): a trailing launch whose tile
+/* MXC_OPT_FOLD_TRSM (default 0, FP64 factors): a trailing launch whose tile
  * (0, 0) factors the next 64-column diagonal block also applies that block's
  * TRSM to the rows below it (the tiles of its first column wait for the
- * block's inverse), instead of a separate TRSM launch per step. */
+ * block's inverse), instead of a separate TRSM launch per step.  Bitwise the
+ * same factor; off by default because the waiting tiles hold their CUs
+ * through the diagonal factorization, which costs more than the launch. */
 #define MXC_OPT_FOLD_TRSM 7
 int mxc_set_option(mxc_factor *F, int option, int64_t value);
 

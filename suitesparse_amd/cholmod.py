@@ -219,7 +219,7 @@ class Factor:
 
     def set_fold_trsm(self, on: bool):
         """FP64: the trailing launch that factors the next diagonal block also
-        applies its TRSM (default on); off: one TRSM launch per step."""
+        applies its TRSM (default off: measured slower); off: one TRSM launch per step."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 7, int(bool(on))))
 
     def set_graphs(self, on: bool):

@@ -265,7 +265,7 @@ __device__ __forceinline__ void redo_block64(const DiagTask &t, T *__restrict__ 
 template <typename T, bool SAFE>
 __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W, T *S, T *X, int *sfail,
                                             int *__restrict__ fail, const double *__restrict__ adiag,
-                                            double rho, const Redo &rd, double *tmp, bool preloaded = false);
+                                            double rho, const Redo &rd, double *tmp);
 
 // Arguments of the diagonal-block tail of a trailing launch (PD): after its
 // epilogue, the workgroup of tile (0, 0) of a block with pd >= 0 has the final
@@ -292,21 +292,17 @@ struct PdArgs {
 constexpr int FOLD_SPIN = 1 << 22;
 
 // The tail itself: S and X of potrf_block carved from the tile's LDS buffer.
-// preloaded: the tile's epilogue already left the block in S (lower part,
-// zero above, identity on the padding) and zeros in X.
 template <typename T>
 __device__ __forceinline__ void pd_tail(const PdArgs &pda, int pd, T *__restrict__ W, unsigned char *smem,
-                                        const double *__restrict__ adiag, bool publish, bool preloaded = false) {
+                                        const double *__restrict__ adiag, bool publish) {
     __shared__ int sfail[2];
     __syncthreads();                        // the tile's stores and LDS reads are done
     T *S = reinterpret_cast<T *>(smem);
     if (pda.rho > 0) {
         __shared__ double tmp[NB];
-        potrf_block<T, true>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rd, tmp,
-                             preloaded);
+        potrf_block<T, true>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rd, tmp);
     } else {
-        potrf_block<T, false>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rd, nullptr,
-                              preloaded);
+        potrf_block<T, false>(pda.diags[pd], W, S, S + NB * DS, sfail, pda.fail, adiag, pda.rho, pda.rd, nullptr);
     }
     if (publish) {
         __syncthreads();                    // every wave's stores of L and the inverse are issued
@@ -452,20 +448,6 @@ __global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
     T (*Af)[TS + LDS_PAD] = reinterpret_cast<T (*)[TS + LDS_PAD]>(smem);               // Af[k][i]
     T (*If)[TS + LDS_PAD] = reinterpret_cast<T (*)[TS + LDS_PAD]>(smem + FOLD_HALF);   // If[k][j]
     if (CAN_FOLD && fold) __syncthreads();     // every wave is done with the staged chunks
-    // the diagonal-block tail: the epilogue also leaves the block in LDS for
-    // potrf_block (S: lower part, zero above, identity on the padding; X: 0)
-    const bool tail = PD && d.pd >= 0 && tl.ti == 0 && tl.tj == 0;
-    T *Sd = reinterpret_cast<T *>(smem);
-    int nbd = 0;
-    if constexpr (PD) {
-        if (tail) {
-            nbd = pda.diags[d.pd].nb;
-            __syncthreads();                   // every wave is done with the staged chunks
-            for (int e = tid; e < 2 * NB * DS; e += NT) Sd[e] = (T)0;
-            __syncthreads();
-            for (int i = nbd + tid; i < NB; i += NT) Sd[i * DS + i] = (T)1;
-        }
-    }
 
     // FP32 TRSM with Lh: the panel is also stored in split-FP16 form, each row
     // scaled by its own exponent, for the update and trailing tiles that read it
@@ -500,13 +482,8 @@ __global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
                 } else if (MODE == MODE_TRAIL) {
                     if (i < j) continue;
                     T *w = W + d.c_off + (long long)j * d.ldc + i;
-                    if (CAN_FOLD && fold) {
-                        Af[jl][il] = *w - c;   // the next TRSM's input, kept here
-                    } else {
-                        const T v = *w - c;
-                        *w = v;
-                        if (PD && tail && i < nbd && j < nbd) Sd[il * DS + jl] = v;
-                    }
+                    if (CAN_FOLD && fold) Af[jl][il] = *w - c;   // the next TRSM's input, kept here
+                    else *w -= c;
                 } else {
                     W[d.c_off + (long long)j * d.ldc + i] = c;
                     if (std::is_same<T, float>::value && Lh)
@@ -555,7 +532,7 @@ __global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
         }
     }
     if constexpr (PD) {
-        if (tail) pd_tail<T>(pda, d.pd, W, smem, adiag, d.fold != 0, true);
+        if (d.pd >= 0 && tl.ti == 0 && tl.tj == 0) pd_tail<T>(pda, d.pd, W, smem, adiag, d.fold != 0);
     }
 }
 
@@ -828,12 +805,9 @@ __device__ __forceinline__ void store16(T *S, int r0, int c0, const typename Mfm
 // set when a pivot falls to rho * a_jj or below (a_jj the assembled diagonal
 // of A): the block's result is then not used (k_potrf_redo64 refactors it).
 //
-// The serial path is one pivot after another, so each step broadcasts column
-// j (a_cj, c > j) before its pivot is known, while the reciprocal square root
-// runs, and updates with a_ic -= (a_ij r_j^2) a_cj: only lane j+1's entry of
-// column j+1 waits for the pivot.  The inverse is formed column by column
-// (lane c: x_i = r_i (delta_ic - sum_{k<i} L_ik x_k)) with each partial sum
-// updated as soon as x_k is final: a chain of 16 products, not 120.
+// The inverse is formed column by column (lane c: x_i = r_i (delta_ic -
+// sum_{k<i} L_ik x_k)) with each partial sum updated as soon as x_k is
+// final, the same products in the same order as the row-by-row form.
 template <typename T, bool SAFE>
 __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, int lane,
                                             const double *__restrict__ adiag, int col0, double rho,
@@ -850,9 +824,6 @@ __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, i
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         T djj = lane_bcast(a[j], j);
-        T b[16];
-#pragma unroll
-        for (int c = j + 1; c < 16; c++) b[c] = lane_bcast(a[j], c);   // a_cj, ahead of the pivot
         const bool real = R + j < nb;
         if (SAFE) low_any = low_any || (real && djj <= lane_bcast(floor_l, j));
         const bool bad = !(djj > (T)0);
@@ -860,10 +831,10 @@ __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, i
         djj = bad ? (T)1 : djj;
         const T rj = rsqrt_nr(djj);
         rinv[j] = rj;
-        const T tj = a[j] * (rj * rj);
+        const T l = (lane > j) ? a[j] * rj : ((lane == j) ? djj * rj : (T)0);
+        a[j] = l;
 #pragma unroll
-        for (int c = j + 1; c < 16; c++) a[c] = fma(-tj, b[c], a[c]);
-        a[j] = (lane > j) ? a[j] * rj : ((lane == j) ? djj * rj : (T)0);
+        for (int c = j + 1; c < 16; c++) a[c] -= l * lane_bcast(l, c);
     }
     if (SAFE && low_any) *brk = true;
     if (lane < 16) {
@@ -906,12 +877,12 @@ __device__ __forceinline__ int potrf16_wave(T *S, T *X, int ld, int R, int nb, i
 template <typename T, bool SAFE>
 __device__ __forceinline__ void potrf_block(const DiagTask &t, T *__restrict__ W, T *S, T *X, int *sfail,
                                             int *__restrict__ fail, const double *__restrict__ adiag,
-                                            double rho, const Redo &rd, double *tmp, bool preloaded) {
+                                            double rho, const Redo &rd, double *tmp) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nb = t.nb;
     const int np = (nb + 15) >> 4;          // 16-column panels that hold real columns
-    if (!preloaded) {
+    {
         // all 16 loads of a thread in flight at once, then the LDS stores
         T v[16];
         const int i = tid & (NB - 1), c0 = tid >> 6;
@@ -1095,8 +1066,8 @@ __global__ __launch_bounds__(256) void k_potrf_small(const DiagTask *__restrict_
 // mostly one to five columns): one lane factors and inverts a whole block, with
 // potrf16_wave's arithmetic: pivot d_jj (when SAFE, a pivot at or below
 // rho a_jj sends the block to the FP64 redo queue), a non-positive pivot
-// reported and replaced by 1, r_j = 1/sqrt(d_jj), a_ic -= (a_ij r_j^2) a_cj,
-// L_jj = d_jj r_j, L_ij = a_ij r_j, the inverse by forward substitution with r_j.
+// reported and replaced by 1, r_j = 1/sqrt(d_jj), L_jj = d_jj r_j,
+// L_ij = a_ij r_j, the inverse by forward substitution with r_j.
 constexpr int TINY_DIAG = 6;
 
 template <typename T, bool SAFE>
@@ -1125,14 +1096,13 @@ __global__ __launch_bounds__(256) void k_potrf_tiny(const DiagTask *__restrict__
         djj = bad ? (T)1 : djj;
         const T rj = rsqrt_nr(djj);
         rinv[j] = rj;
-        const T r2 = rj * rj;
-#pragma unroll
-        for (int c = j + 1; c < TINY_DIAG; c++)
-#pragma unroll
-            for (int i = c; i < TINY_DIAG; i++) a[i][c] = fma(-(a[i][j] * r2), a[c][j], a[i][c]);
         a[j][j] = djj * rj;
 #pragma unroll
         for (int i = j + 1; i < TINY_DIAG; i++) a[i][j] *= rj;
+#pragma unroll
+        for (int c = j + 1; c < TINY_DIAG; c++)
+#pragma unroll
+            for (int i = c; i < TINY_DIAG; i++) a[i][c] -= a[i][j] * a[c][j];
     }
     if (SAFE && low_any) {
         redo_push(rd, t);            // breakdown: the FP64 redo refactors the assembled block
