@@ -245,6 +245,29 @@ int mxc_rcond(mxc_factor *F, double *rcond);
  * (a process-wide lock; the call is synchronous). */
 int mxc_rcond_noalloc(mxc_factor *F, double *rcond);
 
+/* Rank-k update or downdate of the resident factor in place:
+ *   Lnew Lnew' = L L' + C C' (update != 0) or L L' - C C' (update == 0),
+ * replacing cholmod_l_updown (CHOLMOD/Include/cholmod.h:3465,
+ * CHOLMOD/Modify/cholmod_updown.c:74).  C is n x ncol, CSC (Cp[ncol+1],
+ * Ci, Cx on the host, rows sorted within each column, a later entry of a
+ * repeated row winning), its rows in the factor's order: L->Perm is not
+ * applied, as in the reference.  Each entry of C(:, c) must lie in the
+ * pattern of L(:, k), k the column's first row; otherwise the modification
+ * needs new fill and the call returns MXC_INVALID with L unchanged (the
+ * reference adds the fill to a simplicial factor).  Columns are taken 8 at a
+ * time, as the reference's maxrank.  *modfl = the reference's
+ * Common->modfl; info[0] = the first column whose new pivot is not positive
+ * (n if none), info[1] = pivots raised to dbound (dbound > 0, as
+ * cholmod_dbound; 0 = off).  L stays a supernodal LL' factor on the device
+ * (the reference turns it into a simplicial LDL' one), and its diagonal-block
+ * inverses are renewed for the solves.  A downdate that loses positive
+ * definiteness returns MXC_NOT_POSDEF and leaves the factor without values
+ * (the reference keeps an indefinite LDL').  After a modification the stored
+ * A no longer matches: mxc_solve_refine refuses until the next
+ * mxc_factorize, and an FP64 twin of a low-precision factor is dropped. */
+int mxc_updown(mxc_factor *F, int update, int64_t ncol, const int64_t *Cp, const int64_t *Ci, const double *Cx,
+               double dbound, double *modfl, int64_t *info);
+
 /* R = B - A X in FP64 with the values of the last mxc_factorize. */
 int mxc_residual(mxc_factor *F, const double *B, const double *X, int64_t nrhs, double *R,
                  int location);

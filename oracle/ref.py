@@ -57,6 +57,9 @@ def lib():
                                    C.POINTER(C.c_double)]
         L.ref_order.argtypes = [C.c_int64, _i64p, _i64p, C.c_int, _i64p, _f64p]
         L.ref_order.restype = C.c_int
+        L.ref_updown.argtypes = [C.c_void_p, C.c_int, C.c_int64, _i64p, _i64p, _f64p, C.c_double, _f64p,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.ref_updown.restype = C.c_int
         L.ref_free.argtypes = [C.c_void_p]
         L.ref_free.restype = None
         L.ref_read_sparse.argtypes = [C.c_char_p] + [C.c_void_p] * 4
@@ -207,6 +210,25 @@ class RefFactor:
         r = np.zeros_like(b)
         lib().ref_residual(self._h, b, x, nrhs, r)
         return r.reshape(nrhs, self.n).T.copy()
+
+    def updown(self, update: bool, Cp, Ci, Cx, dbound: float = 0.0) -> dict:
+        """cholmod_l_updown with C (CSC, rows in the factor's order).  The result
+        comes back as LL' values in the supernodal layout of the factor before the
+        first call (the reference keeps a simplicial LDL' factor afterwards)."""
+        Cp = np.ascontiguousarray(Cp, dtype=np.int64)
+        ncol = len(Cp) - 1
+        Ci = np.ascontiguousarray(Ci, dtype=np.int64)
+        Cx = np.ascontiguousarray(Cx, dtype=np.float64)
+        if Ci.size == 0:
+            Ci, Cx = np.zeros(1, np.int64), np.zeros(1)
+        Lx = np.zeros(self.xsize)
+        fl, nb, t = C.c_double(0), C.c_int64(0), C.c_double(0)
+        st = lib().ref_updown(self._h, int(bool(update)), ncol, Cp, Ci, Cx, float(dbound), Lx,
+                              C.byref(fl), C.byref(nb), C.byref(t))
+        if st < 0:
+            raise RuntimeError(f"reference updown failed ({st})")
+        self.t_updown = t.value
+        return {"Lx": Lx, "status": st, "modfl": fl.value, "ndbounds_hit": nb.value, "t": t.value}
 
     def __del__(self):
         h = getattr(self, "_h", None)

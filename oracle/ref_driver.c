@@ -13,6 +13,7 @@
 //   cholmod_l_solve2 (sparse b, Bset)        cholmod.h:2845
 //   cholmod_l_sdmult (residual)              cholmod.h (MatrixOps)
 //   cholmod_l_read_sparse2 (file reader)     CHOLMOD/Check/cholmod_read.c
+//   cholmod_l_updown (rank-k modification)   CHOLMOD/Modify/cholmod_updown.c:74
 // Input matrices are the lower triangle of a symmetric matrix in CSC form
 // (stype = -1), int64 indices, as CHOLMOD's int64 ("_l_") interface takes.
 
@@ -31,6 +32,8 @@ typedef struct {
     cholmod_sparse *A;
     cholmod_factor *L;
     int dtype;             // CHOLMOD_DOUBLE or CHOLMOD_SINGLE for the factor
+    // supernodal structure kept by ref_updown (the reference turns L simplicial)
+    int64_t nsuper, *Super, *Lpi, *Lpx, *Ls;
 } ref_handle;
 
 static double now_s(void) {
@@ -297,9 +300,93 @@ int ref_residual(void *vh, const double *B, const double *X, int64_t nrhs, doubl
     return 0;
 }
 
+// cholmod_l_updown (Modify/cholmod_updown.c:74) on the handle's factor with
+// C (n x ncol, CSC, sorted rows in the factor's order).  The first call needs
+// the supernodal factor of ref_factorize; the reference turns it into a
+// simplicial LDL' one (cholmod_updown.c:478-491), which later calls keep
+// modifying.  Lx receives the result as LL' values in the supernodal layout
+// the factor had: l_jj = sqrt(d_j), l_ij * l_jj (as the reference's
+// LDL' -> LL' conversion computes them).  modfl = Common->modfl,
+// ndbounds = Common->ndbounds_hit, t = seconds in cholmod_l_updown.  Returns
+// the cholmod status, -3 if a column's pattern no longer matches the
+// supernodal one (new fill), -4 without a supernodal structure.
+int ref_updown(void *vh, int update, int64_t ncol, const int64_t *Cp, const int64_t *Ci, const double *Cx,
+               double dbound, double *Lx, double *modfl, int64_t *ndbounds, double *t) {
+    ref_handle *h = (ref_handle *)vh;
+    cholmod_common *cm = &h->cm;
+    cholmod_factor *L = h->L;
+    int64_t n = (int64_t)L->n;
+    if (!h->Super) {
+        if (!L->is_super) return -4;
+        int64_t ns = (int64_t)L->nsuper;
+        h->nsuper = ns;
+        h->Super = malloc(sizeof(int64_t) * (ns + 1));
+        h->Lpi = malloc(sizeof(int64_t) * (ns + 1));
+        h->Lpx = malloc(sizeof(int64_t) * (ns + 1));
+        h->Ls = malloc(sizeof(int64_t) * (L->ssize > 0 ? L->ssize : 1));
+        memcpy(h->Super, L->super, sizeof(int64_t) * (ns + 1));
+        memcpy(h->Lpi, L->pi, sizeof(int64_t) * (ns + 1));
+        memcpy(h->Lpx, L->px, sizeof(int64_t) * (ns + 1));
+        memcpy(h->Ls, L->s, sizeof(int64_t) * L->ssize);
+    }
+    int xt = CHOLMOD_REAL + (h->dtype == CHOLMOD_SINGLE ? CHOLMOD_SINGLE : CHOLMOD_DOUBLE);
+    int64_t nz = Cp[ncol];
+    cholmod_sparse *C = cholmod_l_allocate_sparse(n, ncol, nz > 0 ? nz : 1, 1, 1, 0, xt, cm);
+    if (!C) return -2;
+    memcpy(C->p, Cp, sizeof(int64_t) * (ncol + 1));
+    memcpy(C->i, Ci, sizeof(int64_t) * nz);
+    if (h->dtype == CHOLMOD_SINGLE) {
+        float *x = (float *)C->x;
+        for (int64_t p = 0; p < nz; p++) x[p] = (float)Cx[p];
+    } else {
+        memcpy(C->x, Cx, sizeof(double) * nz);
+    }
+    cm->dbound = dbound;
+    cm->sbound = (float)dbound;
+    cm->ndbounds_hit = 0;
+    double t0 = now_s();
+    int ok = cholmod_l_updown(update, C, L, cm);
+    if (t) *t = now_s() - t0;
+    cholmod_l_free_sparse(&C, cm);
+    if (modfl) *modfl = cm->modfl;
+    if (ndbounds) *ndbounds = (int64_t)cm->ndbounds_hit;
+    if (!ok) return cm->status != 0 ? cm->status : -99;
+    const int64_t *Lp = (const int64_t *)L->p, *Lnz = (const int64_t *)L->nz, *Li = (const int64_t *)L->i;
+    for (int64_t s = 0; s < h->nsuper; s++) {
+        int64_t k1 = h->Super[s], nscol = h->Super[s + 1] - k1;
+        int64_t psi = h->Lpi[s], nsrow = h->Lpi[s + 1] - psi;
+        for (int64_t jj = 0; jj < nscol; jj++) {
+            int64_t j = k1 + jj, p = Lp[j];
+            if (Lnz[j] != nsrow - jj || Li[p] != j) return -3;
+            double d, l;
+            if (h->dtype == CHOLMOD_SINGLE) {
+                const float *x = (const float *)L->x;
+                float ljj = sqrtf(x[p]);
+                Lx[h->Lpx[s] + jj * nsrow + jj] = ljj;
+                for (int64_t q = 1; q < nsrow - jj; q++) {
+                    if (Li[p + q] != h->Ls[psi + jj + q]) return -3;
+                    Lx[h->Lpx[s] + jj * nsrow + jj + q] = (double)(x[p + q] * ljj);
+                }
+            } else {
+                const double *x = (const double *)L->x;
+                d = x[p];
+                double ljj = sqrt(d);
+                Lx[h->Lpx[s] + jj * nsrow + jj] = ljj;
+                for (int64_t q = 1; q < nsrow - jj; q++) {
+                    if (Li[p + q] != h->Ls[psi + jj + q]) return -3;
+                    l = x[p + q];
+                    Lx[h->Lpx[s] + jj * nsrow + jj + q] = l * ljj;
+                }
+            }
+        }
+    }
+    return cm->status;
+}
+
 void ref_free(void *vh) {
     ref_handle *h = (ref_handle *)vh;
     if (!h) return;
+    free(h->Super), free(h->Lpi), free(h->Lpx), free(h->Ls);
     cholmod_l_free_factor(&h->L, &h->cm);
     cholmod_l_free_sparse(&h->A, &h->cm);
     cholmod_l_finish(&h->cm);

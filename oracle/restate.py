@@ -29,6 +29,8 @@ def lib():
         L.oracle_solve.restype = None
         L.oracle_permute_lower.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_permute_lower.restype = C.c_int
+        L.oracle_updown.argtypes = [i64, i64, vp, vp, vp, vp, vp, C.c_int, i64, vp, vp, vp, C.c_double, vp, vp]
+        L.oracle_updown.restype = C.c_int
         _lib = L
     return _lib
 
@@ -117,3 +119,20 @@ def dense_L(sym: dict, Lx: np.ndarray) -> np.ndarray:
             r = rows[c:]
             L[r, k1 + c] = blk[c:, c]
     return L
+
+
+def updown(sym: dict, Lx: np.ndarray, update: bool, Cp, Ci, Cx, dbound: float = 0.0) -> dict:
+    """Rank-k update (update=True) or downdate of the supernodal LL' factor Lx by
+    C (CSC, rows in the factor's order); returns the new values, modfl, the first
+    column whose new pivot is not positive (n if none) and the bounded pivots."""
+    out = np.ascontiguousarray(Lx, dtype=np.float64).copy()
+    Cp = _i64(Cp)
+    Ci = _i64(Ci) if len(Ci) else np.zeros(1, np.int64)
+    Cx = np.ascontiguousarray(Cx, dtype=np.float64) if len(Cx) else np.zeros(1)
+    ns = len(sym["Super"]) - 1
+    keep = [_i64(sym[k]) for k in ("Super", "Lpi", "Lpx", "Ls")]
+    fl = C.c_double(0)
+    info = np.zeros(2, dtype=np.int64)
+    rc = lib().oracle_updown(len(sym["Perm"]), ns, *[_p(a) for a in keep], _p(out), int(bool(update)),
+                             len(Cp) - 1, _p(Cp), _p(Ci), _p(Cx), float(dbound), C.byref(fl), _p(info))
+    return {"Lx": out, "rc": int(rc), "modfl": fl.value, "minor": int(info[0]), "ndbounds_hit": int(info[1])}

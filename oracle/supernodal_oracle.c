@@ -23,6 +23,8 @@
 //     :20-303   lsolve: gather E, x1 = L1 \ x1, E -= L2 x1, scatter
 //     :309-580  ltsolve: gather E, x1 -= L2' E, x1 = L1' \ x1
 //   permutation: CHOLMOD/Cholesky/t_cholmod_psolve_worker.c:40-110 (Y = P B, X = P' Y)
+//   update/downdate: CHOLMOD/Modify/t_cholmod_updown_numkr.c:294-372 on the
+//     paths of Modify/t_cholmod_updown_worker.c (oracle_updown, below)
 //
 // Every integer array is int64 (CHOLMOD's "_l_" interface).
 
@@ -309,4 +311,130 @@ int oracle_permute_lower(Int n, const Int *Ap, const Int *Ai, const double *Ax,
     free(Pinv);
     free(cnt);
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Rank-k update/downdate of a supernodal LL' factor in place:
+//   Lnew Lnew' = L L' + C C' (update) or L L' - C C' (downdate)
+// with C (n x ncol, CSC, sorted rows) in the factor's row order, as
+// cholmod_updown takes it (CHOLMOD/Modify/cholmod_updown.c:26-43: L->Perm is
+// not applied).
+//   - C is taken 8 columns at a time (Common->maxrank,
+//     Modify/t_cholmod_updown_worker.c:166-183); each column's path starts at
+//     its first row, an empty column at n-1 (:196-209), and climbs the column
+//     elimination tree of the factor's pattern: the next column of the
+//     supernode, then the first row below it.  A supernodal factor keeps its
+//     full pattern when the reference converts it (Utility/
+//     t_cholmod_change_factor_3_template.c:41), so these are its parents too.
+//   - W(:, r) = C(:, r), later entries of a column overwriting earlier ones
+//     (Modify/t_cholmod_updown_wdim.c:118-137); Alpha[r] = 1.
+//   - Each column j on the union of the paths, in increasing order, is turned
+//     into its LDL' form (d = l_jj^2, l_ij / l_jj, as t_cholmod_change_factor_3
+//     does), modified by the recurrence of Modify/t_cholmod_updown_numkr.c:
+//     :294-324 (ALPHA_GAMMA) for the columns of W whose path holds j, then
+//     :363-372 for each row below, and turned back (l_jj = sqrt(d), l_ij l_jj).
+//     Ranks are applied in the order of C's columns; the reference orders
+//     merged paths by a depth-first walk (t_cholmod_updown_worker.c:915-935),
+//     which changes only the rounding.
+//   - dbound > 0 bounds each new d as cholmod_dbound does
+//     (Utility/t_cholmod_bound.c): counted in info[1].
+//   - modfl = sum over the paths of (6 + 4 |L(:,j)|) per column
+//     (t_cholmod_updown_worker.c:784).
+// info[0] = first column whose new d is not positive (an LL' factor cannot
+// hold it), else n.  Returns 0, or -1 if an entry of C lies outside the
+// pattern of L(:, first row of its column) (the reference would add fill).
+// ---------------------------------------------------------------------------
+#define UD_RANK 8
+
+int oracle_updown(Int n, Int nsuper, const Int *Super, const Int *Lpi, const Int *Lpx, const Int *Ls,
+                  double *Lx, int update, Int ncol, const Int *Cp, const Int *Ci, const double *Cx,
+                  double dbound, double *modfl, Int *info) {
+    Int *colsuper = (Int *)malloc(sizeof(Int) * (n > 0 ? n : 1));
+    double *W = (double *)malloc(sizeof(double) * UD_RANK * (n > 0 ? n : 1));
+    unsigned char *act = (unsigned char *)malloc(n > 0 ? n : 1);
+    if (!colsuper || !W || !act) {
+        free(colsuper), free(W), free(act);
+        return -2;
+    }
+    for (Int s = 0; s < nsuper; s++)
+        for (Int j = Super[s]; j < Super[s + 1]; j++) colsuper[j] = s;
+    info[0] = n;
+    info[1] = 0;
+    double fl = 0;
+    int rc = 0;
+    // pattern check: rows of C(:, r) inside {k} U pattern of L(:, k), k its first row
+    for (Int r = 0; r < ncol && rc == 0; r++) {
+        if (Cp[r + 1] == Cp[r]) continue;
+        Int k = Ci[Cp[r]], s = colsuper[k], k2 = Super[s + 1];
+        Int q = Lpi[s] + (k2 - Super[s]), qend = Lpi[s + 1];
+        for (Int p = Cp[r]; p < Cp[r + 1]; p++) {
+            Int i = Ci[p];
+            if (i < k || i >= n) { rc = -1; break; }
+            if (i < k2) continue;
+            while (q < qend && Ls[q] < i) q++;
+            if (q == qend || Ls[q] != i) { rc = -1; break; }
+        }
+    }
+    for (Int r0 = 0; r0 < ncol && rc == 0; r0 += UD_RANK) {
+        int k = (int)((ncol - r0) < UD_RANK ? (ncol - r0) : UD_RANK);
+        memset(W, 0, sizeof(double) * UD_RANK * n);
+        memset(act, 0, n);
+        double alpha[UD_RANK];
+        for (int r = 0; r < k; r++) {
+            alpha[r] = 1.0;
+            Int c = r0 + r;
+            Int j = Cp[c + 1] > Cp[c] ? Ci[Cp[c]] : n - 1;
+            const int numeric = Cp[c + 1] > Cp[c];
+            for (Int p = Cp[c]; p < Cp[c + 1]; p++) W[Ci[p] * UD_RANK + r] = Cx[p];
+            while (j >= 0) {
+                Int s = colsuper[j], k1 = Super[s], nscol = Super[s + 1] - k1, nsrow = Lpi[s + 1] - Lpi[s];
+                fl += 6.0 + 4.0 * (double)(nsrow - (j - k1));
+                if (numeric) act[j] |= (unsigned char)(1u << r);
+                j = (j + 1 < k1 + nscol) ? j + 1 : (nsrow > nscol ? Ls[Lpi[s] + nscol] : -1);
+            }
+        }
+        for (Int j = 0; j < n; j++) {
+            if (!act[j]) continue;
+            Int s = colsuper[j], k1 = Super[s], nscol = Super[s + 1] - k1, nsrow = Lpi[s + 1] - Lpi[s];
+            Int jl = j - k1, psi = Lpi[s];
+            double *col = Lx + Lpx[s] + jl * nsrow;
+            double lold = col[jl], dj = lold * lold;
+            double wj[UD_RANK], gam[UD_RANK];
+            for (int r = 0; r < k; r++) {
+                if (!(act[j] & (1u << r))) continue;
+                double w = W[j * UD_RANK + r], al = alpha[r];
+                double t = w * w;
+                t = t / dj;
+                double a = update ? al + t : al - t;
+                dj = dj * a;
+                alpha[r] = a;
+                gam[r] = update ? (-w) / dj : w / dj;
+                dj = dj / al;
+                wj[r] = w;
+            }
+            if (dbound > 0 && !isnan(dj)) {
+                if (dj >= 0 ? dj < dbound : dj > -dbound) {
+                    dj = dj >= 0 ? dbound : -dbound;
+                    info[1]++;
+                }
+            }
+            if (!(dj > 0) && info[0] == n) info[0] = j;
+            double lnew = sqrt(dj);
+            col[jl] = lnew;
+            for (Int ii = jl + 1; ii < nsrow; ii++) {
+                Int i = ii < nscol ? k1 + ii : Ls[psi + ii];
+                double l = col[ii] / lold;
+                for (int r = 0; r < k; r++) {
+                    if (!(act[j] & (1u << r))) continue;
+                    double *w = W + i * UD_RANK + r;
+                    *w = *w - wj[r] * l;
+                    l = l - gam[r] * *w;
+                }
+                col[ii] = l * lnew;
+            }
+        }
+    }
+    *modfl = fl;
+    free(colsuper), free(W), free(act);
+    return rc;
 }

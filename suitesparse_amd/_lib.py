@@ -63,6 +63,7 @@ _PROTOS = {
     "mxc_get_tiles": (i32, [vp, vp]),
     "mxc_rcond": (i32, [vp, vp]),
     "mxc_rcond_noalloc": (i32, [vp, vp]),
+    "mxc_updown": (i32, [vp, i32, i64, vp, vp, vp, f64, vp, vp]),
     "mxc_synchronize": (i32, [vp]),
     "mxc_set_option": (i32, [vp, i32, i64]),
     "mxc_set_timing": (i32, [vp, i32]),

@@ -8,6 +8,7 @@ error behaviour -- on top of the C ABI (include/cholmod_mi355x.h):
   ``factorize(A, L, cm)``           cholmod_l_factorize   cholmod.h:2775
   ``factorize_p(A, beta, L, cm)``   cholmod_l_factorize_p cholmod.h:2794
   ``solve(sys, L, B, cm)``          cholmod_l_solve       cholmod.h:2823
+  ``updown(update, C, L, cm)``      cholmod_l_updown      cholmod.h:3465
   ``Common``                        cholmod_common: status, fl, lnz, nmethods, method[0].ordering,
                                     selected, method[0..2].fl/lnz, anz, useGPU, print
   ``Factor``                        cholmod_factor: n, minor, Perm, ColCount, super, pi,
@@ -41,6 +42,7 @@ CHOLMOD_TOO_LARGE = -3
 CHOLMOD_INVALID = -4
 CHOLMOD_GPU_PROBLEM = -5
 CHOLMOD_NOT_POSDEF = 1
+CHOLMOD_DSMALL = 2
 
 # orderings (cholmod.h:448-460)
 CHOLMOD_NATURAL = 0
@@ -73,6 +75,10 @@ class Common:
     device: int = 0
     precision: str = "fp64"            # numeric factor: "fp64" or "fp32" (+ FP64 refinement)
     print: int = 0
+    # update/downdate: Common->dbound (0: off), and what updown leaves
+    dbound: float = 0.0
+    modfl: float = 0.0
+    ndbounds_hit: int = 0
 
     def __post_init__(self):
         self._started = False
@@ -614,6 +620,69 @@ def rcond(L: Factor, cm: Common) -> float:
         cm.status = st
         return -1.0
     return out.value
+
+
+def _csc(Cm, n: int):
+    """(p, i, x) of an n-row sparse C: a scipy.sparse matrix, a dense array or a
+    (p, i, x) triple of CSC arrays (rows sorted within each column)."""
+    if isinstance(Cm, tuple):
+        p, i, x = Cm
+        return (np.ascontiguousarray(p, dtype=np.int64), np.ascontiguousarray(i, dtype=np.int64),
+                np.ascontiguousarray(x, dtype=np.float64))
+    if hasattr(Cm, "tocsc"):
+        Cs = Cm.tocsc()
+        if Cs.shape[0] != n:
+            raise ValueError("C and L dimensions do not match")
+        Cs.sort_indices()
+        return (np.ascontiguousarray(Cs.indptr, dtype=np.int64), np.ascontiguousarray(Cs.indices, dtype=np.int64),
+                np.ascontiguousarray(Cs.data, dtype=np.float64))
+    D = np.asarray(Cm, dtype=np.float64).reshape(n, -1)
+    nz = D != 0
+    p = np.concatenate([[0], np.cumsum(nz.sum(axis=0))]).astype(np.int64)
+    i = np.nonzero(nz.T)[1].astype(np.int64)
+    return p, i, np.ascontiguousarray(D.T[nz.T])
+
+
+def updown(update: bool, Cm, L: Factor, cm: Common) -> bool:
+    """cholmod_l_updown: L L' + C C' (update) or L L' - C C' (downdate) in place
+    on the device.  C's rows are in L's order (L.Perm is not applied, as in the
+    reference); every entry of C(:, c) must lie in the pattern of L(:, k), k the
+    column's first row, else CHOLMOD_INVALID (the modification needs new fill).
+    Leaves cm.modfl and cm.ndbounds_hit; a pivot raised to cm.dbound gives
+    CHOLMOD_DSMALL.  A downdate that loses positive definiteness returns False
+    with CHOLMOD_NOT_POSDEF and L.minor (L then holds no values: the device
+    factor is LL', where the reference keeps an indefinite LDL')."""
+    cm.status = CHOLMOD_OK
+    cm.modfl = 0.0
+    cm.ndbounds_hit = 0
+    if L is None or not L.is_numeric or L._num is None:
+        cm.status = CHOLMOD_INVALID
+        return False
+    try:
+        p, i, x = _csc(Cm, L.n)
+    except ValueError:
+        cm.status = CHOLMOD_INVALID
+        return False
+    ncol = len(p) - 1
+    if i.size == 0:
+        i, x = np.zeros(1, np.int64), np.zeros(1)
+    fl = C.c_double(0)
+    info = np.zeros(2, dtype=np.int64)
+    st = lib().mxc_updown(L._num, int(bool(update)), ncol, _ptr(p), _ptr(i), _ptr(x), float(cm.dbound),
+                          C.byref(fl), _ptr(info))
+    cm.modfl = fl.value
+    cm.ndbounds_hit = int(info[1])
+    if st == _lib.MXC_NOT_POSDEF:
+        cm.status = CHOLMOD_NOT_POSDEF
+        L.minor = int(info[0])
+        L.is_numeric = False
+        return False
+    if st != 0:
+        cm.status = st
+        return False
+    if cm.ndbounds_hit:
+        cm.status = CHOLMOD_DSMALL
+    return True
 
 
 def residual(L: Factor, B, X):

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <mutex>
 #include <climits>
 #include <cmath>
@@ -33,6 +34,7 @@
 #include "../../include/cholmod_mi355x.h"
 #include "kernels_factor.h"
 #include "kernels_solve.h"
+#include "kernels_modify.h"
 #include "symbolic_internal.h"
 
 using namespace mxc;
@@ -314,6 +316,18 @@ struct mxc_factor {
 
     // algorithmic flops per factorization, by kernel class
     double flops_update = 0, flops_potrf = 0, flops_trsm = 0, flops_trail = 0;
+
+    // rank-k update/downdate (mxc_updown): column -> supernode, supernodal
+    // parents and inverse-block bases, built on the first call; its device
+    // buffers are kept for later calls
+    std::vector<int64_t> col_super, sparent_h, winv_base_h;
+    void *d_mw = nullptr;         // T[n * UD_RANK]: the columns of C of one pass
+    void *d_mpar = nullptr;       // T[UD_RANK * NB * UD_PAR]: pivot data of the blocks of a launch
+    void *d_malpha = nullptr;     // T[UD_RANK]
+    int *d_mflag = nullptr;       // [0] first column whose new pivot is not positive, [1] bounded pivots
+    void *d_mdesc = nullptr;      // blocks, row ranges and C's entries of one pass
+    size_t mdesc_cap = 0;
+    bool modified = false;        // L changed by mxc_updown since the last factorization
 
     // timing: an event pair around each launch of a class whose bit is set
     unsigned timing_mask = 0;
@@ -1263,7 +1277,8 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
     void *ptrs[] = {F->W, F->d_Lh, F->d_Ls, F->d_pos, F->d_dpos, F->d_perm, F->d_desc, F->d_tiles,
                     F->d_diag, F->d_stask, F->d_fail, F->d_adiag, F->d_rowptr, F->d_src, F->d_col,
                     F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_part, F->d_b, F->d_x, F->d_r,
-                    F->d_cnt, F->d_abort, F->d_seg, F->d_segout, F->d_pinv, F->d_redo, F->d_ready};
+                    F->d_cnt, F->d_abort, F->d_seg, F->d_segout, F->d_pinv, F->d_redo, F->d_ready,
+                    F->d_mw, F->d_mpar, F->d_malpha, F->d_mflag, F->d_mdesc};
     for (void *p : ptrs) ffree(F, p);
 
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
@@ -1739,6 +1754,7 @@ extern "C" int mxc_factorize(mxc_factor *F, const double *Ax, int location, doub
     HIPTRY(hipSetDevice(g_device));
     hipStream_t sm = F->stream;
     F->escalated = false;
+    F->modified = false;
     HIPTRY(hipEventRecord(F->ev_f0, sm));
     if (F->nnz_a > 0)
         HIPTRY(hipMemcpyAsync(F->d_Ax, Ax, F->nnz_a * sizeof(double),
@@ -2404,6 +2420,11 @@ static int solve_refine_impl(mxc_factor *F, const double *B, int64_t nrhs, doubl
             if (segoff[g + 1] < segoff[g]) return MXC_INVALID;
     }
     if (!F->values_loaded) return MXC_INVALID;
+    if (F->modified) {
+        snprintf(g_err, sizeof(g_err), "refinement: the factor was changed by mxc_updown and no longer matches "
+                                       "the stored A; factorize A +- CC' to refine against it");
+        return MXC_INVALID;
+    }
     if (nrhs == 0 || F->n == 0) return MXC_OK;
     HIPTRY(hipSetDevice(g_device));
     const bool escalated = F->escalated && F->fallback;
@@ -2698,6 +2719,269 @@ extern "C" int mxc_rcond_noalloc(mxc_factor *F, double *rcond) {
     HIPTRY(hipSetDevice(g_device));
     std::lock_guard<std::mutex> lk(g_scratch_mutex);
     return rcond_partials(F, g_scratch[g_device], rcond);
+}
+
+
+// ---------------------------------------------------------------------------
+// rank-k update/downdate of the resident factor (cholmod_updown,
+// CHOLMOD/Modify/cholmod_updown.c:74; paths and modfl as
+// Modify/t_cholmod_updown_worker.c:166-240 and :784)
+// ---------------------------------------------------------------------------
+static void updown_structure(mxc_factor *F) {
+    if (!F->col_super.empty() || F->n == 0) return;
+    const int64_t ns = F->nsuper;
+    F->col_super.assign(F->n, 0);
+    F->sparent_h.assign(ns, -1);
+    F->winv_base_h.assign(ns, 0);
+    int64_t woff = F->xsize;                 // the same slots as the factorization's plan
+    for (int64_t s = 0; s < ns; s++) {
+        const int64_t k1 = F->Super[s], nscol = F->Super[s + 1] - k1;
+        for (int64_t j = k1; j < k1 + nscol; j++) F->col_super[j] = s;
+        F->winv_base_h[s] = woff;
+        for (int64_t c0 = 0; c0 < nscol; c0 += NB) {
+            const int64_t nb = std::min<int64_t>(NB, nscol - c0);
+            woff += nb * nb;
+        }
+    }
+    for (int64_t s = 0; s < ns; s++) {
+        const int64_t nscol = F->Super[s + 1] - F->Super[s], nsrow = F->Lpi[s + 1] - F->Lpi[s];
+        if (nsrow > nscol) F->sparent_h[s] = F->col_super[F->Ls_host[F->Lpi[s] + nscol]];
+    }
+}
+
+// sum over columns jl = st .. nscol-1 of supernode s of (6 + 4 |L(:, k1 + jl)|)
+static double updown_fl(const mxc_factor *F, int64_t s, int64_t st) {
+    const double nscol = (double)(F->Super[s + 1] - F->Super[s]), nsrow = (double)(F->Lpi[s + 1] - F->Lpi[s]);
+    const double m = nscol - (double)st;
+    return 6.0 * m + 4.0 * (m * nsrow - ((double)st + nscol - 1.0) * m / 2.0);
+}
+
+struct UdLaunch {
+    int64_t b0, nb, r0, nr;
+};
+
+template <typename T>
+static int updown_run(mxc_factor *F, int update, int64_t ncol, const int64_t *Cp, const int64_t *Ci,
+                      const double *Cx, double dbound, double *modfl, int64_t *fail, int64_t *nbound) {
+    const int64_t n = F->n, ns = F->nsuper;
+    hipStream_t sm = F->stream;
+    {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+        int rc;
+        if (!F->d_mw && (rc = falloc(F, &F->d_mw, sizeof(T) * (size_t)n * UD_RANK))) return rc;
+        if (!F->d_mpar && (rc = falloc(F, &F->d_mpar, sizeof(T) * (size_t)UD_RANK * NB * UD_PAR))) return rc;
+        if (!F->d_malpha && (rc = falloc(F, &F->d_malpha, sizeof(T) * UD_RANK))) return rc;
+        if (!F->d_mflag && (rc = falloc(F, (void **)&F->d_mflag, sizeof(int) * 2))) return rc;
+    }
+    T *W = (T *)F->W, *Wc = (T *)F->d_mw, *par = (T *)F->d_mpar, *alpha = (T *)F->d_malpha;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_mflag, INT_MAX);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_mflag + 1, 0);
+    double fl = 0;
+    std::vector<int64_t> task_of(ns, -1);
+    for (int64_t r0 = 0; r0 < ncol; r0 += UD_RANK) {
+        const int k = (int)std::min<int64_t>(UD_RANK, ncol - r0);
+        // the supernodes on the paths, with each rank's first column in them
+        std::vector<int64_t> ts;
+        std::vector<std::array<int, UD_RANK>> starts;
+        std::vector<long long> sidx;
+        std::vector<double> sval;
+        for (int r = 0; r < k; r++) {
+            const int64_t c = r0 + r;
+            const bool numeric = Cp[c + 1] > Cp[c];
+            const int64_t j = numeric ? Ci[Cp[c]] : n - 1;
+            for (int64_t p = Cp[c]; p < Cp[c + 1]; p++) {
+                if (p + 1 < Cp[c + 1] && Ci[p + 1] == Ci[p]) continue;   // a later entry of the row wins
+                sidx.push_back((long long)Ci[p] * UD_RANK + r);
+                sval.push_back(Cx[p]);
+            }
+            int64_t s = F->col_super[j], st = j - F->Super[s];
+            while (s >= 0) {
+                fl += updown_fl(F, s, st);
+                if (numeric) {
+                    if (task_of[s] < 0) {
+                        task_of[s] = (int64_t)ts.size();
+                        ts.push_back(s);
+                        std::array<int, UD_RANK> a;
+                        a.fill(INT_MAX);
+                        starts.push_back(a);
+                    }
+                    starts[task_of[s]][r] = (int)st;
+                }
+                s = F->sparent_h[s];
+                st = 0;
+            }
+        }
+        for (int64_t s : ts) task_of[s] = -1;
+        if (ts.empty()) continue;
+        // blocks by level, then by 64-column step within the level
+        std::vector<int64_t> order(ts.size());
+        for (size_t q = 0; q < ts.size(); q++) order[q] = (int64_t)q;
+        std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+            const int la = F->level_of[ts[a]], lb = F->level_of[ts[b]];
+            return la != lb ? la < lb : ts[a] < ts[b];
+        });
+        std::vector<UdBlock> blocks;
+        std::vector<UdRows> rows;
+        std::vector<UdLaunch> launches;
+        for (size_t q0 = 0; q0 < order.size();) {
+            size_t q1 = q0;
+            while (q1 < order.size() && F->level_of[ts[order[q1]]] == F->level_of[ts[order[q0]]]) q1++;
+            int64_t steps = 0;
+            std::vector<int64_t> fb(q1 - q0), nblk(q1 - q0);
+            for (size_t q = q0; q < q1; q++) {
+                const int64_t s = ts[order[q]], nscol = F->Super[s + 1] - F->Super[s];
+                int first = INT_MAX;
+                for (int r = 0; r < UD_RANK; r++) first = std::min(first, starts[order[q]][r]);
+                fb[q - q0] = first / NB;
+                nblk[q - q0] = ceil_div(nscol, NB);
+                steps = std::max(steps, nblk[q - q0] - fb[q - q0]);
+            }
+            for (int64_t step = 0; step < steps; step++) {
+                UdLaunch L{(int64_t)blocks.size(), 0, (int64_t)rows.size(), 0};
+                int slot = 0;
+                for (size_t q = q0; q < q1; q++) {
+                    const int64_t t = fb[q - q0] + step;
+                    if (t >= nblk[q - q0]) continue;
+                    const int64_t s = ts[order[q]], k1 = F->Super[s], nscol = F->Super[s + 1] - k1;
+                    const int64_t nsrow = F->Lpi[s + 1] - F->Lpi[s], c0 = t * NB;
+                    UdBlock ub;
+                    memset(&ub, 0, sizeof(ub));
+                    ub.lx = F->Lpx[s];
+                    ub.winv = F->winv_base_h[s] + c0 * NB;      // every earlier block is 64 wide
+                    ub.nsrow = (int)nsrow;
+                    ub.nscol = (int)nscol;
+                    ub.k1 = (int)k1;
+                    ub.psi = (int)F->Lpi[s];
+                    ub.c0 = (int)c0;
+                    ub.nb = (int)std::min<int64_t>(NB, nscol - c0);
+                    for (int r = 0; r < UD_RANK; r++) ub.start[r] = starts[order[q]][r];
+                    ub.slot = slot++;
+                    blocks.push_back(ub);
+                    for (int64_t rr = c0 + ub.nb; rr < nsrow; rr += 64)
+                        rows.push_back(UdRows{(int)(blocks.size() - 1), (int)rr});
+                }
+                L.nb = (int64_t)blocks.size() - L.b0;
+                L.nr = (int64_t)rows.size() - L.r0;
+                launches.push_back(L);
+            }
+            q0 = q1;
+        }
+        // one upload: blocks, row ranges, C's entries (positions, values)
+        const size_t ob = 0, orr = ob + blocks.size() * sizeof(UdBlock);
+        const size_t oi = (orr + rows.size() * sizeof(UdRows) + 255) / 256 * 256;
+        const size_t ov = oi + sidx.size() * sizeof(long long);
+        const size_t need = ov + sval.size() * sizeof(double) + 8;
+        if (need > F->mdesc_cap) {
+            std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+            HIPTRY(hipStreamSynchronize(sm));
+            ffree(F, F->d_mdesc);
+            F->d_mdesc = nullptr;
+            F->mdesc_cap = 0;
+            const size_t cap = std::max<size_t>(need, 1 << 16) * 3 / 2;
+            int rc = falloc(F, &F->d_mdesc, cap);
+            if (rc) return rc;
+            F->mdesc_cap = cap;
+        }
+        char *dm = (char *)F->d_mdesc;
+        HIPTRY(hipMemcpyAsync(dm + ob, blocks.data(), blocks.size() * sizeof(UdBlock), hipMemcpyHostToDevice, sm));
+        if (!rows.empty())
+            HIPTRY(hipMemcpyAsync(dm + orr, rows.data(), rows.size() * sizeof(UdRows), hipMemcpyHostToDevice, sm));
+        if (!sidx.empty()) {
+            HIPTRY(hipMemcpyAsync(dm + oi, sidx.data(), sidx.size() * sizeof(long long), hipMemcpyHostToDevice, sm));
+            HIPTRY(hipMemcpyAsync(dm + ov, sval.data(), sval.size() * sizeof(double), hipMemcpyHostToDevice, sm));
+        }
+        const UdBlock *d_blocks = (const UdBlock *)(dm + ob);
+        const UdRows *d_rows = (const UdRows *)(dm + orr);
+        hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n * UD_RANK, 256)), dim3(256), 0, sm, Wc,
+                           (long long)(n * UD_RANK), (T)0);
+        hipLaunchKernelGGL(k_fill<T>, dim3(1), dim3(256), 0, sm, alpha, (long long)UD_RANK, (T)1);
+        if (!sidx.empty())
+            hipLaunchKernelGGL(k_updown_scatter<T>, dim3(grid_for((int64_t)sidx.size(), 256)), dim3(256), 0, sm,
+                               (const long long *)(dm + oi), (const double *)(dm + ov), (long long)sidx.size(), Wc);
+        for (const UdLaunch &L : launches) {
+            if (L.nb > 0)
+                hipLaunchKernelGGL(k_updown_block<T>, dim3((unsigned)L.nb), dim3(64), 0, sm, d_blocks + L.b0, W, Wc,
+                                   alpha, par, update, dbound, F->d_mflag);
+            if (L.nr > 0)
+                hipLaunchKernelGGL(k_updown_rows<T>, dim3((unsigned)L.nr), dim3(64), 0, sm, d_blocks,
+                                   d_rows + L.r0, W, (const int *)F->d_Ls, Wc, (const T *)par);
+        }
+        HIPTRY(hipGetLastError());
+        HIPTRY(hipStreamSynchronize(sm));     // the host arrays above are read by the queued copies
+    }
+    int flags[2] = {INT_MAX, 0};
+    HIPTRY(hipMemcpyAsync(flags, F->d_mflag, sizeof(flags), hipMemcpyDeviceToHost, sm));
+    HIPTRY(hipStreamSynchronize(sm));
+    *modfl = fl;
+    *fail = flags[0] == INT_MAX ? n : flags[0];
+    *nbound = flags[1];
+    return MXC_OK;
+}
+
+extern "C" int mxc_updown(mxc_factor *F, int update, int64_t ncol, const int64_t *Cp, const int64_t *Ci,
+                          const double *Cx, double dbound, double *modfl, int64_t *info) {
+    if (modfl) *modfl = 0;
+    if (info) info[0] = info[1] = 0;
+    if (!F || ncol < 0 || (ncol > 0 && !Cp)) return MXC_INVALID;
+    const int64_t n = F->n;
+    if (ncol > 0 && (Cp[0] != 0 || (Cp[ncol] > 0 && (!Ci || !Cx)))) return MXC_INVALID;
+    if (!F->values_loaded) {
+        snprintf(g_err, sizeof(g_err), "updown: the factor holds no numeric values");
+        return MXC_INVALID;
+    }
+    mxc_factor *G = active(F);
+    updown_structure(G);
+    for (int64_t c = 0; c < ncol; c++) {
+        if (Cp[c + 1] < Cp[c]) return MXC_INVALID;
+        if (Cp[c + 1] == Cp[c]) continue;
+        const int64_t k = Ci[Cp[c]];
+        if (k < 0 || k >= n) return MXC_INVALID;
+        const int64_t s = G->col_super[k], k2 = G->Super[s + 1];
+        int64_t q = G->Lpi[s] + (k2 - G->Super[s]);
+        const int64_t qend = G->Lpi[s + 1];
+        for (int64_t p = Cp[c]; p < Cp[c + 1]; p++) {
+            const int64_t i = Ci[p];
+            if (i < 0 || i >= n || (p > Cp[c] && i < Ci[p - 1])) {
+                snprintf(g_err, sizeof(g_err), "updown: column %lld of C is not sorted", (long long)c);
+                return MXC_INVALID;
+            }
+            if (i < k2) continue;
+            while (q < qend && G->Ls_host[q] < i) q++;
+            if (q == qend || G->Ls_host[q] != i) {
+                snprintf(g_err, sizeof(g_err),
+                         "updown: C(%lld, %lld) lies outside the pattern of L(:, %lld); the modification "
+                         "needs new fill (analyze A + CC' instead)", (long long)i, (long long)c, (long long)k);
+                return MXC_INVALID;
+            }
+        }
+    }
+    if (ncol == 0 || n == 0) return MXC_OK;
+    HIPTRY(hipSetDevice(g_device));
+    if (F->fallback && !F->escalated) {          // an unused FP64 twin no longer matches
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+        mxc_factor_free(F->fallback);
+        F->fallback = nullptr;
+    }
+    double fl = 0;
+    int64_t fail = n, nbound = 0;
+    int rc = G->precision == MXC_FP64
+                 ? updown_run<double>(G, update, ncol, Cp, Ci, Cx, dbound, &fl, &fail, &nbound)
+                 : updown_run<float>(G, update, ncol, Cp, Ci, Cx, dbound, &fl, &fail, &nbound);
+    if (rc) return rc;
+    F->modified = G->modified = true;
+    if (modfl) *modfl = fl;
+    if (info) {
+        info[0] = fail;
+        info[1] = nbound;
+    }
+    if (fail < n) {
+        F->values_loaded = G->values_loaded = false;
+        F->minor = G->minor = fail;
+        snprintf(g_err, sizeof(g_err), "updown: the new pivot of column %lld is not positive; the factor is "
+                                       "no longer valid", (long long)fail);
+        return MXC_NOT_POSDEF;
+    }
+    return MXC_OK;
 }
 
 extern "C" int mxc_synchronize(mxc_factor *F) {
