@@ -2807,8 +2807,12 @@ static int updown_run(mxc_factor *F, int update, int64_t ncol, const int64_t *Cp
                     }
                     starts[task_of[s]][r] = (int)st;
                 }
+                // the path leaves s at its first row below, which may lie inside the parent
+                const int64_t nscol = F->Super[s + 1] - F->Super[s], nsrow = F->Lpi[s + 1] - F->Lpi[s];
+                if (nsrow == nscol) break;
+                const int64_t pj = F->Ls_host[F->Lpi[s] + nscol];
                 s = F->sparent_h[s];
-                st = 0;
+                st = pj - F->Super[s];
             }
         }
         for (int64_t s : ts) task_of[s] = -1;
