@@ -17,7 +17,8 @@
 // it).  The rows below the block take the block's 64 columns of pivot data in
 // a second launch (k_updown_rows), a lane per row.  Every operation is
 // written out and contraction is off, so the result does not depend on the
-// block a column falls in or on the launch that takes its rows.
+// block a column falls in or on the launch that takes its rows; a column that
+// no path reaches is not touched (its LDL' round trip would not be exact).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "mxc_kernels.h"
@@ -77,10 +78,14 @@ __global__ __launch_bounds__(64) void k_updown_block(const UdBlock *__restrict__
     const int last = b.c0 + nb - 1;
     const bool own = lane < UD_RANK && blocks[blockIdx.x].start[lane] <= last;   // rank `lane` passes here
     if (own) al[lane] = alpha[lane];
+    int first = INT_MAX;                   // columns before every path are left alone
+#pragma unroll
+    for (int r = 0; r < UD_RANK; r++) first = min(first, b.start[r]);
     __syncthreads();
 
     for (int c = 0; c < nb; c++) {
         const int jl = b.c0 + c;
+        if (jl < first) continue;
         if (lane == c) {
             const T lold = S[c][c];
             T dj = lold * lold;
@@ -153,6 +158,9 @@ __global__ __launch_bounds__(64) void k_updown_rows(const UdBlock *__restrict__ 
     const int lane = threadIdx.x, nb = b.nb;
     const T *ps = par + (long long)b.slot * NB * UD_PAR;
     for (int e = lane; e < nb * UD_PAR; e += 64) P[e] = ps[e];
+    int first = INT_MAX;                   // columns before every path are left alone
+#pragma unroll
+    for (int r = 0; r < UD_RANK; r++) first = min(first, b.start[r]);
     const int ii = rr.r0 + lane;
     const bool on = ii < b.nsrow;
     const long long g = on ? (long long)(ii < b.nscol ? b.k1 + ii : Ls[b.psi + ii]) * UD_RANK : 0;
@@ -171,6 +179,7 @@ __global__ __launch_bounds__(64) void k_updown_rows(const UdBlock *__restrict__ 
             const int c = c0 + q;
             if (c >= nb) break;
             const int jl = b.c0 + c;
+            if (jl < first) continue;
             const T *pc = P + c * UD_PAR;
             T l = v[q] / pc[0];
 #pragma unroll
