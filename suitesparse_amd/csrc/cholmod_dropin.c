@@ -15,6 +15,8 @@
  *   cholmod_l_factorize_p   cholmod.h:2794
  *   cholmod_l_solve         cholmod.h:2823  (Cholesky/cholmod_solve.c:130-724)
  *   cholmod_l_solve2        cholmod.h:2845
+ *   cholmod_l_updown        cholmod.h:3465  (Modify/cholmod_updown.c:74; on the device
+ *                                            factor when C stays in L's pattern)
  *   cholmod_l_free_factor   cholmod.h:1692  (releases the factor's device plan first)
  *   cholmod_l_finish        cholmod.h:1035  (releases the Common's device pool:
  *                                            GPU/cholmod_gpu.c:234-260, gpu_deallocate)
@@ -31,12 +33,12 @@
  * Those calls go to the next definition of the same entry point (the host
  * library), found with dlsym(RTLD_NEXT).
  *
- * Analysis is this library's own (mxc_analyze: the given or natural ordering,
- * or AMD, bit-identical to cholmod_l_analyze_p's).  METIS is not part of it:
- * the default strategy is the reference's without the Partition module
- * (UserPerm if given, then AMD: cholmod_analyze.c:470-489), and a METIS or
- * NESDIS method reports CHOLMOD_NOT_INSTALLED and falls back to AMD as the
- * reference's amd_backup does.
+ * Analysis is this library's own: the given or natural ordering, AMD and
+ * METIS (mxc_amd_order_stats, mxc_metis_order), and CHOLMOD's default
+ * strategy over them (UserPerm if given, AMD, then METIS unless AMD's
+ * prediction is already good: cholmod_analyze.c:554-782), bit-identical to
+ * cholmod_l_analyze_p's.  NESDIS, non-default AMD controls and postorder =
+ * FALSE go to the host library.
  *
  * Device state.  Each factor L that reaches the device owns an entry in a side
  * table keyed by the cholmod_factor pointer (with L->Perm / L->super as stamps
@@ -78,6 +80,7 @@ typedef cholmod_dense *(*solve_fn)(int, cholmod_factor *, cholmod_dense *, cholm
 typedef int (*solve2_fn)(int, cholmod_factor *, cholmod_dense *, cholmod_sparse *, cholmod_dense **,
                          cholmod_sparse **, cholmod_dense **, cholmod_dense **, cholmod_common *);
 typedef int (*free_factor_fn)(cholmod_factor **, cholmod_common *);
+typedef int (*updown_fn)(int, cholmod_sparse *, cholmod_factor *, cholmod_common *);
 typedef int (*finish_fn)(cholmod_common *);
 typedef int (*gpu_stats_fn)(cholmod_common *);
 
@@ -884,6 +887,94 @@ int cholmod_l_solve2(int sys, cholmod_factor *L, cholmod_dense *B, cholmod_spars
         pthread_mutex_unlock(&g_tab_mu);
     }
     return NEXT(solve2_fn, "cholmod_l_solve2")(sys, L, B, Bset, X_Handle, Xset_Handle, Y_Handle, E_Handle, cm);
+}
+
+/* ------------------------------------------------------------------------ */
+/* update / downdate                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* cholmod_l_updown on the device factor when L's device copy is current and C
+ * is real, double and sorted: mxc_updown modifies it in place (L stays a
+ * supernodal LL' factor) and L->x is refreshed.  The host library takes the
+ * rest from L->x as it was before the call, and turns L into a simplicial
+ * LDL' factor as the reference does: C with an entry outside the pattern of
+ * L(:, first row of its column) (new fill), a downdate whose new pivot is not
+ * positive (the reference keeps that indefinite LDL'; the device factor is
+ * dropped), and every other input. */
+int cholmod_l_updown(int update, cholmod_sparse *C, cholmod_factor *L, cholmod_common *cm) {
+    if (!cm) return 0;
+    if (!C || !L) {
+        report(cm, CHOLMOD_INVALID, "argument missing");
+        return 0;
+    }
+    const char *keep = getenv("CHOLMOD_MI355X_KEEP_L_ON_DEVICE");
+    const int lx_on_host = !(keep && atoi(keep) != 0);
+    pthread_mutex_lock(&g_tab_mu);
+    entry *e = fresh_entry(L);
+    if (e && L->is_super && C->xtype == CHOLMOD_REAL && C->dtype == CHOLMOD_DOUBLE && C->sorted &&
+        C->nrow == L->n) {
+        const int64_t ncol = (int64_t)C->ncol;
+        const int64_t *Cp = (const int64_t *)C->p, *Ci = (const int64_t *)C->i;
+        const double *Cx = (const double *)C->x;
+        int64_t *pp = NULL, *pi = NULL;
+        double *px = NULL;
+        if (!C->packed) {                 /* a packed copy of an unpacked C */
+            const int64_t *Cnz = (const int64_t *)C->nz;
+            int64_t nz = 0;
+            for (int64_t c = 0; c < ncol; c++) nz += Cnz[c];
+            pp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(ncol + 1));
+            pi = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nz > 0 ? nz : 1));
+            px = (double *)malloc(sizeof(double) * (size_t)(nz > 0 ? nz : 1));
+            if (!pp || !pi || !px) {
+                free(pp), free(pi), free(px);
+                pthread_mutex_unlock(&g_tab_mu);
+                report(cm, CHOLMOD_OUT_OF_MEMORY, "out of memory");
+                return 0;
+            }
+            pp[0] = 0;
+            for (int64_t c = 0; c < ncol; c++) {
+                memcpy(pi + pp[c], Ci + Cp[c], sizeof(int64_t) * (size_t)Cnz[c]);
+                memcpy(px + pp[c], Cx + Cp[c], sizeof(double) * (size_t)Cnz[c]);
+                pp[c + 1] = pp[c] + Cnz[c];
+            }
+            Cp = pp, Ci = pi, Cx = px;
+        }
+        double modfl = 0;
+        int64_t info[2] = {0, 0};
+        int st = mxc_updown(e->F, update, ncol, Cp, Ci, Cx, cm->dbound, &modfl, info);
+        free(pp), free(pi), free(px);
+        if (st == MXC_OK) {
+            if (lx_on_host) st = mxc_factor_get_Lx(e->F, (double *)L->x);
+            pthread_mutex_unlock(&g_tab_mu);
+            if (st != MXC_OK) {
+                report(cm, st, mxc_last_error());
+                return 0;
+            }
+            cm->status = CHOLMOD_OK;
+            cm->modfl = modfl;
+            cm->ndbounds_hit += (double)info[1];
+            if (info[1] > 0) report(cm, CHOLMOD_DSMALL, "diagonal entry is below threshold");
+            return 1;
+        }
+        if (st != MXC_INVALID && st != MXC_NOT_POSDEF) {
+            pthread_mutex_unlock(&g_tab_mu);
+            report(cm, st, mxc_last_error());
+            return 0;
+        }
+        if (!lx_on_host) {
+            /* L->x was never filled: copy the device factor out first, if it still holds L */
+            if (st == MXC_NOT_POSDEF || mxc_factor_get_Lx(e->F, (double *)L->x) != MXC_OK) {
+                pthread_mutex_unlock(&g_tab_mu);
+                tab_drop(L);
+                report(cm, st == MXC_NOT_POSDEF ? CHOLMOD_NOT_POSDEF : CHOLMOD_GPU_PROBLEM,
+                       "update/downdate needs the host factor, which was kept on the device only");
+                return 0;
+            }
+        }
+    }
+    pthread_mutex_unlock(&g_tab_mu);
+    tab_drop(L);                       /* the host path changes L */
+    return NEXT(updown_fn, "cholmod_l_updown")(update, C, L, cm);
 }
 
 /* ------------------------------------------------------------------------ */

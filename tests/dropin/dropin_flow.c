@@ -10,6 +10,11 @@
  *
  *   dropin_flow <matrix file> [supernodal] [gpu|host]
  *
+ * After the demo's sequence, a fresh factor takes a rank-2 update with C
+ * inside L's pattern (cholmod_l_updown, Modify/cholmod_updown.c:74), a solve
+ * against A + P'CC'P, the same downdate (back to A), and an update whose C
+ * needs new fill (which the drop-in hands to the host library).
+ *
  * supernodal: Common->supernodal = CHOLMOD_SUPERNODAL (default: CHOLMOD_AUTO).
  * gpu / host: run only that side (default: both, and compare).
  */
@@ -33,6 +38,10 @@ typedef struct {
     int ordering, is_super_after_bset;
     int selected, called_nd;             /* Common->selected, called_nd after analyze */
     double anz, method_lnz[3];           /* Common->anz, method[0..2].lnz */
+    /* update / downdate */
+    int ud_done, ud_status, ud_is_super, fill_status, fill_is_super;
+    double ud_modfl, ud_resid, back_err, fill_modfl, fill_resid;
+    cholmod_dense *Xup, *Xfill;
 } side;
 
 static double norm_inf_dense(cholmod_dense *X) {
@@ -53,6 +62,104 @@ static double resid_of(side *s, cholmod_dense *X) {
     double v = r / (anorm * norm_inf_dense(X) + norm_inf_dense(s->B) + (s->A->nrow == 0));
     cholmod_l_free_dense(&R, &s->c);
     return v;
+}
+
+/* |b - (A + P'CC'P) x|_inf / (|A|_inf |x|_inf + |b|_inf) */
+static double resid_mod(side *s, cholmod_sparse *C, cholmod_dense *X) {
+    const size_t n = s->A->nrow;
+    double one[2] = {1, 0}, minusone[2] = {-1, 0};
+    cholmod_dense *R = cholmod_l_copy_dense(s->B, &s->c);
+    cholmod_l_sdmult(s->A, 0, minusone, one, X, R, &s->c);
+    const int64_t *Perm = (const int64_t *)s->L->Perm, *Cp = (const int64_t *)C->p, *Ci = (const int64_t *)C->i;
+    const double *Cx = (const double *)C->x, *x = (const double *)X->x;
+    double *r = (double *)R->x;
+    for (size_t c = 0; c < C->ncol; c++) {
+        double t = 0;                      /* C(:,c)' P x */
+        for (int64_t p = Cp[c]; p < Cp[c + 1]; p++) t += Cx[p] * x[Perm[Ci[p]]];
+        for (int64_t p = Cp[c]; p < Cp[c + 1]; p++) r[Perm[Ci[p]]] -= Cx[p] * t;
+    }
+    double v = cholmod_l_norm_dense(R, 0, &s->c) /
+               (cholmod_l_norm_sparse(s->A, 0, &s->c) * norm_inf_dense(X) + norm_inf_dense(s->B) + (n == 0));
+    cholmod_l_free_dense(&R, &s->c);
+    return v;
+}
+
+/* C (n x ncol) with column c starting at row ks[c] and taking every other row
+ * of the pattern of L(:, ks[c]) (supernodal L); the values are fixed numbers. */
+static cholmod_sparse *pattern_c(side *s, const int64_t *ks, int ncol, double scale) {
+    const cholmod_factor *L = s->L;
+    const int64_t *Super = (const int64_t *)L->super, *Lpi = (const int64_t *)L->pi, *Ls = (const int64_t *)L->s;
+    cholmod_sparse *C = cholmod_l_allocate_sparse(L->n, ncol, L->n * ncol + 1, 1, 1, 0,
+                                                  CHOLMOD_REAL + CHOLMOD_DOUBLE, &s->c);
+    int64_t *Cp = (int64_t *)C->p, *Ci = (int64_t *)C->i;
+    double *Cx = (double *)C->x;
+    Cp[0] = 0;
+    for (int c = 0; c < ncol; c++) {
+        const int64_t k = ks[c];
+        int64_t sn = 0;
+        while (Super[sn + 1] <= k) sn++;
+        int64_t q = Cp[c];
+        for (int64_t p = Lpi[sn] + (k - Super[sn]); p < Lpi[sn + 1]; p += 2) {
+            Ci[q] = Ls[p];
+            Cx[q] = scale * (0.3 + 0.1 * (double)(q % 7));
+            q++;
+        }
+        Cp[c + 1] = q;
+    }
+    return C;
+}
+
+static void updown_step(side *s) {
+    const int64_t n = (int64_t)s->A->nrow;
+    /* a row outside the pattern of L(:, 0), for the update that needs fill
+       (found now: the host library leaves L simplicial after its first updown) */
+    int64_t out = -1;
+    {
+        const int64_t *Lpi = (const int64_t *)s->L->pi, *Ls = (const int64_t *)s->L->s;
+        for (int64_t r = 1, p = Lpi[0] + 1; r < n; r++) {
+            while (p < Lpi[1] && Ls[p] < r) p++;
+            if (p == Lpi[1] || Ls[p] != r) {
+                out = r;
+                break;
+            }
+        }
+    }
+    int64_t ks[2] = {n / 3, (2 * n) / 3};
+    cholmod_sparse *C = pattern_c(s, ks, 2, 1.0);
+    s->ud_status = cholmod_l_updown(1, C, s->L, &s->c) ? s->c.status : -100 + s->c.status;
+    s->ud_modfl = s->c.modfl;
+    s->ud_is_super = s->L->is_super;
+    s->Xup = cholmod_l_solve(CHOLMOD_A, s->L, s->B, &s->c);
+    s->ud_resid = s->Xup ? resid_mod(s, C, s->Xup) : -1;
+    /* the same downdate: back to A */
+    cholmod_l_updown(0, C, s->L, &s->c);
+    cholmod_dense *Xb = cholmod_l_solve(CHOLMOD_A, s->L, s->B, &s->c);
+    s->back_err = 0;
+    const double xmax = norm_inf_dense(s->X) + (n == 0);
+    for (int64_t i = 0; Xb && i < n; i++)
+        s->back_err = fmax(s->back_err, fabs(((double *)Xb->x)[i] - ((double *)s->X->x)[i]) / xmax);
+    cholmod_l_free_dense(&Xb, &s->c);
+    cholmod_l_free_sparse(&C, &s->c);
+    /* an update that needs new fill: row 0 and a row outside the pattern of L(:, 0) */
+    s->fill_status = -1000;
+    {
+        if (out > 0) {
+            cholmod_sparse *F = cholmod_l_allocate_sparse(n, 1, 2, 1, 1, 0, CHOLMOD_REAL + CHOLMOD_DOUBLE, &s->c);
+            ((int64_t *)F->p)[0] = 0;
+            ((int64_t *)F->p)[1] = 2;
+            ((int64_t *)F->i)[0] = 0;
+            ((int64_t *)F->i)[1] = out;
+            ((double *)F->x)[0] = 0.7;
+            ((double *)F->x)[1] = -0.4;
+            s->fill_status = cholmod_l_updown(1, F, s->L, &s->c) ? s->c.status : -100 + s->c.status;
+            s->fill_modfl = s->c.modfl;
+            s->fill_is_super = s->L->is_super;
+            s->Xfill = cholmod_l_solve(CHOLMOD_A, s->L, s->B, &s->c);
+            s->fill_resid = s->Xfill ? resid_mod(s, F, s->Xfill) : -1;
+            cholmod_l_free_sparse(&F, &s->c);
+        }
+    }
+    s->ud_done = 1;
 }
 
 static int run(side *s, const char *path, int gpu, int supernodal) {
@@ -144,6 +251,13 @@ static int run(side *s, const char *path, int gpu, int supernodal) {
     cholmod_l_free_dense(&R, &s->c);
     cholmod_l_free_dense(&R2, &s->c);
     cholmod_l_free_dense(&Xir, &s->c);
+
+    /* a fresh factor for the update / downdate (the Bset solves left L simplicial) */
+    cholmod_l_free_factor(&s->L, &s->c);
+    s->L = cholmod_l_analyze(s->A, &s->c);
+    if (!s->L) return 4;
+    cholmod_l_factorize(s->A, s->L, &s->c);
+    if (s->c.status == CHOLMOD_OK && s->L->is_super && n > 0) updown_step(s);
     return 0;
 }
 
@@ -161,9 +275,17 @@ static void report(const char *tag, side *s) {
     printf("%s_gpu_calls %.0f %.0f %.0f %.0f\n", tag, (double)s->c.cholmod_gpu_syrk_calls,
            (double)s->c.cholmod_gpu_gemm_calls, (double)s->c.cholmod_gpu_potrf_calls,
            (double)s->c.cholmod_gpu_trsm_calls);
+    if (s->ud_done) {
+        printf("%s_ud_status %d\n%s_ud_modfl %.17g\n%s_ud_is_super %d\n%s_ud_resid %.6e\n%s_back_err %.6e\n", tag,
+               s->ud_status, tag, s->ud_modfl, tag, s->ud_is_super, tag, s->ud_resid, tag, s->back_err);
+        printf("%s_fill_status %d\n%s_fill_modfl %.17g\n%s_fill_is_super %d\n%s_fill_resid %.6e\n", tag,
+               s->fill_status, tag, s->fill_modfl, tag, s->fill_is_super, tag, s->fill_resid);
+    }
 }
 
 static void finish(side *s) {
+    cholmod_l_free_dense(&s->Xup, &s->c);
+    cholmod_l_free_dense(&s->Xfill, &s->c);
     cholmod_l_free_factor(&s->L, &s->c);
     cholmod_l_free_dense(&s->X, &s->c);
     cholmod_l_free_dense(&s->B, &s->c);
@@ -216,6 +338,21 @@ int main(int argc, char **argv) {
                 den += b * b;
             }
             printf("x_rel_diff %.6e\n", den > 0 ? sqrt(num / den) : sqrt(num));
+        }
+        if (g.Xup && h.Xup) {
+            double num = 0, den = 0, fnum = 0, fden = 0;
+            for (size_t i = 0; i < g.Xup->nrow; i++) {
+                const double a = ((double *)g.Xup->x)[i], b = ((double *)h.Xup->x)[i];
+                num += (a - b) * (a - b);
+                den += b * b;
+                if (g.Xfill && h.Xfill) {
+                    const double fa = ((double *)g.Xfill->x)[i], fb = ((double *)h.Xfill->x)[i];
+                    fnum += (fa - fb) * (fa - fb);
+                    fden += fb * fb;
+                }
+            }
+            printf("xup_rel_diff %.6e\n", den > 0 ? sqrt(num / den) : sqrt(num));
+            if (g.Xfill && h.Xfill) printf("xfill_rel_diff %.6e\n", fden > 0 ? sqrt(fnum / fden) : sqrt(fnum));
         }
         printf("perm_equal %d\n", g.L && h.L && g.L->n == h.L->n &&
                !memcmp(g.L->Perm, h.L->Perm, g.L->n * sizeof(int64_t)));

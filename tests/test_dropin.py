@@ -17,7 +17,7 @@ FLOW = ROOT / "tests" / "dropin" / "_build" / "dropin_flow"
 FIX = ROOT / "tests" / "golden" / "cholmod_matrices"
 ENTRY_POINTS = ["cholmod_l_analyze", "cholmod_l_analyze_p", "cholmod_l_factorize", "cholmod_l_factorize_p",
                 "cholmod_l_solve", "cholmod_l_solve2", "cholmod_l_free_factor", "cholmod_l_finish",
-                "cholmod_l_gpu_stats"]
+                "cholmod_l_gpu_stats", "cholmod_l_updown"]
 
 
 def run_flow(path, *args, env=None):

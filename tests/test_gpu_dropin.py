@@ -3,7 +3,9 @@ reference compiled from its sources) in the same process: the caller
 tests/dropin/dropin_flow.c runs the reference demo's sequence
 (CHOLMOD/Demo/cholmod_dl_demo.c:287-759: read, analyze, factorize three times,
 rcond, solve, solve2 with reused workspaces, solve2 with a sparse right-hand
-side, one step of refinement) once with Common->useGPU = 1 and once with 0.
+side, one step of refinement), then a rank-2 update and downdate
+(cholmod_l_updown) of a fresh factor and an update that needs new fill, once
+with Common->useGPU = 1 and once with 0.
 
 Checked per matrix: the same ordering (Perm, the method CHOLMOD's default
 strategy selected, its fl / lnz and the per-method lnz), status and L->minor; the
@@ -52,6 +54,20 @@ def _check(v, supernodal_expected=None):
         assert v["gpu_is_super_after_bset"] == v["host_is_super_after_bset"]
         rg, rh = float(v["gpu_rcond"]), float(v["host_rcond"])
         assert abs(rg - rh) <= 1e-10 * rh
+    if "gpu_ud_status" in v or "host_ud_status" in v:
+        # cholmod_l_updown: the device keeps L supernodal LL', the host turns it
+        # simplicial LDL'; the same modfl, solutions of A + P'CC'P, and the same
+        # downdate back to A.  C that needs fill goes to the host on both sides.
+        assert v["gpu_ud_status"] == v["host_ud_status"] == "0"
+        assert v["gpu_ud_modfl"] == v["host_ud_modfl"]
+        assert v["gpu_ud_is_super"] == "1" and v["host_ud_is_super"] == "0"
+        assert float(v["gpu_ud_resid"]) <= 1e-13 and float(v["xup_rel_diff"]) <= 1e-10
+        assert float(v["gpu_back_err"]) <= 1e-10
+        assert v["gpu_fill_status"] == v["host_fill_status"]
+        if v["host_fill_status"] == "0":
+            assert v["gpu_fill_modfl"] == v["host_fill_modfl"]
+            assert v["gpu_fill_is_super"] == "0"
+            assert float(v["gpu_fill_resid"]) <= 1e-13 and float(v["xfill_rel_diff"]) <= 1e-10
 
 
 @pytest.mark.parametrize("mode", ["supernodal", "auto"])

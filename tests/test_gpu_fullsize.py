@@ -18,6 +18,8 @@ BLAS, FP64) factors and solves the same seeded inputs on the host cores:
   relative residual <= 1e-10 in FP64 and in FP32 + refinement; forward error
   against the reference (each system alone) on a seeded sample of 64 systems,
   <= 1e-10 for FP64.
+* 64^3 again: a rank-8 update of the resident factor (cholmod_l_updown) equals
+  the oracle's bit for bit, with the reference's modfl and updated solution.
 * cholmod_l_factorize_p with beta != 0 (A + beta I), including a matrix that is
   not positive definite until beta is added and one that stays indefinite
   (status, L->minor and L against the reference).
@@ -80,6 +82,24 @@ def test_diffusion3d_64_fp64_full(gpu, ref_lib):
     rr = _relres_cols(A, B, X).max()
     print(f"  fwd err vs ref {fe:.3e}, vs x_true {fe_true:.3e}, relres {rr:.3e}", flush=True)
     assert fe <= 1e-10 and rr <= 1e-12
+    # a rank-8 update of the resident factor (cholmod_l_updown): bit for bit the
+    # oracle's on the same values, modfl and the updated solution as the reference's
+    from oracle import restate
+    from updown_cases import pattern_columns, sym_of
+    rng = np.random.default_rng(31)
+    Cp, Ci, Cx = pattern_columns(sym_of(L), rng.integers(0, A.n, 8), rng, dens=0.3)
+    o = restate.updown(sym_of(L), L.x, True, Cp, Ci, Cx)
+    assert cholmod.updown(True, (Cp, Ci, Cx), L, cm) and cm.modfl == o["modfl"]
+    assert np.array_equal(L.x, o["Lx"])
+    del o
+    r = R.updown(True, Cp, Ci, Cx)
+    assert r["modfl"] == cm.modfl
+    Xu = cholmod.solve(cholmod.CHOLMOD_A, L, B, cm)
+    Xu_ref = R.solve(B)
+    fe_u = np.linalg.norm(Xu - Xu_ref) / np.linalg.norm(Xu_ref)
+    print(f"  rank-8 update: modfl {cm.modfl:.6g}, fwd err vs ref {fe_u:.3e}, reference updown "
+          f"{r['t']:.3f} s", flush=True)
+    assert fe_u <= 1e-10
     L.free()
 
 
