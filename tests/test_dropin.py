@@ -28,7 +28,7 @@ def run_flow(path, *args, env=None):
     vals = {}
     for line in out.stdout.splitlines():
         k, _, v = line.partition(" ")
-        if k and v and (k.startswith(("gpu_", "host_")) or k in ("L_rel_diff", "x_rel_diff", "perm_equal")):
+        if k and v and (k.startswith(("gpu_", "host_")) or k in ("L_rel_diff", "x_rel_diff", "xup_rel_diff", "xfill_rel_diff", "perm_equal")):
             vals[k] = v
     return out, vals
 
