@@ -572,6 +572,7 @@ def section_64(args) -> dict:
                                 "traffic_bytes_per_launch": read_traffic(),
                                 "launches_per_step": c["launches"]["update"] / c["steps"]}
     log(f"[bench] 64^3 fp64: {ms:.1f} ms/step, {value:.1f} GFLOP/s")
+    sec["modify"] = modify_record(A, perm, L, args)
     L.free()
     if not args.no_cpu:
         cpu = host_cpu_info()
@@ -585,6 +586,65 @@ def section_64(args) -> dict:
         except Exception as e:
             sec["cpu_baseline"] = {"value": None, "error": str(e)}
     return sec
+
+
+def modify_columns(A, perm, ks, scale=0.1):
+    """C whose column c is column ks[c] of tril(P A P') times scale: inside the
+    pattern of L(:, ks[c]), so the device modifies the resident factor."""
+    import scipy.sparse as sp
+    Al = sp.csc_matrix((A.x, A.i, A.p), shape=(A.n, A.n))
+    Af = (Al + sp.tril(Al, -1).T).tocsc()
+    Pa = sp.tril(Af[perm][:, perm]).tocsc()
+    Pa.sort_indices()
+    cols = [Pa[:, int(k)] for k in ks]
+    C = sp.hstack(cols).tocsc() * scale
+    C.sort_indices()
+    return (C.indptr.astype(np.int64), C.indices.astype(np.int64), C.data.astype(np.float64))
+
+
+def modify_record(A, perm, L, args) -> dict:
+    """cholmod_l_updown on the resident 64^3 factor: rank-1 and rank-8 update
+    followed by the same downdate (the factor comes back to A's), wall time per
+    call (host plan + kernels; the call returns when L is modified).  The
+    reference times the same calls on the host cores (its first call also
+    converts the supernodal factor to simplicial LDL')."""
+    import torch
+    from suitesparse_amd import cholmod
+    cm = cholmod.Common()
+    rng = np.random.default_rng(31)
+    rec = {"workload": "update then downdate of the 64^3 factor by columns of tril(PAP') x 0.1, "
+                       "C in L's order, FP64"}
+    Cs = {k: modify_columns(A, perm, rng.integers(0, A.n // 2, k)) for k in (1, 8)}
+    for k, C in Cs.items():
+        ts = []
+        for _ in range(3):
+            for upd in (True, False):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ok = cholmod.updown(upd, C, L, cm)
+                ts.append(time.perf_counter() - t0)
+                assert ok and cm.status == 0, cm.status
+        rec[f"rank{k}"] = {"ms_per_call": round(float(np.median(ts)) * 1e3, 3), "modfl": cm.modfl,
+                           "gflops": round(cm.modfl / float(np.median(ts)) / 1e9, 2)}
+    log(f"[bench] 64^3 updown: rank-1 {rec['rank1']['ms_per_call']} ms, rank-8 {rec['rank8']['ms_per_call']} ms")
+    if not args.no_cpu:
+        try:
+            from oracle import ref
+            cpu = host_cpu_info()
+            ref.set_threads(baseline_threads(cpu))
+            R = ref.RefFactor(A, ref.GIVEN, perm=perm)
+            R.factorize()
+            first = R.updown(True, *Cs[1])["t"]
+            t1 = R.updown(False, *Cs[1])["t"]
+            t8 = [R.updown(u, *Cs[8])["t"] for u in (True, False)]
+            rec["cpu_baseline"] = {"kind": "reference", "rank1_first_call_s": round(first, 4),
+                                   "rank1_s": round(t1, 4), "rank8_s": round(float(np.median(t8)), 4),
+                                   "sample": "the same C on the same factor; the first call converts L to "
+                                             "simplicial LDL'"}
+            rec["speedup_rank8_vs_cpu"] = round(float(np.median(t8)) / (rec["rank8"]["ms_per_call"] / 1e3), 2)
+        except Exception as e:
+            rec["cpu_baseline"] = {"error": str(e)}
+    return rec
 
 
 def batch_section(args, out):
