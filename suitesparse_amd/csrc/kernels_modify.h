@@ -13,9 +13,10 @@
 //   then l_jj = sqrt(d), l_ij = l * l_jj.
 // A 64-column block of a supernode is one workgroup (k_updown_block): lane c
 // forms column c's pivot data while the rows of the diagonal block follow one
-// column behind, then the block's new inverse is formed (the solve sweeps use
-// it).  The rows below the block take the block's 64 columns of pivot data in
-// a second launch (k_updown_rows), a lane per row.  Every operation is
+// column behind.  The rows below the block take the block's 64 columns of pivot
+// data in a second launch (k_updown_rows), a lane per row.  The modified
+// blocks' inverses (the solve sweeps use them) are formed once at the end, all
+// blocks in parallel (k_updown_inv).  Every operation is
 // written out and contraction is off, so the result does not depend on the
 // block a column falls in or on the launch that takes its rows; a column that
 // no path reaches is not touched (its LDL' round trip would not be exact).
@@ -63,7 +64,6 @@ __global__ __launch_bounds__(64) void k_updown_block(const UdBlock *__restrict__
                                                      int *__restrict__ flag) {
 #pragma clang fp contract(off)
     __shared__ T S[NB][DS];                // the diagonal block, S[i][c]
-    __shared__ T X[NB][DS];                // its inverse, X[i][c]
     __shared__ T P[NB][UD_PAR];
     __shared__ T al[UD_RANK];
     const UdBlock b = blocks[blockIdx.x];
@@ -133,17 +133,40 @@ __global__ __launch_bounds__(64) void k_updown_block(const UdBlock *__restrict__
     T *ps = par + (long long)b.slot * NB * UD_PAR;
     for (int e = lane; e < nb * UD_PAR; e += 64) ps[e] = P[e / UD_PAR][e % UD_PAR];
     if (own) alpha[lane] = al[lane];
+}
 
-    // the block's new inverse: lane c forms column c by forward substitution,
-    // X[m][c] collecting sum_k L[m][k] x_k until row m is reached
-    for (int m = 0; m < nb; m++) X[m][lane] = (T)0;
+// The new inverses of the modified diagonal blocks (the solve sweeps use
+// them), all at once after the last pass: a workgroup per block; lane c forms
+// column c of inv(L) by forward substitution, row after row,
+//   x_i = (delta_ic - sum_{c <= k < i} L_ik x_k) / L_ii,
+// with four partial sums.
+template <typename T>
+__global__ __launch_bounds__(64) void k_updown_inv(const DiagTask *__restrict__ tasks, T *__restrict__ W) {
+    __shared__ T S[NB][DS];
+    __shared__ T X[NB][DS];
+    __shared__ T rd[NB];
+    const DiagTask t = tasks[blockIdx.x];
+    const int lane = threadIdx.x, nb = t.nb;
+    for (int c = 0; c < nb; c++)
+        if (lane < nb && lane >= c) S[lane][c] = W[t.off + (long long)c * t.ld + lane];
+    if (lane < nb) rd[lane] = (T)1 / W[t.off + (long long)lane * t.ld + lane];
+    __syncthreads();
     for (int i = 0; i < nb; i++) {
-        const T xi = lane <= i ? ((i == lane ? (T)1 : (T)0) - X[i][lane]) / S[i][i] : (T)0;
-        X[i][lane] = xi;
-        for (int m = i + 1; m < nb; m++) X[m][lane] = X[m][lane] + S[m][i] * xi;
+        T s0 = (T)0, s1 = (T)0, s2 = (T)0, s3 = (T)0;
+        int k = lane;
+        for (; k + 3 < i; k += 4) {
+            s0 += S[i][k] * X[k][lane];
+            s1 += S[i][k + 1] * X[k + 1][lane];
+            s2 += S[i][k + 2] * X[k + 2][lane];
+            s3 += S[i][k + 3] * X[k + 3][lane];
+        }
+        for (; k < i; k++) s0 += S[i][k] * X[k][lane];
+        const T v = ((i == lane ? (T)1 : (T)0) - ((s0 + s1) + (s2 + s3))) * rd[i];
+        X[i][lane] = lane <= i ? v : (T)0;
     }
-    if (lane < nb)
-        for (int i = 0; i < nb; i++) W[b.winv + i + (long long)lane * nb] = X[i][lane];
+    __syncthreads();
+    for (int c = 0; c < nb; c++)
+        if (lane < nb) W[t.winv + lane + (long long)c * nb] = X[lane][c];
 }
 
 template <typename T>
@@ -169,31 +192,32 @@ __global__ __launch_bounds__(64) void k_updown_rows(const UdBlock *__restrict__ 
     for (int r = 0; r < UD_RANK; r++) w[r] = on ? Wc[g + r] : (T)0;
     __syncthreads();
     if (!on) return;
+    // the row's nb entries and their divisions by the old pivots first (they do
+    // not depend on w), then the chain through w, column after column
     T *col = W + b.lx + (long long)b.c0 * b.nsrow + ii;
-    for (int c0 = 0; c0 < nb; c0 += 16) {
-        T v[16];
+    T v[NB];
 #pragma unroll
-        for (int q = 0; q < 16; q++) v[q] = c0 + q < nb ? col[(long long)(c0 + q) * b.nsrow] : (T)0;
+    for (int c = 0; c < NB; c++) v[c] = c < nb ? col[(long long)c * b.nsrow] : (T)0;
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int c = c0 + q;
-            if (c >= nb) break;
-            const int jl = b.c0 + c;
-            if (jl < first) continue;
-            const T *pc = P + c * UD_PAR;
-            T l = v[q] / pc[0];
+    for (int c = 0; c < NB; c++)
+        if (c < nb && b.c0 + c >= first) v[c] = v[c] / P[c * UD_PAR];
 #pragma unroll
-            for (int r = 0; r < UD_RANK; r++) {
-                if (b.start[r] > jl) continue;
-                w[r] = w[r] - pc[2 + r] * l;
-                l = l - pc[2 + UD_RANK + r] * w[r];
-            }
-            v[q] = l * pc[1];
+    for (int c = 0; c < NB; c++) {
+        const int jl = b.c0 + c;
+        if (c >= nb || jl < first) continue;
+        const T *pc = P + c * UD_PAR;
+        T l = v[c];
+#pragma unroll
+        for (int r = 0; r < UD_RANK; r++) {
+            if (b.start[r] > jl) continue;
+            w[r] = w[r] - pc[2 + r] * l;
+            l = l - pc[2 + UD_RANK + r] * w[r];
         }
-#pragma unroll
-        for (int q = 0; q < 16; q++)
-            if (c0 + q < nb) col[(long long)(c0 + q) * b.nsrow] = v[q];
+        v[c] = l * pc[1];
     }
+#pragma unroll
+    for (int c = 0; c < NB; c++)
+        if (c < nb && b.c0 + c >= first) col[(long long)c * b.nsrow] = v[c];
     const int last = b.c0 + nb - 1;
 #pragma unroll
     for (int r = 0; r < UD_RANK; r++)

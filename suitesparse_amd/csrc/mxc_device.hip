@@ -2778,6 +2778,7 @@ static int updown_run(mxc_factor *F, int update, int64_t ncol, const int64_t *Cp
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, sm, F->d_mflag + 1, 0);
     double fl = 0;
     std::vector<int64_t> task_of(ns, -1);
+    std::map<long long, DiagTask> inv;          // modified blocks (by inverse slot), inverted at the end
     for (int64_t r0 = 0; r0 < ncol; r0 += UD_RANK) {
         const int k = (int)std::min<int64_t>(UD_RANK, ncol - r0);
         // the supernodes on the paths, with each rank's first column in them
@@ -2861,6 +2862,14 @@ static int updown_run(mxc_factor *F, int update, int64_t ncol, const int64_t *Cp
                     for (int r = 0; r < UD_RANK; r++) ub.start[r] = starts[order[q]][r];
                     ub.slot = slot++;
                     blocks.push_back(ub);
+                    DiagTask dt;
+                    dt.off = ub.lx + c0 * nsrow + c0;
+                    dt.winv = ub.winv;
+                    dt.ld = ub.nsrow;
+                    dt.nb = ub.nb;
+                    dt.col0 = (int)(k1 + c0);
+                    dt.pad = 0;
+                    inv[dt.winv] = dt;
                     for (int64_t rr = c0 + ub.nb; rr < nsrow; rr += 64)
                         rows.push_back(UdRows{(int)(blocks.size() - 1), (int)rr});
                 }
@@ -2912,6 +2921,27 @@ static int updown_run(mxc_factor *F, int update, int64_t ncol, const int64_t *Cp
         }
         HIPTRY(hipGetLastError());
         HIPTRY(hipStreamSynchronize(sm));     // the host arrays above are read by the queued copies
+    }
+    if (!inv.empty()) {
+        std::vector<DiagTask> tasks;
+        tasks.reserve(inv.size());
+        for (const auto &kv : inv) tasks.push_back(kv.second);
+        const size_t need = tasks.size() * sizeof(DiagTask);
+        if (need > F->mdesc_cap) {
+            std::lock_guard<std::recursive_mutex> lk(g_rt_mutex);
+            ffree(F, F->d_mdesc);
+            F->d_mdesc = nullptr;
+            F->mdesc_cap = 0;
+            const size_t cap = std::max<size_t>(need, 1 << 16) * 3 / 2;
+            int rc = falloc(F, &F->d_mdesc, cap);
+            if (rc) return rc;
+            F->mdesc_cap = cap;
+        }
+        HIPTRY(hipMemcpyAsync(F->d_mdesc, tasks.data(), need, hipMemcpyHostToDevice, sm));
+        hipLaunchKernelGGL(k_updown_inv<T>, dim3((unsigned)tasks.size()), dim3(64), 0, sm,
+                           (const DiagTask *)F->d_mdesc, W);
+        HIPTRY(hipGetLastError());
+        HIPTRY(hipStreamSynchronize(sm));
     }
     int flags[2] = {INT_MAX, 0};
     HIPTRY(hipMemcpyAsync(flags, F->d_mflag, sizeof(flags), hipMemcpyDeviceToHost, sm));
