@@ -69,8 +69,14 @@ __global__ __launch_bounds__(64) void k_updown_block(const UdBlock *__restrict__
     const UdBlock b = blocks[blockIdx.x];
     const int lane = threadIdx.x, nb = b.nb;
     const long long base = b.lx + (long long)b.c0 * b.nsrow + b.c0;   // element (c0, c0)
-    for (int c = 0; c < nb; c++)
-        if (lane < nb && lane >= c) S[lane][c] = W[base + (long long)c * b.nsrow + lane];
+    {
+        T v[NB];                           // every load of the row in flight, then the LDS stores
+#pragma unroll
+        for (int c = 0; c < NB; c++)
+            v[c] = (lane < nb && c <= lane) ? W[base + (long long)c * b.nsrow + lane] : (T)0;
+#pragma unroll
+        for (int c = 0; c < NB; c++) S[lane][c] = v[c];
+    }
     T w[UD_RANK];
     const long long g = (long long)(b.k1 + b.c0 + lane) * UD_RANK;
 #pragma unroll
@@ -147,8 +153,13 @@ __global__ __launch_bounds__(64) void k_updown_inv(const DiagTask *__restrict__ 
     __shared__ T rd[NB];
     const DiagTask t = tasks[blockIdx.x];
     const int lane = threadIdx.x, nb = t.nb;
-    for (int c = 0; c < nb; c++)
-        if (lane < nb && lane >= c) S[lane][c] = W[t.off + (long long)c * t.ld + lane];
+    {
+        T v[NB];
+#pragma unroll
+        for (int c = 0; c < NB; c++) v[c] = (lane < nb && c <= lane) ? W[t.off + (long long)c * t.ld + lane] : (T)0;
+#pragma unroll
+        for (int c = 0; c < NB; c++) S[lane][c] = v[c];
+    }
     if (lane < nb) rd[lane] = (T)1 / W[t.off + (long long)lane * t.ld + lane];
     __syncthreads();
     for (int i = 0; i < nb; i++) {
@@ -180,7 +191,13 @@ __global__ __launch_bounds__(64) void k_updown_rows(const UdBlock *__restrict__ 
     const UdBlock b = blocks[rr.blk];
     const int lane = threadIdx.x, nb = b.nb;
     const T *ps = par + (long long)b.slot * NB * UD_PAR;
-    for (int e = lane; e < nb * UD_PAR; e += 64) P[e] = ps[e];
+    {
+        T pv[UD_PAR];
+#pragma unroll
+        for (int q = 0; q < UD_PAR; q++) pv[q] = lane + 64 * q < nb * UD_PAR ? ps[lane + 64 * q] : (T)0;
+#pragma unroll
+        for (int q = 0; q < UD_PAR; q++) P[lane + 64 * q] = pv[q];
+    }
     int first = INT_MAX;                   // columns before every path are left alone
 #pragma unroll
     for (int r = 0; r < UD_RANK; r++) first = min(first, b.start[r]);
