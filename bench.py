@@ -151,6 +151,89 @@ def traffic_summary(fetch_dir: str, write_dir: str, calib_mib: int, cmd: str) ->
     return {"traffic_bytes_per_launch": total, **res}
 
 
+# kernel classes of the counter summaries: (label, test on the kernel name, algorithmic note)
+COUNTER_CLASSES = [
+    ("descendant update + scatter", lambda k: "k_tile<" in k and ", 0," in k or "k_tile_h<0" in k),
+    ("in-supernode trailing update", lambda k: "k_tile<" in k and ", 1," in k or "k_tile_h<1" in k),
+    ("panel TRSM", lambda k: "k_tile<" in k and ", 2," in k or "k_tile_h<2" in k or "k_trsm_tiny" in k),
+    ("diagonal POTRF", lambda k: "k_potrf" in k),
+    ("assembly (fill, scatter A, beta)", lambda k: any(t in k for t in ("k_fill", "k_scatter_a", "k_add_diag",
+                                                                         "k_gather_vals"))),
+    ("solve sweeps", lambda k: any(t in k for t in ("k_chain_", "k_sweep_", "k_perm_"))),
+    ("residual + norms", lambda k: any(t in k for t in ("k_residual", "k_sumsq"))),
+    ("update/downdate", lambda k: "k_updown" in k),
+]
+
+
+def counter_summary(dirs, calib_mib: int, title: str, cmds, out: pathlib.Path, cus: int = 256):
+    """Per kernel class, from rocprofv3 --pmc folders of the same command: dispatches,
+    device time (the dispatches' own timestamps), HBM bytes (FETCH_SIZE, WRITE_SIZE
+    scaled by the calibration copy) and GB/s, MFMA busy share
+    (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE * CUs * 4)) and MFMA flops
+    (512 * SQ_INSTS_VALU_MFMA_MOPS_*) and TFLOP/s."""
+    import csv
+    per = {}          # (dir, dispatch) -> {name, t, counters}
+    for d in dirs:
+        for f in sorted(pathlib.Path(d).rglob("*counter_collection.csv")):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    key = (d, int(r["Dispatch_Id"]))
+                    e = per.setdefault(key, {"name": r["Kernel_Name"], "t": 0.0, "c": {}})
+                    e["t"] = max(e["t"], (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9)
+                    e["c"][r["Counter_Name"]] = e["c"].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    scale = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = max((e["c"][ctr] for e in per.values() if ctr in e["c"] and e["name"].startswith(CALIB_KERNEL)),
+                  default=0.0)
+        if cal > 0:
+            scale[ctr] = calib_mib * 1024.0 / cal
+    agg = {}
+    for e in per.values():
+        if e["name"].startswith(CALIB_KERNEL):
+            continue
+        cls = next((lab for lab, f in COUNTER_CLASSES if f(e["name"])), None)
+        if cls is None:
+            continue
+        a = agg.setdefault(cls, {"n": {}, "t": {}, "c": {}})
+        for ctr, v in e["c"].items():
+            a["c"][ctr] = a["c"].get(ctr, 0.0) + v
+            a["n"][ctr] = a["n"].get(ctr, 0) + 1
+            a["t"][ctr] = a["t"].get(ctr, 0.0) + e["t"]
+    lines = [f"# Hardware counters per kernel class — {title}", "", "Counter passes, each a run of its own "
+             "(repository root, one MI355X):", "", "```"] + list(cmds) + ["```", "",
+             f"HBM bytes: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) times the factor that makes the calibration "
+             f"copy of {calib_mib} MiB in the same process read and write exactly {calib_mib} MiB "
+             f"(FETCH x{scale.get('FETCH_SIZE', float('nan')):.4f}, WRITE x{scale.get('WRITE_SIZE', float('nan')):.4f}). "
+             "GB/s = bytes / the dispatches' own duration in that pass (dispatches are serialized while counters "
+             "are read). MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 256 CUs x 4), summed over the "
+             "class's dispatches. MFMA flops = 512 x SQ_INSTS_VALU_MFMA_MOPS_<type>; peaks: FP64 78.6 TFLOP/s, "
+             "FP32 157.3, FP16 2,500 (dense).", "",
+             "| class | dispatches | fetch GB | write GB | HBM GB/s | MFMA busy | MFMA TFLOP (F64 / F32 / F16) | MFMA TFLOP/s |",
+             "|---|---|---|---|---|---|---|---|"]
+    for lab, _ in COUNTER_CLASSES:
+        if lab not in agg:
+            continue
+        a = agg[lab]
+        c = a["c"]
+        fetch = c["FETCH_SIZE"] * scale["FETCH_SIZE"] * 1024 / 1e9 if "FETCH_SIZE" in c and "FETCH_SIZE" in scale else None
+        write = c["WRITE_SIZE"] * scale["WRITE_SIZE"] * 1024 / 1e9 if "WRITE_SIZE" in c and "WRITE_SIZE" in scale else None
+        gbs = None
+        if fetch is not None and write is not None and a["t"].get("FETCH_SIZE") and a["t"].get("WRITE_SIZE"):
+            gbs = fetch / a["t"]["FETCH_SIZE"] + write / a["t"]["WRITE_SIZE"]
+        busy = (c["SQ_VALU_MFMA_BUSY_CYCLES"] / (c["GRBM_GUI_ACTIVE"] * cus * 4)
+                if c.get("GRBM_GUI_ACTIVE") and "SQ_VALU_MFMA_BUSY_CYCLES" in c else None)
+        fl = [c.get(f"SQ_INSTS_VALU_MFMA_MOPS_{t}") for t in ("F64", "F32", "F16")]
+        fl = [None if v is None else 512 * v / 1e12 for v in fl]
+        tmf = a["t"].get("SQ_VALU_MFMA_BUSY_CYCLES")
+        rate = sum(v for v in fl if v) / tmf if tmf and any(fl) else None
+        n = max(a["n"].values())
+        fmt = lambda v, f: "–" if v is None else f.format(v)
+        lines.append(f"| {lab} | {n} | {fmt(fetch, '{:.2f}')} | {fmt(write, '{:.2f}')} | {fmt(gbs, '{:.0f}')} | "
+                     f"{fmt(busy, '{:.1%}')} | {' / '.join(fmt(v, '{:.3f}') for v in fl)} | {fmt(rate, '{:.1f}')} |")
+    out.write_text("\n".join(lines) + "\n")
+    return agg
+
+
 def read_traffic():
     """Bytes per update launch from the committed counter summary, or None."""
     if not TRAFFIC_FILE.exists():
@@ -405,9 +488,18 @@ def main():
                     help="run one device copy of this size first (counter calibration)")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="no GPU: turn two rocprofv3 --pmc outputs into the traffic summary")
+    ap.add_argument("--counter-summary", nargs="+", metavar="PMC_DIR",
+                    help="no GPU: turn rocprofv3 --pmc outputs of one command into a per-class counter table")
+    ap.add_argument("--counter-title", default="")
+    ap.add_argument("--counter-cmd", action="append", default=[], help="a command line to quote (repeatable)")
+    ap.add_argument("--counter-out", default="")
     ap.add_argument("--traffic-cmd", default="python3 bench.py --steps 1 --warmup 1 --no-cpu "
                     "--only-64 --calib-copy-mib 1024")
     args = ap.parse_args()
+    if args.counter_summary:
+        counter_summary(args.counter_summary, args.calib_copy_mib or 1024, args.counter_title, args.counter_cmd,
+                        pathlib.Path(args.counter_out))
+        return
     if args.traffic_summary:
         r = traffic_summary(args.traffic_summary[0], args.traffic_summary[1],
                             args.calib_copy_mib or 1024, args.traffic_cmd)
