@@ -136,7 +136,7 @@ def traffic_summary(fetch_dir: str, write_dir: str, calib_mib: int, cmd: str) ->
     lines = ["# HBM traffic of the descendant-update kernel, 64^3 7-point diffusion, FP64", "",
              "Two counter passes over the same command (repository root, one MI355X):", "", "```"]
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
-        lines.append(f"timeout -s KILL 180 rocprofv3 --pmc {counter} --output-format csv -d {d} -o run -- {cmd}")
+        lines.append(f"timeout -s KILL 300 rocprofv3 --pmc {counter} --output-format csv -d {d} -o run -- {cmd}")
     lines += ["```", "",
               f"Calibration: one device-to-device copy of {calib_mib} MiB ({CALIB_KERNEL}) in the same "
               "process; scale = known KiB / counter KiB.", "",
@@ -165,12 +165,13 @@ COUNTER_CLASSES = [
 ]
 
 
-def counter_summary(dirs, calib_mib: int, title: str, cmds, out: pathlib.Path, cus: int = 256):
+def counter_summary(dirs, calib_mib: int, title: str, cmds, out: pathlib.Path, cus: int = 256, xcds: int = 8):
     """Per kernel class, from rocprofv3 --pmc folders of the same command: dispatches,
     device time (the dispatches' own timestamps), HBM bytes (FETCH_SIZE, WRITE_SIZE
-    scaled by the calibration copy) and GB/s, MFMA busy share
-    (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE * CUs * 4)) and MFMA flops
-    (512 * SQ_INSTS_VALU_MFMA_MOPS_*) and TFLOP/s."""
+    scaled by the calibration copy) and GB/s, MFMA busy share and MFMA flops
+    (512 * SQ_INSTS_VALU_MFMA_MOPS_*) and TFLOP/s.  GRBM_GUI_ACTIVE is summed over
+    the XCDs (it reads xcds * clock * duration), so the busy share is
+    SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / xcds * CUs * 4)."""
     import csv
     per = {}          # (dir, dispatch) -> {name, t, counters}
     for d in dirs:
@@ -205,8 +206,9 @@ def counter_summary(dirs, calib_mib: int, title: str, cmds, out: pathlib.Path, c
              f"copy of {calib_mib} MiB in the same process read and write exactly {calib_mib} MiB "
              f"(FETCH x{scale.get('FETCH_SIZE', float('nan')):.4f}, WRITE x{scale.get('WRITE_SIZE', float('nan')):.4f}). "
              "GB/s = bytes / the dispatches' own duration in that pass (dispatches are serialized while counters "
-             "are read). MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 256 CUs x 4), summed over the "
-             "class's dispatches. MFMA flops = 512 x SQ_INSTS_VALU_MFMA_MOPS_<type>; peaks: FP64 78.6 TFLOP/s, "
+             "are read). MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / 8 x 256 CUs x 4 SIMDs), summed "
+             "over the class's dispatches; GRBM_GUI_ACTIVE is summed over the 8 XCDs (it reads 16-19 GHz of "
+             "dispatch time here). MFMA flops = 512 x SQ_INSTS_VALU_MFMA_MOPS_<type>; peaks: FP64 78.6 TFLOP/s, "
              "FP32 157.3, FP16 2,500 (dense).", "",
              "| class | dispatches | fetch GB | write GB | HBM GB/s | MFMA busy | MFMA TFLOP (F64 / F32 / F16) | MFMA TFLOP/s |",
              "|---|---|---|---|---|---|---|---|"]
@@ -220,7 +222,7 @@ def counter_summary(dirs, calib_mib: int, title: str, cmds, out: pathlib.Path, c
         gbs = None
         if fetch is not None and write is not None and a["t"].get("FETCH_SIZE") and a["t"].get("WRITE_SIZE"):
             gbs = fetch / a["t"]["FETCH_SIZE"] + write / a["t"]["WRITE_SIZE"]
-        busy = (c["SQ_VALU_MFMA_BUSY_CYCLES"] / (c["GRBM_GUI_ACTIVE"] * cus * 4)
+        busy = (c["SQ_VALU_MFMA_BUSY_CYCLES"] / (c["GRBM_GUI_ACTIVE"] / xcds * cus * 4)
                 if c.get("GRBM_GUI_ACTIVE") and "SQ_VALU_MFMA_BUSY_CYCLES" in c else None)
         fl = [c.get(f"SQ_INSTS_VALU_MFMA_MOPS_{t}") for t in ("F64", "F32", "F16")]
         fl = [None if v is None else 512 * v / 1e12 for v in fl]
