@@ -595,13 +595,14 @@ static int flatten_upload(mxc_factor *F) {
                 emit(L_TRSM, big, false);
             }
             // At the end of panel P: wait for the second-stream update of panel
-            // P-1 (it also wrote the columns of panel P+1), then start panel P's
-            // own second-stream update (columns right of panel P+1; it needs only
-            // the finished panel P) and update panel P+1's columns here.  Panel
-            // P+1's steps then overlap the second-stream update.
+            // P-1 (it also wrote the columns of panel P+1), update panel P+1's
+            // columns here, then start panel P's own second-stream update (columns
+            // right of panel P+1; it needs only the finished panel P).  Panel P+1's
+            // steps then overlap the second-stream update.
             // (a join only after a fork: the second stream must already be part of
             // the work, also inside a graph capture)
             if (sb.panel_end && level_forked) F->factor_launches.push_back(Launch{L_JOIN, 0, 0, 0});
+            bool fork_t2 = false;
             if (!sb.trail2.empty()) {
                 // FP64: fold the update right of the next panel into the next panel's
                 // three trailing launches, one column slice each, where it runs beside
@@ -646,9 +647,7 @@ static int flatten_upload(mxc_factor *F) {
                         merged_s[t + 1 + k].insert(merged_s[t + 1 + k].end(), add_s[k].begin(), add_s[k].end());
                     }
                 } else {
-                    F->factor_launches.push_back(Launch{L_FORK, 0, 0, 0});
-                    emit(L_TRAIL2, sb.trail2, true);
-                    level_forked = true;
+                    fork_t2 = true;
                 }
             }
             // the next step's diagonal blocks, one per trailing block, and (FP64)
@@ -686,6 +685,16 @@ static int flatten_upload(mxc_factor *F) {
                     force.push_back(e);
                 }
                 emit(L_TRAIL, blocks, true, &follow, &fused, &force, &foldable, &folded);
+            }
+            if (fork_t2) {
+                // the second-stream update starts when this step's own trailing
+                // update has finished, together with the next diagonal block's
+                // factorization: that single workgroup then finds a CU before the
+                // second stream's tiles fill them all (started beside the trailing
+                // update, it waited for the whole second-stream launch to drain)
+                F->factor_launches.push_back(Launch{L_FORK, 0, 0, 0});
+                emit(L_TRAIL2, sb.trail2, true);
+                level_forked = true;
             }
             fused_s.clear();
             folded_s.clear();
