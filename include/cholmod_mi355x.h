@@ -332,6 +332,12 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * same factor; off by default because the waiting tiles hold their CUs
  * through the diagonal factorization, which costs more than the launch. */
 #define MXC_OPT_FOLD_TRSM 7
+/* MXC_OPT_T2_GRID (default 3, FP64 factors): the look-ahead's second-stream
+ * trailing update runs on a grid of this many workgroups per CU, each walking
+ * its share of the tiles (0: one workgroup per tile).  Every CU then keeps
+ * room for the main stream's diagonal factorizations, which otherwise wait
+ * for the whole second-stream launch to drain. */
+#define MXC_OPT_T2_GRID 8
 int mxc_set_option(mxc_factor *F, int option, int64_t value);
 
 /* As mxc_solve_refine for a block-diagonal batch of independent systems: rows

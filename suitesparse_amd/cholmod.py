@@ -228,6 +228,11 @@ class Factor:
         applies its TRSM (default off: measured slower); off: one TRSM launch per step."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 7, int(bool(on))))
 
+    def set_t2_grid(self, per_cu: int):
+        """FP64: workgroups per CU of the look-ahead's second-stream trailing update,
+        each walking its share of the tiles (default 3; 0: one workgroup per tile)."""
+        check("mxc_set_option", lib().mxc_set_option(self._num, 8, int(per_cu)))
+
     def set_graphs(self, on: bool):
         """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 1, int(bool(on))))

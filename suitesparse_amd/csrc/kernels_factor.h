@@ -326,14 +326,13 @@ __device__ __forceinline__ void fold_wait(const int *flag, int *abort) {
     __syncthreads();
 }
 
-template <typename T, int MODE, int TS, bool PD = false>
-__global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile(const UpdDesc *__restrict__ descs,
-                                                              const Tile *__restrict__ tiles,
-                                                              T *__restrict__ W,
-                                                              const int *__restrict__ Ls,
-                                                              unsigned *__restrict__ Lh,
-                                                              const double *__restrict__ adiag,
-                                                              PdArgs pda = PdArgs{}) {
+// One tile of a launch: virtual workgroup bx of g (k_tile: the workgroup
+// itself; k_tile_grid: each of its workgroups takes bx, bx + grid, ...).
+template <typename T, int MODE, int TS, bool PD>
+__device__ __forceinline__ void tile_work(const UpdDesc *__restrict__ descs, const Tile *__restrict__ tiles,
+                                          T *__restrict__ W, const int *__restrict__ Ls, unsigned *__restrict__ Lh,
+                                          const double *__restrict__ adiag, const PdArgs &pda, const int bx,
+                                          const int g) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     constexpr int NT = TS <= 32 ? 64 : 256;     // threads (tiles of 16 and 32: one wave)
     constexpr int WB = TS <= 32 ? TS : TS / 2;  // edge of a wave's block of C
@@ -362,8 +361,8 @@ __global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
     __shared__ int relr[TS];
     __shared__ int relc[TS];
 
-    const int bx = (int)blockIdx.x, nf = PD ? pda.nfirst : 0;
-    const Tile tl = tiles[bx < nf ? bx : nf + xcd_tile(bx - nf, (int)gridDim.x - nf)];
+    const int nf = PD ? pda.nfirst : 0;
+    const Tile tl = tiles[bx < nf ? bx : nf + xcd_tile(bx - nf, g - nf)];
     const UpdDesc d = descs[tl.desc];
     const int i0 = tl.ti * TS, j0 = tl.tj * TS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -533,6 +532,33 @@ __global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile
     }
     if constexpr (PD) {
         if (d.pd >= 0 && tl.ti == 0 && tl.tj == 0) pd_tail<T>(pda, d.pd, W, smem, adiag, d.fold != 0);
+    }
+}
+
+template <typename T, int MODE, int TS, bool PD = false>
+__global__ __launch_bounds__(TS <= 32 ? 64 : 256, TS == 128 ? 2 : 1) void k_tile(const UpdDesc *__restrict__ descs,
+                                                              const Tile *__restrict__ tiles,
+                                                              T *__restrict__ W,
+                                                              const int *__restrict__ Ls,
+                                                              unsigned *__restrict__ Lh,
+                                                              const double *__restrict__ adiag,
+                                                              PdArgs pda = PdArgs{}) {
+    tile_work<T, MODE, TS, PD>(descs, tiles, W, Ls, Lh, adiag, pda, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// The tiles of a launch on a grid of at most a few workgroups per CU (a
+// multiple of 8, so a workgroup's tiles keep its XCD): the second-stream
+// trailing update leaves room on every CU for the diagonal factorizations
+// of the main stream, which otherwise wait until the whole launch drains.
+template <typename T, int MODE, int TS>
+__global__ __launch_bounds__(256, 1) void k_tile_grid(const UpdDesc *__restrict__ descs,
+                                                      const Tile *__restrict__ tiles, T *__restrict__ W,
+                                                      const int *__restrict__ Ls, const double *__restrict__ adiag,
+                                                      int ntiles) {
+    static_assert(TS >= 64, "four-wave tiles");
+    for (int b = (int)blockIdx.x; b < ntiles; b += (int)gridDim.x) {
+        tile_work<T, MODE, TS, false>(descs, tiles, W, Ls, nullptr, adiag, PdArgs{}, b, ntiles);
+        __syncthreads();                   // the next tile restages the LDS buffers
     }
 }
 

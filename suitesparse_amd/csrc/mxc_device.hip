@@ -238,6 +238,7 @@ struct mxc_factor {
     int *d_ready = nullptr;              // per diagonal task: published by a folding tail; + abort flag
     int64_t nready = 0;                  // diagonal tasks covered by d_ready
     bool merge_t2 = true;                // FP64: fold second-stream updates into later trailing launches
+    int t2_per_cu = 3;                   // FP64 second-stream trailing launches: workgroups per CU (0: one per tile)
     bool deterministic = false;          // descendant updates in rounds: one pair per target per launch
     size_t bytes = 0;
     mxc_pool *pool = nullptr;            // device memory comes from this pool when set
@@ -1545,9 +1546,32 @@ static inline Redo redo_args(const mxc_factor *F) {
                 (const double *)F->d_beta};
 }
 
+// CUs of the current device (queried once)
+static int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, g_device) != hipSuccess || v <= 0) v = 256;
+        cus = v;
+    }
+    return cus;
+}
+
 template <typename T, int MODE, bool PD = false>
 static void launch_tile(mxc_factor *F, const Launch &L, T *W, hipStream_t st = nullptr) {
     if (!st) st = F->stream;
+    if constexpr (std::is_same<T, double>::value && MODE == MODE_TRAIL && !PD) {
+        // FP64 second-stream update: a grid of t2_per_cu workgroups per CU that
+        // walk all its tiles, so that every CU keeps room for a diagonal
+        // factorization of the main stream (k_potrf_diag: 144 registers per
+        // lane and 66.5 KB of LDS beside three 96-register, 17 KB tiles)
+        const long long cap = ((long long)F->t2_per_cu * device_cus()) / 8 * 8;
+        if (L.kind == L_TRAIL2 && L.tile == 64 && cap > 0 && L.count > cap) {
+            hipLaunchKernelGGL((k_tile_grid<double, MODE_TRAIL, 64>), dim3((unsigned)cap), dim3(256), 0, st,
+                               F->d_desc, F->d_tiles + L.begin, W, F->d_Ls, F->d_adiag, (int)L.count);
+            return;
+        }
+    }
     const bool h16 = std::is_same<T, float>::value && F->split16 && F->d_Lh;
     const double *adiag = F->d_adiag;
     const PdArgs pda{F->d_diag, F->d_fail, redo_safe<T>(F) ? F->rho : 0.0, redo_args(F), F->d_ready,
@@ -2632,6 +2656,11 @@ extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
             if (F->deterministic == (value != 0)) return MXC_OK;
             F->deterministic = value != 0;
             return flatten_upload(F);           // new launch list (drops the factor graph)
+        case MXC_OPT_T2_GRID:
+            if (value < 0 || value > 64) return MXC_INVALID;
+            F->t2_per_cu = (int)value;
+            drop_factor_graph(F);
+            return MXC_OK;
         case MXC_OPT_FOLD_TRSM:
             if (F->fold_trsm == (value != 0)) return MXC_OK;
             F->fold_trsm = value != 0;
