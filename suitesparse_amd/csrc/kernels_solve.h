@@ -577,6 +577,94 @@ __global__ __launch_bounds__(256) void k_sweep_tiny_bwd(const SweepTask *__restr
     }
 }
 
+// Quarter-wave supernodes (more than tiny, at most QUAD_COLS columns and
+// QUAD_ROWS rows below): sixteen lanes solve one supernode, four per wave and
+// sixteen per workgroup, where a wave of its own would leave 48 lanes idle.
+// Lane q owns row q of the diagonal block and row nb + q below it; the
+// inverse's row (forward) or column (backward) and the L entries of its row
+// below are loaded once for every right-hand side.  Groups share no data
+// (each has its own LDS slice); synchronization is per wave.
+constexpr int QUAD_COLS = 16;
+constexpr int QUAD_ROWS = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_quad_fwd(const SweepTask *__restrict__ tasks, int ntask,
+                                                        const T *__restrict__ W,
+                                                        const int *__restrict__ Ls, T *__restrict__ X,
+                                                        long long n, int nrhs) {
+    __shared__ T ys[16][QUAD_COLS];
+    const int q = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int ti = blockIdx.x * 16 + grp;
+    const bool on = ti < ntask;
+    SweepTask t = {};
+    if (on) t = tasks[ti];
+    const int nb = on ? t.nb : 0, below = on ? t.nsrow - t.nb : 0;
+    T iv[QUAD_COLS], lr[QUAD_COLS];
+#pragma unroll
+    for (int c = 0; c < QUAD_COLS; c++) {
+        iv[c] = (q < nb && c <= q) ? W[t.winv + q + (long long)c * nb] : (T)0;
+        lr[c] = (q < below && c < nb) ? W[t.lx + nb + q + (long long)c * t.nsrow] : (T)0;
+    }
+    const long long g = q < below ? (long long)Ls[t.psi + nb + q] : 0;
+    for (int z = 0; z < nrhs; z++) {
+        T *xz = X + (long long)z * n;
+        ys[grp][q] = q < nb ? xz[t.k1 + q] : (T)0;
+        wave_sync_lds();
+        T y = 0;
+#pragma unroll
+        for (int c = 0; c < QUAD_COLS; c++) y += iv[c] * ys[grp][c];
+        wave_sync_lds();
+        if (q < nb) xz[t.k1 + q] = y;
+        ys[grp][q] = y;
+        wave_sync_lds();
+        if (q < below) {
+            T a = 0;
+#pragma unroll
+            for (int c = 0; c < QUAD_COLS; c++) a += lr[c] * ys[grp][c];
+            atomic_add(xz + g, -a);
+        }
+        wave_sync_lds();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_quad_bwd(const SweepTask *__restrict__ tasks, int ntask,
+                                                        const T *__restrict__ W,
+                                                        const int *__restrict__ Ls, T *__restrict__ X,
+                                                        long long n, int nrhs) {
+    __shared__ T vs[16][QUAD_COLS];
+    const int q = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int ti = blockIdx.x * 16 + grp;
+    const bool on = ti < ntask;
+    SweepTask t = {};
+    if (on) t = tasks[ti];
+    const int nb = on ? t.nb : 0, below = on ? t.nsrow - t.nb : 0;
+    // lane q as column q: the inverse's column q and L's column q below the block
+    T iv[QUAD_COLS], lc[QUAD_ROWS];
+#pragma unroll
+    for (int i = 0; i < QUAD_COLS; i++) iv[i] = (q < nb && i >= q && i < nb) ? W[t.winv + i + (long long)q * nb] : (T)0;
+#pragma unroll
+    for (int r = 0; r < QUAD_ROWS; r++) lc[r] = (q < nb && r < below) ? W[t.lx + nb + r + (long long)q * t.nsrow] : (T)0;
+    const long long g = q < below ? (long long)Ls[t.psi + nb + q] : 0;
+    for (int z = 0; z < nrhs; z++) {
+        T *xz = X + (long long)z * n;
+        vs[grp][q] = q < below ? xz[g] : (T)0;       // ancestor rows: final before the launch
+        wave_sync_lds();
+        T p = 0;
+#pragma unroll
+        for (int r = 0; r < QUAD_ROWS; r++) p += lc[r] * vs[grp][r];
+        const T v = q < nb ? xz[t.k1 + q] - p : (T)0;
+        wave_sync_lds();
+        vs[grp][q] = v;
+        wave_sync_lds();
+        T y = 0;
+#pragma unroll
+        for (int i = 0; i < QUAD_COLS; i++) y += iv[i] * vs[grp][i];
+        if (q < nb) xz[t.k1 + q] = y;
+        wave_sync_lds();
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_sweep_small_bwd(const SweepTask *__restrict__ tasks, int ntask,
                                                          const T *__restrict__ W,

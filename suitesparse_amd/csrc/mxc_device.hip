@@ -149,12 +149,14 @@ static inline int kind_of(int launch_kind) {
 // the second stream; L_FORK / L_JOIN: the event edges between the two streams;
 // L_FCHAIN / L_BCHAIN: a level's chained sweep launch; L_TRAILD: a trailing
 // launch whose (0, 0) tiles also factor the next step's diagonal blocks;
-// L_FTINY / L_BTINY: a level's tiny supernodes, one lane each; L_DIAG_TINY:
+// L_FTINY / L_BTINY: a level's tiny supernodes, one lane each; L_FQUAD /
+// L_BQUAD: its quarter-wave supernodes, sixteen lanes each; L_DIAG_TINY:
 // diagonal blocks of at most TINY_DIAG columns, one lane each; L_TRSM_TINY:
 // the panel TRSM of tiny supernodes, one lane each.
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7,
        L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11, L_FCHAIN = 12, L_BCHAIN = 13,
-       L_TRAILD = 14, L_FTINY = 15, L_BTINY = 16, L_DIAG_TINY = 17, L_TRSM_TINY = 18 };
+       L_TRAILD = 14, L_FTINY = 15, L_BTINY = 16, L_DIAG_TINY = 17, L_TRSM_TINY = 18, L_FQUAD = 19,
+       L_BQUAD = 20 };
 
 // ---------------------------------------------------------------------------
 // device memory pool: one hipMalloc, sub-allocated first-fit with coalescing
@@ -1041,14 +1043,20 @@ static int build_plan(mxc_factor *F, const mxc_symbolic *S, const int64_t *Ap, c
     auto is_tiny = [&](int64_t s) {
         return S->Super[s + 1] - S->Super[s] <= TINY_COLS && S->Lpi[s + 1] - S->Lpi[s] <= TINY_ROWS;
     };
+    // quarter-wave supernodes (sixteen lanes each), between the two
+    auto size_class = [&](int64_t s) {
+        if (is_tiny(s)) return 0;
+        const int64_t nscol = S->Super[s + 1] - S->Super[s], nsrow = S->Lpi[s + 1] - S->Lpi[s];
+        return (nscol <= QUAD_COLS && nsrow - nscol <= QUAD_ROWS) ? 1 : 2;
+    };
     auto push_small = [&](int l, std::vector<Launch> &out, int kind) {
-        const int tkind = kind == L_FSMALL ? L_FTINY : L_BTINY;
-        for (int tiny = 1; tiny >= 0; tiny--) {
+        const bool f = kind == L_FSMALL;
+        const int kinds[3] = {f ? L_FTINY : L_BTINY, f ? L_FQUAD : L_BQUAD, kind};
+        for (int c = 0; c < 3; c++) {
             long long b = (long long)st.size();
             for (int64_t s : by_level[l])
-                if (is_small(s) && is_tiny(s) == (tiny == 1)) st.push_back(make_task(s, 0));
-            if ((long long)st.size() > b)
-                out.push_back(Launch{tiny ? tkind : kind, b, (long long)st.size() - b, 0});
+                if (is_small(s) && size_class(s) == c) st.push_back(make_task(s, 0));
+            if ((long long)st.size() > b) out.push_back(Launch{kinds[c], b, (long long)st.size() - b, 0});
         }
     };
     for (int l = 0; l < nlev; l++) {
@@ -1976,11 +1984,22 @@ static void launch_chain(mxc_factor *F, const Launch &L, T *Xw, int nr, T *part)
     }
 }
 
-// A level's tiny supernodes, one lane each.
+// A level's tiny supernodes, one lane each, or its quarter-wave ones, sixteen
+// lanes each.
 template <typename T>
 static void launch_tiny(mxc_factor *F, const Launch &L, T *Xw, int nr) {
     const dim3 g((unsigned)((L.count + 255) / 256)), b(256);
     const T *W = (const T *)F->W;
+    if (L.kind == L_FQUAD || L.kind == L_BQUAD) {
+        const dim3 gq((unsigned)((L.count + 15) / 16));
+        if (L.kind == L_FQUAD)
+            hipLaunchKernelGGL(k_sweep_quad_fwd<T>, gq, b, 0, F->stream, F->d_stask + L.begin, (int)L.count, W,
+                               F->d_Ls, Xw, (long long)F->n, nr);
+        else
+            hipLaunchKernelGGL(k_sweep_quad_bwd<T>, gq, b, 0, F->stream, F->d_stask + L.begin, (int)L.count, W,
+                               F->d_Ls, Xw, (long long)F->n, nr);
+        return;
+    }
     if (L.kind == L_FTINY)
         hipLaunchKernelGGL(k_sweep_tiny_fwd<T>, g, b, 0, F->stream, F->d_stask + L.begin, (int)L.count, W, F->d_Ls,
                            Xw, (long long)F->n, nr);
@@ -2028,7 +2047,7 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd,
                     hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(ncnt, 256, 1LL << 30)), dim3(256),
                                        0, sm, F->d_cnt, ncnt);
                 for (const Launch &L : dir == 0 ? F->fwd_chain : F->bwd_chain) {
-                    if (L.kind == L_FTINY || L.kind == L_BTINY)
+                    if (L.kind == L_FTINY || L.kind == L_BTINY || L.kind == L_FQUAD || L.kind == L_BQUAD)
                         launch_tiny<T>(F, L, Xz, zn);
                     else if (L.kind == L_FSMALL)
                         hipLaunchKernelGGL(k_sweep_small_fwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256),
@@ -2053,7 +2072,7 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd,
     if (fwd) {
         tick(F, 5, &tt);
         for (const Launch &L : F->fwd_launches) {
-            if (L.kind == L_FTINY)
+            if (L.kind == L_FTINY || L.kind == L_FQUAD)
                 launch_tiny<T>(F, L, Xw, nr);
             else if (L.kind == L_FSMALL)
                 hipLaunchKernelGGL(k_sweep_small_fwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,
@@ -2068,7 +2087,7 @@ static int enqueue_sweep(mxc_factor *F, T *Xw, int64_t nrhs, bool fwd, bool bwd,
     if (bwd) {
         tick(F, 6, &tt);
         for (const Launch &L : F->bwd_launches) {
-            if (L.kind == L_BTINY)
+            if (L.kind == L_BTINY || L.kind == L_BQUAD)
                 launch_tiny<T>(F, L, Xw, nr);
             else if (L.kind == L_BSMALL)
                 hipLaunchKernelGGL(k_sweep_small_bwd<T>, dim3((unsigned)((L.count + 3) / 4)), dim3(256), 0,
