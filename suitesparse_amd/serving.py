@@ -388,7 +388,7 @@ def run_stream(server: Server, reqs, arrivals, bank: PatternBank | None = None,
     """Feed requests at their arrival times; return latency / throughput stats.
     Requests built by request_specs are materialized ahead of their arrival
     by a pool of producer threads, in arrival order, at most `lookahead` ahead.
-    progress > 0 prints a line each time that many more requests were submitted."""
+    progress > 0 prints a line (to stderr) each time that many more requests were submitted."""
     import concurrent.futures
     import faulthandler
     import queue
@@ -425,7 +425,7 @@ def run_stream(server: Server, reqs, arrivals, bank: PatternBank | None = None,
         server.submit(req)
         if progress and (req.rid + 1) % progress == 0:
             print(f"[serving] {req.rid + 1} submitted, {len(server.completed)} done at t = "
-                  f"{server.now():.1f} s", flush=True)
+                  f"{server.now():.1f} s", file=sys.stderr, flush=True)
     prod.join()
     server.drain()
     done = sorted(server.completed, key=lambda r: r.rid)
