@@ -802,6 +802,18 @@ def batch_section(args, out):
                      "systems_per_s": round(len(mats) / tt, 1),
                      "factor_gflops": round(L.fl / tf / 1e9, 1),
                      "max_relres_per_system": rel, "refine": info}
+        if prec == "fp64":
+            # the same factorization with each level's small supernodes in the
+            # level's update, POTRF and TRSM launches instead of the fused one
+            L.set_small_fused(False)
+            cholmod.factorize(b.A, L, cmp, values=Ax)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                cholmod.factorize(b.A, L, cmp, values=Ax)
+            torch.cuda.synchronize()
+            res[prec]["t_factor_ms_small_unfused"] = round((time.perf_counter() - t1) / steps * 1e3, 2)
+            L.set_small_fused(True)
         log(f"[bench]   batch {prec}: {res[prec]}")
         del X, Xh, R
         gc.collect()

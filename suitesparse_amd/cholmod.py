@@ -233,6 +233,11 @@ class Factor:
         each walking its share of the tiles (default 3; 0: one workgroup per tile)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 8, int(per_cu)))
 
+    def set_small_fused(self, on: bool):
+        """FP64: factor each level's supernodes of at most 16 columns and 64 rows
+        whole, one wave each, in one launch per level (default on)."""
+        check("mxc_set_option", lib().mxc_set_option(self._num, 9, 1 if on else 0))
+
     def set_graphs(self, on: bool):
         """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 1, int(bool(on))))

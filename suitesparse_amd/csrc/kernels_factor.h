@@ -776,6 +776,13 @@ __device__ __forceinline__ float rsqrt_nr(float x) {
     return fmaf(0.5f * y, e, y);
 }
 
+// LDS stores of one lane visible to the other lanes of its wave
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // value of v in lane k (k wave-uniform), through a scalar register
 __device__ __forceinline__ double lane_bcast(double v, int k) {
     int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
@@ -1219,6 +1226,159 @@ __global__ __launch_bounds__(256) void k_trsm_tiny(const UpdDesc *__restrict__ d
             W[q] = c;
             if (std::is_same<T, float>::value && Lh) Lh[q] = split_pack((float)c, ex);
         }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Small supernodes of one etree level, fused: a supernode of at most
+// SMALL_COLS columns and SMALL_NROW rows has one 64-column step, so its whole
+// numeric factorization (t_cholmod_super_numeric_worker.c:537-1190: the
+// descendant updates by dsyrk/dgemm scattered through the relative map, then
+// dpotrf of the diagonal block and dtrsm of the rows below) is done by one
+// wave, the supernode held in LDS from its assembly to its last store.  Four
+// waves per workgroup walk the level's tasks on a grid sized to the device
+// (a persistent grid: task w + 4 b, then + 4 gridDim.x, ...).  This replaces
+// the level's descendant-update launch (atomic scatter into W), its POTRF
+// launch and its TRSM launch for these supernodes by one launch with no
+// atomics, and each descendant's contributions land in descendant order.
+//
+// Per descendant pair (the UpdDesc of the update launch, MODE_SCATTER
+// semantics): C = Y X' by 16x16x4 FP64 MFMAs straight from W (row blocks of
+// 16 rows of Y against the N <= 16 rows of X), then S[rel(i)][rel(j)] -= C(i, j)
+// for i >= j, rel the block rows' positions among the supernode's rows.
+// Factorization: right-looking over the nscol columns with lane r owning row
+// r of the panel, potrf16_wave's pivot arithmetic (r_j = 1/sqrt(d_jj),
+// L_jj = d_jj r_j, L_rj = a_rj r_j, a non-positive pivot reported and replaced
+// by 1) applied to the rows below the diagonal block as well, which makes the
+// TRSM part of the same pass.  The inverse of the diagonal block is formed as
+// potrf16_wave forms it, column c by lane c.
+// ---------------------------------------------------------------------------
+constexpr int SMALL_COLS = 16;
+constexpr int SMALL_NROW = 64;
+
+__global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict__ tasks, int ntask,
+                                                     const UpdDesc *__restrict__ descs, double *__restrict__ W,
+                                                     const int *__restrict__ Ls, int *__restrict__ fail) {
+    constexpr int SD = SMALL_COLS + 1;
+    __shared__ double S[4][SMALL_NROW * SD];
+    __shared__ int rows[4][SMALL_NROW];
+    __shared__ int rel[4][SMALL_NROW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double *Sw = S[w];
+    int *rw = rows[w], *rl = rel[w];
+    for (int ti = blockIdx.x * 4 + w; ti < ntask; ti += gridDim.x * 4) {
+        const SmallTask t = tasks[ti];
+        const int nsrow = t.nsrow, nscol = t.nscol;
+        double a[SMALL_COLS];
+#pragma unroll
+        for (int c = 0; c < SMALL_COLS; c++)
+            a[c] = (lane < nsrow && c < nscol) ? W[t.lx + lane + (long long)c * nsrow] : 0.0;
+        rw[lane] = lane < nsrow ? Ls[t.psi + lane] : INT_MAX;
+#pragma unroll
+        for (int c = 0; c < SMALL_COLS; c++) Sw[lane * SD + c] = a[c];
+        wave_sync_lds();
+
+        const int jl = lane & 15, kq = lane >> 4;
+        for (int p = t.u0; p < t.u0 + t.nu; p++) {
+            const UpdDesc u = descs[p];
+            if (lane < u.M) {
+                const int g = Ls[u.ls_off + lane];
+                int r;
+                if (g < t.k1 + nscol) {
+                    r = g - t.k1;
+                } else {
+                    int lo = nscol, hi = nsrow - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (rw[mid] < g) lo = mid + 1; else hi = mid;
+                    }
+                    r = lo;
+                }
+                rl[lane] = r;
+            }
+            const int nblk = (u.M + 15) >> 4;
+            f64x4 acc[4];
+#pragma unroll
+            for (int n = 0; n < 4; n++) acc[n] = f64x4{0, 0, 0, 0};
+            const double *xp = W + u.x_off + jl, *yp = W + u.y_off + jl;
+            for (int k0 = 0; k0 < u.K; k0 += 4) {
+                const int k = k0 + kq;
+                const bool kin = k < u.K;
+                const double xa = (kin && jl < u.N) ? xp[(long long)k * u.ldx] : 0.0;
+                double yb[4];
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    yb[n] = (kin && 16 * n + jl < u.M) ? yp[16 * n + (long long)k * u.ldy] : 0.0;
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    if (n < nblk) acc[n] = MfmaOp<double>::mma(xa, yb[n], acc[n]);
+            }
+            wave_sync_lds();                 // rel[] of every lane
+            // acc[n][v] = C(i, j), i = 16 n + lane % 16, j = row(lane, v)
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const int i = 16 * n + jl;
+                if (n >= nblk || i >= u.M) continue;
+                const int ri = rl[i];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int j = MfmaOp<double>::row(lane, v);
+                    if (j < u.N && j <= i) Sw[ri * SD + rl[j]] -= acc[n][v];
+                }
+            }
+            wave_sync_lds();                 // the block's stores, before the next pair's map
+        }
+
+        // the panel, right-looking, lane r owning row r
+#pragma unroll
+        for (int c = 0; c < SMALL_COLS; c++) a[c] = c < nscol ? Sw[lane * SD + c] : 0.0;
+        double rinv[SMALL_COLS];
+        int badj = SMALL_COLS;
+#pragma unroll
+        for (int j = 0; j < SMALL_COLS; j++) {
+            rinv[j] = 0.0;
+            if (j >= nscol) continue;
+            double djj = lane_bcast(a[j], j);
+            const bool bad = !(djj > 0.0);
+            badj = (bad && badj == SMALL_COLS) ? j : badj;
+            djj = bad ? 1.0 : djj;
+            const double rj = rsqrt_nr(djj);
+            rinv[j] = rj;
+            const double l = (lane > j) ? a[j] * rj : ((lane == j) ? djj * rj : 0.0);
+            a[j] = l;
+#pragma unroll
+            for (int c = j + 1; c < SMALL_COLS; c++)
+                if (c < nscol) a[c] -= l * lane_bcast(l, c);
+        }
+        if (lane == 0 && badj < nscol) atomicMin(fail, t.k1 + badj);
+        if (lane < nsrow) {
+#pragma unroll
+            for (int c = 0; c < SMALL_COLS; c++)
+                if (c < nscol && c <= lane) W[t.lx + lane + (long long)c * nsrow] = a[c];
+        }
+        // the inverse of the diagonal block: lane c forms column c
+        if (lane < nscol) {
+#pragma unroll
+            for (int c = 0; c < SMALL_COLS; c++) Sw[lane * SD + c] = a[c];
+        }
+        wave_sync_lds();
+        double x[SMALL_COLS], sm[SMALL_COLS];
+#pragma unroll
+        for (int i = 0; i < SMALL_COLS; i++) sm[i] = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < SMALL_COLS; k++) {
+            x[k] = sm[k] * rinv[k];
+#pragma unroll
+            for (int i = k + 1; i < SMALL_COLS; i++)
+                if (i < nscol) sm[i] -= Sw[i * SD + k] * x[k];
+        }
+        if (lane < nscol) {
+#pragma unroll
+            for (int i = 0; i < SMALL_COLS; i++)
+                if (i < nscol) W[t.winv + i + (long long)lane * nscol] = (i >= lane) ? x[i] : 0.0;
+        }
+        wave_sync_lds();                     // this task's LDS reads, before the next task's stores
     }
 }
 

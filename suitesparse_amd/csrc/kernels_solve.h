@@ -425,12 +425,6 @@ __global__ __launch_bounds__(256) void k_sweep_bwd(const SweepTask *__restrict__
 constexpr int SMALL_ROWS = 128;
 constexpr int ZS = 4;          // right-hand sides per pass
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_sweep_small_fwd(const SweepTask *__restrict__ tasks, int ntask,
                                                          const T *__restrict__ W,

@@ -152,11 +152,12 @@ static inline int kind_of(int launch_kind) {
 // L_FTINY / L_BTINY: a level's tiny supernodes, one lane each; L_FQUAD /
 // L_BQUAD: its quarter-wave supernodes, sixteen lanes each; L_DIAG_TINY:
 // diagonal blocks of at most TINY_DIAG columns, one lane each; L_TRSM_TINY:
-// the panel TRSM of tiny supernodes, one lane each.
+// the panel TRSM of tiny supernodes, one lane each; L_SMALL: a level's small
+// supernodes (FP64), each factored whole by one wave (k_small_level).
 enum { L_UPD = 0, L_DIAG = 1, L_TRSM = 2, L_TRAIL = 3, L_FSTEP = 4, L_BSTEP = 5, L_FSMALL = 6, L_BSMALL = 7,
        L_DIAG_SMALL = 8, L_TRAIL2 = 9, L_FORK = 10, L_JOIN = 11, L_FCHAIN = 12, L_BCHAIN = 13,
        L_TRAILD = 14, L_FTINY = 15, L_BTINY = 16, L_DIAG_TINY = 17, L_TRSM_TINY = 18, L_FQUAD = 19,
-       L_BQUAD = 20 };
+       L_BQUAD = 20, L_SMALL = 21 };
 
 // ---------------------------------------------------------------------------
 // device memory pool: one hipMalloc, sub-allocated first-fit with coalescing
@@ -241,6 +242,8 @@ struct mxc_factor {
     int64_t nready = 0;                  // diagonal tasks covered by d_ready
     bool merge_t2 = true;                // FP64: fold second-stream updates into later trailing launches
     int t2_per_cu = 3;                   // FP64 second-stream trailing launches: workgroups per CU (0: one per tile)
+    bool small_fused = true;             // FP64: small supernodes of a level in one launch (k_small_level)
+    int64_t nsmall = 0;                  // supernodes the plan factors that way
     bool deterministic = false;          // descendant updates in rounds: one pair per target per launch
     size_t bytes = 0;
     mxc_pool *pool = nullptr;            // device memory comes from this pool when set
@@ -265,6 +268,7 @@ struct mxc_factor {
     UpdDesc *d_desc = nullptr;
     Tile *d_tiles = nullptr;
     DiagTask *d_diag = nullptr;
+    SmallTask *d_small = nullptr;
     SweepTask *d_stask = nullptr;
     int *d_fail = nullptr;            // [0] first failing column, [1..3] FP64 redo queue counters
                                       // (queued, refactored in FP64, finished redo workgroups)
@@ -304,7 +308,7 @@ struct mxc_factor {
     // host copy of the factorization's blocks (re-flattened after autotuning)
     std::vector<std::vector<UpdDesc>> blk_upd;        // per level
     std::vector<std::vector<StepBlocks>> blk_steps;   // per level, per 64-column step
-    size_t plan_bytes = 0;                            // bytes of d_desc + d_tiles + d_diag
+    size_t plan_bytes = 0;                            // bytes of d_desc + d_tiles + d_diag + d_small
 
     // tile edge per (M, N) bucket: 32 up to 64 x 32, 128 for M > 128 and N > 32,
     // else 64 (mxc_set_tile overrides an entry before the plan is built)
@@ -451,8 +455,22 @@ static int flatten_upload(mxc_factor *F) {
     std::vector<UpdDesc> descs;
     std::vector<Tile> tiles;
     std::vector<DiagTask> diags;
+    std::vector<SmallTask> smalls;
     F->factor_launches.clear();
     int64_t nupd = 0;
+    // FP64: supernodes of at most SMALL_COLS columns and SMALL_NROW rows are
+    // factored whole by k_small_level, one launch at the head of their level
+    // (their update pairs, diagonal tasks and TRSM blocks leave the level's
+    // other launches).  task_of[s]: the level's task of supernode s, or -1.
+    const bool fuse = F->small_fused && F->precision == MXC_FP64;
+    std::vector<int> task_of(fuse ? F->nsuper : 0, -1);
+    auto small_s = [&](int64_t s) {
+        return F->Super[s + 1] - F->Super[s] <= SMALL_COLS && F->Lpi[s + 1] - F->Lpi[s] <= SMALL_NROW;
+    };
+    auto super_of = [&](long long base) {
+        return (int64_t)(std::upper_bound(F->Super.begin(), F->Super.end(), (int64_t)base) - F->Super.begin()) - 1;
+    };
+    F->nsmall = 0;
     // follow[i] (trailing blocks): the next step's diagonal task of block i's
     // supernode, or null.  A block tiled at 64 or 128 gets it as its tail (pd);
     // fused[i] records that, so the next step's diagonal launch skips it.
@@ -535,8 +553,51 @@ static int flatten_upload(mxc_factor *F) {
     };
     for (size_t l = 0; l < F->blk_upd.size(); l++) {
         bool level_forked = false;
+        const std::vector<UpdDesc> *upd = &F->blk_upd[l];
+        std::vector<UpdDesc> rest;                 // update pairs of the other targets
+        std::vector<int64_t> fused_small;          // the level's small supernodes (ascending)
+        if (fuse && !F->blk_steps[l].empty()) {
+            const StepBlocks &s0 = F->blk_steps[l][0];
+            const long long t0 = (long long)smalls.size();
+            for (size_t q = 0; q < s0.diags.size(); q++) {
+                const int64_t s = s0.diag_s[q];
+                if (!small_s(s)) continue;
+                task_of[s] = (int)(smalls.size() - t0);
+                smalls.push_back(SmallTask{(long long)F->Lpx[s], s0.diags[q].winv,
+                                           (int)(F->Lpi[s + 1] - F->Lpi[s]), (int)(F->Super[s + 1] - F->Super[s]),
+                                           (int)F->Super[s], (int)F->Lpi[s], 0, 0});
+                fused_small.push_back(s);
+            }
+            const long long nt = (long long)smalls.size() - t0;
+            if (nt > 0) {
+                std::vector<std::vector<UpdDesc>> pairs(nt);
+                for (const UpdDesc &u : F->blk_upd[l]) {
+                    const int64_t s = super_of(u.base);
+                    if (s >= 0 && task_of[s] >= 0) pairs[task_of[s]].push_back(u);
+                    else rest.push_back(u);
+                }
+                for (long long i = 0; i < nt; i++) {
+                    SmallTask &st = smalls[t0 + i];
+                    st.u0 = (int)descs.size();
+                    st.nu = (int)pairs[i].size();
+                    for (const UpdDesc &u : pairs[i]) {
+                        descs.push_back(u);
+                        descs.back().pd = -1;
+                        descs.back().fold = 0;
+                    }
+                }
+                for (int64_t s : fused_small) task_of[s] = -1;
+                std::sort(fused_small.begin(), fused_small.end());
+                F->factor_launches.push_back(Launch{L_SMALL, t0, nt, 0});
+                F->nsmall += nt;
+                upd = &rest;
+            }
+        }
+        auto fused_here = [&](size_t t, int64_t s) {
+            return t == 0 && std::binary_search(fused_small.begin(), fused_small.end(), s);
+        };
         if (!F->deterministic) {
-            nupd += emit(L_UPD, F->blk_upd[l], true);
+            nupd += emit(L_UPD, *upd, true);
         } else {
             // Rounds: round r holds the r-th pair (by descendant, ascending) of every
             // target supernode of the level, so no two tiles of a launch add into the
@@ -545,7 +606,7 @@ static int flatten_upload(mxc_factor *F) {
             // map sends distinct C entries to distinct entries of the target.)
             std::map<long long, int> rank;          // target (its L offset) -> pairs seen so far
             std::vector<std::vector<UpdDesc>> rounds;
-            for (const UpdDesc &u : F->blk_upd[l]) {
+            for (const UpdDesc &u : *upd) {
                 const int r = rank[u.c_off]++;
                 if ((int)rounds.size() <= r) rounds.resize(r + 1);
                 rounds[r].push_back(u);
@@ -570,6 +631,7 @@ static int flatten_upload(mxc_factor *F) {
                 for (size_t q = 0; q < sb.diags.size(); q++) {
                     const DiagTask &d = sb.diags[q];
                     if (std::binary_search(fused_s.begin(), fused_s.end(), sb.diag_s[q])) continue;
+                    if (fused_here(t, sb.diag_s[q])) continue;
                     const int c = d.nb <= TINY_DIAG ? 2 : d.nb <= 16 ? 1 : 0;
                     if (c == cls) diags.push_back(d);
                 }
@@ -585,6 +647,7 @@ static int flatten_upload(mxc_factor *F) {
                 for (size_t q = 0; q < sb.trsm.size(); q++) {
                     const UpdDesc &u = sb.trsm[q];
                     if (std::binary_search(folded_s.begin(), folded_s.end(), sb.trsm_s[q])) continue;
+                    if (fused_here(t, sb.trsm_s[q])) continue;
                     if (u.K <= TINY_DIAG && u.M <= TINY_TRSM_ROWS) {
                         descs.push_back(u);
                         descs.back().pd = -1;
@@ -716,15 +779,18 @@ static int flatten_upload(mxc_factor *F) {
     ffree(F, F->d_desc);
     ffree(F, F->d_tiles);
     ffree(F, F->d_diag);
+    ffree(F, F->d_small);
     F->d_desc = nullptr;
     F->d_tiles = nullptr;
     F->d_diag = nullptr;
+    F->d_small = nullptr;
     F->bytes -= F->plan_bytes;
     size_t before = F->bytes;
     int rc;
     if ((rc = dupload(F, &F->d_desc, descs, F->stream))) return rc;
     if ((rc = dupload(F, &F->d_tiles, tiles, F->stream))) return rc;
     if ((rc = dupload(F, &F->d_diag, diags, F->stream))) return rc;
+    if (!smalls.empty() && (rc = dupload(F, &F->d_small, smalls, F->stream))) return rc;
     // publication flags of the diagonal tasks (folded TRSMs) + the wait-abort flag
     ffree(F, F->d_ready);
     F->d_ready = nullptr;
@@ -1296,7 +1362,7 @@ extern "C" void mxc_factor_free(mxc_factor *F) {
                     F->d_diag, F->d_stask, F->d_fail, F->d_adiag, F->d_rowptr, F->d_src, F->d_col,
                     F->d_vals, F->d_Ax, F->d_norm, F->d_beta, F->d_work, F->d_part, F->d_b, F->d_x, F->d_r,
                     F->d_cnt, F->d_abort, F->d_seg, F->d_segout, F->d_pinv, F->d_redo, F->d_ready,
-                    F->d_mw, F->d_mpar, F->d_malpha, F->d_mflag, F->d_mdesc};
+                    F->d_mw, F->d_mpar, F->d_malpha, F->d_mflag, F->d_mdesc, F->d_small};
     for (void *p : ptrs) ffree(F, p);
 
     for (hipEvent_t e : F->ev) (void)hipEventDestroy(e);
@@ -1429,7 +1495,7 @@ extern "C" int mxc_factor_stats(const mxc_factor *F, int64_t *counts, double *ms
         for (const Launch &L : F->factor_launches) {
             switch (L.kind) {
                 case L_UPD: counts[0]++; break;
-                case L_DIAG: case L_DIAG_SMALL: case L_DIAG_TINY: counts[1]++; break;
+                case L_DIAG: case L_DIAG_SMALL: case L_DIAG_TINY: case L_SMALL: counts[1]++; break;
                 case L_TRSM: case L_TRSM_TINY: counts[2]++; break;
                 case L_TRAIL: case L_TRAIL2: case L_TRAILD: counts[3]++; break;
                 default: break;
@@ -1664,7 +1730,8 @@ static int enqueue_factor(mxc_factor *F) {
             continue;
         }
         const int cls = L.kind == L_UPD ? 1
-                      : (L.kind == L_DIAG || L.kind == L_DIAG_SMALL || L.kind == L_DIAG_TINY) ? 2
+                      : (L.kind == L_DIAG || L.kind == L_DIAG_SMALL || L.kind == L_DIAG_TINY ||
+                         L.kind == L_SMALL) ? 2
                       : (L.kind == L_TRSM || L.kind == L_TRSM_TINY) ? 3 : 4;
         tick(F, cls, &tt);
         switch (L.kind) {
@@ -1709,6 +1776,14 @@ static int enqueue_factor(mxc_factor *F) {
                 F->counts[1]++;
                 break;
             }
+            case L_SMALL:
+                if constexpr (std::is_same<T, double>::value) {
+                    const long long g = std::min<long long>((L.count + 3) / 4, (long long)device_cus() * 8);
+                    hipLaunchKernelGGL(k_small_level, dim3((unsigned)g), dim3(256), 0, sm, F->d_small + L.begin,
+                                       (int)L.count, F->d_desc, W, F->d_Ls, F->d_fail);
+                }
+                F->counts[1]++;
+                break;
             case L_TRSM_TINY: {
                 const bool h16 = std::is_same<T, float>::value && F->split16 && F->d_Lh;
                 hipLaunchKernelGGL(k_trsm_tiny<T>, dim3((unsigned)((L.count + 255) / 256)), dim3(256), 0, sm,
@@ -2683,6 +2758,10 @@ extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
         case MXC_OPT_FOLD_TRSM:
             if (F->fold_trsm == (value != 0)) return MXC_OK;
             F->fold_trsm = value != 0;
+            return flatten_upload(F);
+        case MXC_OPT_SMALL_FUSED:
+            if (F->small_fused == (value != 0)) return MXC_OK;
+            F->small_fused = value != 0;
             return flatten_upload(F);
         default:
             return MXC_INVALID;

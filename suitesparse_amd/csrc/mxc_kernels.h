@@ -57,6 +57,16 @@ struct DiagTask {
     int pad;
 };
 
+// A small supernode factored whole by one wave (k_small_level): its
+// descendant updates are descs[u0 .. u0 + nu), in descendant order.
+struct SmallTask {
+    long long lx;      // W offset of the supernode (Lpx[s])
+    long long winv;    // W offset of the inverse of its diagonal block (ld nscol)
+    int nsrow, nscol;
+    int k1, psi;       // first column, Ls offset of its rows
+    int u0, nu;
+};
+
 // Solve work item: column block (s, t) and a row range [r0, r1) of s.
 // One workgroup of a supernodal solve step (forward or backward sweep).
 //   diag: block t of supernode s, columns [c0, c0 + nb); r0 = width of the
