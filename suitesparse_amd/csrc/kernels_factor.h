@@ -1264,11 +1264,16 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
     __shared__ double S[4][SMALL_NROW * SD];
     __shared__ int rows[4][SMALL_NROW];
     __shared__ int rel[4][SMALL_NROW];
+    __shared__ double rpiv[4][SMALL_COLS];     // r_j = 1/sqrt(d_jj) of each column
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double *Sw = S[w];
     int *rw = rows[w], *rl = rel[w];
-    for (int ti = blockIdx.x * 4 + w; ti < ntask; ti += gridDim.x * 4) {
-        const SmallTask t = tasks[ti];
+    const int stride = gridDim.x * 4;
+    SmallTask tn = {};
+    if (blockIdx.x * 4 + w < ntask) tn = tasks[blockIdx.x * 4 + w];
+    for (int ti = blockIdx.x * 4 + w; ti < ntask; ti += stride) {
+        const SmallTask t = tn;
+        if (ti + stride < ntask) tn = tasks[ti + stride];      // the next task's descriptor in flight
         const int nsrow = t.nsrow, nscol = t.nscol;
         double a[SMALL_COLS];
 #pragma unroll
@@ -1280,8 +1285,12 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
         wave_sync_lds();
 
         const int jl = lane & 15, kq = lane >> 4;
+        UpdDesc un = {};
+        if (t.nu > 0) un = descs[t.u0];
+#pragma unroll 1
         for (int p = t.u0; p < t.u0 + t.nu; p++) {
-            const UpdDesc u = descs[p];
+            const UpdDesc u = un;
+            if (p + 1 < t.u0 + t.nu) un = descs[p + 1];       // the next pair's descriptor in flight
             if (lane < u.M) {
                 const int g = Ls[u.ls_off + lane];
                 int r;
@@ -1301,18 +1310,29 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
             f64x4 acc[4];
 #pragma unroll
             for (int n = 0; n < 4; n++) acc[n] = f64x4{0, 0, 0, 0};
-            const double *xp = W + u.x_off + jl, *yp = W + u.y_off + jl;
-            for (int k0 = 0; k0 < u.K; k0 += 4) {
-                const int k = k0 + kq;
-                const bool kin = k < u.K;
-                const double xa = (kin && jl < u.N) ? xp[(long long)k * u.ldx] : 0.0;
-                double yb[4];
+            // 8 columns of the descendant per pass: all 10 loads of a lane in
+            // flight at once (32-bit offsets from the pair's base), then the
+            // MFMAs in ascending k
+            const double *xb = W + u.x_off, *yb0 = W + u.y_off;
+#pragma unroll 1
+            for (int k0 = 0; k0 < u.K; k0 += 8) {
+                double xa[2], yb[2][4];
 #pragma unroll
-                for (int n = 0; n < 4; n++)
-                    yb[n] = (kin && 16 * n + jl < u.M) ? yp[16 * n + (long long)k * u.ldy] : 0.0;
+                for (int c = 0; c < 2; c++) {
+                    const int k = k0 + 4 * c + kq;
+                    const bool kin = k < u.K;
+                    xa[c] = (kin && jl < u.N) ? xb[jl + k * u.ldx] : 0.0;
 #pragma unroll
-                for (int n = 0; n < 4; n++)
-                    if (n < nblk) acc[n] = MfmaOp<double>::mma(xa, yb[n], acc[n]);
+                    for (int n = 0; n < 4; n++)
+                        yb[c][n] = (kin && 16 * n + jl < u.M) ? yb0[16 * n + jl + k * u.ldy] : 0.0;
+                }
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    if (k0 + 4 * c >= u.K) break;
+#pragma unroll
+                    for (int n = 0; n < 4; n++)
+                        if (n < nblk) acc[n] = MfmaOp<double>::mma(xa[c], yb[c][n], acc[n]);
+                }
             }
             wave_sync_lds();                 // rel[] of every lane
             // acc[n][v] = C(i, j), i = 16 n + lane % 16, j = row(lane, v)
@@ -1333,18 +1353,16 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
         // the panel, right-looking, lane r owning row r
 #pragma unroll
         for (int c = 0; c < SMALL_COLS; c++) a[c] = c < nscol ? Sw[lane * SD + c] : 0.0;
-        double rinv[SMALL_COLS];
         int badj = SMALL_COLS;
 #pragma unroll
         for (int j = 0; j < SMALL_COLS; j++) {
-            rinv[j] = 0.0;
             if (j >= nscol) continue;
             double djj = lane_bcast(a[j], j);
             const bool bad = !(djj > 0.0);
             badj = (bad && badj == SMALL_COLS) ? j : badj;
             djj = bad ? 1.0 : djj;
             const double rj = rsqrt_nr(djj);
-            rinv[j] = rj;
+            if (lane == j) rpiv[w][j] = rj;
             const double l = (lane > j) ? a[j] * rj : ((lane == j) ? djj * rj : 0.0);
             a[j] = l;
 #pragma unroll
@@ -1363,15 +1381,17 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
             for (int c = 0; c < SMALL_COLS; c++) Sw[lane * SD + c] = a[c];
         }
         wave_sync_lds();
-        double x[SMALL_COLS], sm[SMALL_COLS];
+        // x overwrites the partial sums in place: x_k = sm_k r_k once sm_k is final
+        double x[SMALL_COLS];
 #pragma unroll
-        for (int i = 0; i < SMALL_COLS; i++) sm[i] = (i == lane) ? 1.0 : 0.0;
+        for (int i = 0; i < SMALL_COLS; i++) x[i] = (i == lane) ? 1.0 : 0.0;
 #pragma unroll
         for (int k = 0; k < SMALL_COLS; k++) {
-            x[k] = sm[k] * rinv[k];
+            if (k >= nscol) break;
+            x[k] = x[k] * rpiv[w][k];
 #pragma unroll
             for (int i = k + 1; i < SMALL_COLS; i++)
-                if (i < nscol) sm[i] -= Sw[i * SD + k] * x[k];
+                if (i < nscol) x[i] -= Sw[i * SD + k] * x[k];
         }
         if (lane < nscol) {
 #pragma unroll

@@ -458,12 +458,15 @@ static int flatten_upload(mxc_factor *F) {
     std::vector<SmallTask> smalls;
     F->factor_launches.clear();
     int64_t nupd = 0;
-    // FP64: supernodes of at most SMALL_COLS columns and SMALL_NROW rows are
-    // factored whole by k_small_level, one launch at the head of their level
-    // (their update pairs, diagonal tasks and TRSM blocks leave the level's
-    // other launches).  task_of[s]: the level's task of supernode s, or -1.
+    // FP64: supernodes of at most SMALL_COLS columns and SMALL_NROW rows that
+    // have descendant updates are factored whole by k_small_level, one launch
+    // at the head of their level (their update pairs, diagonal tasks and TRSM
+    // blocks leave the level's other launches).  Those without updates (the
+    // leaves) keep the one-lane and one-wave POTRF / TRSM launches, which take
+    // a whole level of them with far more parallelism than a wave each.
+    // task_of[s]: the level's task of supernode s, or -1; npair[s]: its pairs.
     const bool fuse = F->small_fused && F->precision == MXC_FP64;
-    std::vector<int> task_of(fuse ? F->nsuper : 0, -1);
+    std::vector<int> task_of(fuse ? F->nsuper : 0, -1), npair(fuse ? F->nsuper : 0, 0);
     auto small_s = [&](int64_t s) {
         return F->Super[s + 1] - F->Super[s] <= SMALL_COLS && F->Lpi[s + 1] - F->Lpi[s] <= SMALL_NROW;
     };
@@ -559,9 +562,14 @@ static int flatten_upload(mxc_factor *F) {
         if (fuse && !F->blk_steps[l].empty()) {
             const StepBlocks &s0 = F->blk_steps[l][0];
             const long long t0 = (long long)smalls.size();
+            std::vector<int64_t> tgt(F->blk_upd[l].size());
+            for (size_t q = 0; q < tgt.size(); q++) {
+                tgt[q] = super_of(F->blk_upd[l][q].base);
+                npair[tgt[q]]++;
+            }
             for (size_t q = 0; q < s0.diags.size(); q++) {
                 const int64_t s = s0.diag_s[q];
-                if (!small_s(s)) continue;
+                if (!small_s(s) || npair[s] == 0) continue;
                 task_of[s] = (int)(smalls.size() - t0);
                 smalls.push_back(SmallTask{(long long)F->Lpx[s], s0.diags[q].winv,
                                            (int)(F->Lpi[s + 1] - F->Lpi[s]), (int)(F->Super[s + 1] - F->Super[s]),
@@ -571,9 +579,9 @@ static int flatten_upload(mxc_factor *F) {
             const long long nt = (long long)smalls.size() - t0;
             if (nt > 0) {
                 std::vector<std::vector<UpdDesc>> pairs(nt);
-                for (const UpdDesc &u : F->blk_upd[l]) {
-                    const int64_t s = super_of(u.base);
-                    if (s >= 0 && task_of[s] >= 0) pairs[task_of[s]].push_back(u);
+                for (size_t q = 0; q < tgt.size(); q++) {
+                    const UpdDesc &u = F->blk_upd[l][q];
+                    if (task_of[tgt[q]] >= 0) pairs[task_of[tgt[q]]].push_back(u);
                     else rest.push_back(u);
                 }
                 for (long long i = 0; i < nt; i++) {
