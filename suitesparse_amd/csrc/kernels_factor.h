@@ -1285,83 +1285,53 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
         wave_sync_lds();
 
         const int jl = lane & 15, kq = lane >> 4;
-        // Pairs in a two-stage pipeline: pair p's row indices and its first 8
-        // descendant columns are loaded while pair p - 1 is computed, and pair
-        // p + 1's descriptor while pair p's loads are issued.
-        const int pend = t.u0 + t.nu;
-        UpdDesc u = {}, un = {};
-        if (t.nu > 0) u = descs[t.u0];
-        if (t.nu > 1) un = descs[t.u0 + 1];
-        int gq = 0;                      // global row of block row `lane` (clamped)
-        double xa[2], yb[2][4];
-        auto issue = [&](const UpdDesc &d, int &g, double (&x)[2], double (&y)[2][4]) {
-            g = Ls[d.ls_off + min(lane, d.M - 1)];
-            const double *xb = W + d.x_off, *yb0 = W + d.y_off;
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-                const int k = 4 * c + kq;
-                const bool kin = k < d.K;
-                x[c] = (kin && jl < d.N) ? xb[jl + k * d.ldx] : 0.0;
-#pragma unroll
-                for (int n = 0; n < 4; n++)
-                    y[c][n] = (kin && 16 * n + jl < d.M) ? yb0[16 * n + jl + k * d.ldy] : 0.0;
-            }
-        };
-        if (t.nu > 0) issue(u, gq, xa, yb);
+        UpdDesc un = {};
+        if (t.nu > 0) un = descs[t.u0];
 #pragma unroll 1
-        for (int p = t.u0; p < pend; p++) {
-            int gn = 0;
-            double xn[2], yn[2][4];
-            UpdDesc un2 = {};
-            if (p + 1 < pend) issue(un, gn, xn, yn);
-            if (p + 2 < pend) un2 = descs[p + 2];
-            // relative map of the block rows
+        for (int p = t.u0; p < t.u0 + t.nu; p++) {
+            const UpdDesc u = un;
+            if (p + 1 < t.u0 + t.nu) un = descs[p + 1];       // the next pair's descriptor in flight
             if (lane < u.M) {
+                const int g = Ls[u.ls_off + lane];
                 int r;
-                if (gq < t.k1 + nscol) {
-                    r = gq - t.k1;
+                if (g < t.k1 + nscol) {
+                    r = g - t.k1;
                 } else {
                     int lo = nscol, hi = nsrow - 1;
                     while (lo < hi) {
                         const int mid = (lo + hi) >> 1;
-                        if (rw[mid] < gq) lo = mid + 1; else hi = mid;
+                        if (rw[mid] < g) lo = mid + 1; else hi = mid;
                     }
                     r = lo;
                 }
                 rl[lane] = r;
             }
-            // C = Y X': the first 8 columns from the registers, the rest (a
-            // descendant of more than 8 columns) 8 per pass, loaded here
             const int nblk = (u.M + 15) >> 4;
             f64x4 acc[4];
 #pragma unroll
             for (int n = 0; n < 4; n++) acc[n] = f64x4{0, 0, 0, 0};
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-                if (4 * c >= u.K) break;
-#pragma unroll
-                for (int n = 0; n < 4; n++)
-                    if (n < nblk) acc[n] = MfmaOp<double>::mma(xa[c], yb[c][n], acc[n]);
-            }
+            // 8 columns of the descendant per pass: all 10 loads of a lane in
+            // flight at once (32-bit offsets from the pair's base), then the
+            // MFMAs in ascending k
             const double *xb = W + u.x_off, *yb0 = W + u.y_off;
 #pragma unroll 1
-            for (int k0 = 8; k0 < u.K; k0 += 8) {
-                double xk[2], yk[2][4];
+            for (int k0 = 0; k0 < u.K; k0 += 8) {
+                double xa[2], yb[2][4];
 #pragma unroll
                 for (int c = 0; c < 2; c++) {
                     const int k = k0 + 4 * c + kq;
                     const bool kin = k < u.K;
-                    xk[c] = (kin && jl < u.N) ? xb[jl + k * u.ldx] : 0.0;
+                    xa[c] = (kin && jl < u.N) ? xb[jl + k * u.ldx] : 0.0;
 #pragma unroll
                     for (int n = 0; n < 4; n++)
-                        yk[c][n] = (kin && 16 * n + jl < u.M) ? yb0[16 * n + jl + k * u.ldy] : 0.0;
+                        yb[c][n] = (kin && 16 * n + jl < u.M) ? yb0[16 * n + jl + k * u.ldy] : 0.0;
                 }
 #pragma unroll
                 for (int c = 0; c < 2; c++) {
                     if (k0 + 4 * c >= u.K) break;
 #pragma unroll
                     for (int n = 0; n < 4; n++)
-                        if (n < nblk) acc[n] = MfmaOp<double>::mma(xk[c], yk[c][n], acc[n]);
+                        if (n < nblk) acc[n] = MfmaOp<double>::mma(xa[c], yb[c][n], acc[n]);
                 }
             }
             wave_sync_lds();                 // rel[] of every lane
@@ -1378,15 +1348,6 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
                 }
             }
             wave_sync_lds();                 // the block's stores, before the next pair's map
-            u = un;
-            un = un2;
-            gq = gn;
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-                xa[c] = xn[c];
-#pragma unroll
-                for (int n = 0; n < 4; n++) yb[c][n] = yn[c][n];
-            }
         }
 
         // the panel, right-looking, lane r owning row r
