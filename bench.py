@@ -162,6 +162,7 @@ COUNTER_CLASSES = [
     ("solve sweeps", lambda k: any(t in k for t in ("k_chain_", "k_sweep_", "k_perm_"))),
     ("residual + norms", lambda k: any(t in k for t in ("k_residual", "k_sumsq"))),
     ("update/downdate", lambda k: "k_updown" in k),
+    ("small supernodes, fused", lambda k: "k_small_level" in k),
 ]
 
 
@@ -803,17 +804,17 @@ def batch_section(args, out):
                      "factor_gflops": round(L.fl / tf / 1e9, 1),
                      "max_relres_per_system": rel, "refine": info}
         if prec == "fp64":
-            # the same factorization with each level's small supernodes in the
-            # level's update, POTRF and TRSM launches instead of the fused one
-            L.set_small_fused(False)
+            # the same factorization with each level's small supernodes factored
+            # whole, one wave each, in one launch per level (k_small_level)
+            L.set_small_fused(True)
             cholmod.factorize(b.A, L, cmp, values=Ax)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             for _ in range(steps):
                 cholmod.factorize(b.A, L, cmp, values=Ax)
             torch.cuda.synchronize()
-            res[prec]["t_factor_ms_small_unfused"] = round((time.perf_counter() - t1) / steps * 1e3, 2)
-            L.set_small_fused(True)
+            res[prec]["t_factor_ms_small_fused"] = round((time.perf_counter() - t1) / steps * 1e3, 2)
+            L.set_small_fused(False)
         log(f"[bench]   batch {prec}: {res[prec]}")
         del X, Xh, R
         gc.collect()

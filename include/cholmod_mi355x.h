@@ -338,11 +338,15 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * room for the main stream's diagonal factorizations, which otherwise wait
  * for the whole second-stream launch to drain. */
 #define MXC_OPT_T2_GRID 8
-/* MXC_OPT_SMALL_FUSED (default 1, FP64 factors): the supernodes of at most 16
- * columns and 64 rows of each etree level are factored whole by one launch,
- * one wave per supernode on a persistent grid: its descendant updates,
- * diagonal factorization, inverse and TRSM, held in LDS throughout.  Without
- * it they take the level's update, POTRF and TRSM launches like the others. */
+/* MXC_OPT_SMALL_FUSED (default 0, FP64 factors): the supernodes of at most 16
+ * columns and 64 rows that have descendant updates are factored whole by one
+ * launch per etree level, one wave per supernode on a persistent grid: its
+ * descendant updates, diagonal factorization, inverse and TRSM, held in LDS
+ * throughout.  Off by default: a wave per supernode takes its update pairs
+ * one after another at three waves per SIMD, which measured slower than the
+ * level's update, POTRF and TRSM launches, whose independent tiles keep many
+ * more waves in flight (4,096 2D systems: 34.0 vs 31.4 ms per FP64
+ * factorization; a 300 x 300 grid: 2.9 vs 2.6 ms). */
 #define MXC_OPT_SMALL_FUSED 9
 int mxc_set_option(mxc_factor *F, int option, int64_t value);
 

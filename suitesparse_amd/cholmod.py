@@ -235,7 +235,8 @@ class Factor:
 
     def set_small_fused(self, on: bool):
         """FP64: factor each level's supernodes of at most 16 columns and 64 rows
-        whole, one wave each, in one launch per level (default on)."""
+        (those with descendant updates) whole, one wave each, in one launch per
+        level (default off: measured slower than the level's separate launches)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 9, 1 if on else 0))
 
     def set_graphs(self, on: bool):

@@ -242,7 +242,7 @@ struct mxc_factor {
     int64_t nready = 0;                  // diagonal tasks covered by d_ready
     bool merge_t2 = true;                // FP64: fold second-stream updates into later trailing launches
     int t2_per_cu = 3;                   // FP64 second-stream trailing launches: workgroups per CU (0: one per tile)
-    bool small_fused = true;             // FP64: small supernodes of a level in one launch (k_small_level)
+    bool small_fused = false;            // FP64: small supernodes of a level in one launch (k_small_level)
     int64_t nsmall = 0;                  // supernodes the plan factors that way
     bool deterministic = false;          // descendant updates in rounds: one pair per target per launch
     size_t bytes = 0;

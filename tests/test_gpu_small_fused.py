@@ -1,7 +1,8 @@
 """GPU: the fused small-supernode launch (k_small_level, FP64 factors).
 
-Supernodes of at most 16 columns and 64 rows are factored whole, one wave
-each, in one launch per etree level.  On the same inputs, the factor with
+Supernodes of at most 16 columns and 64 rows that have descendant updates
+are factored whole, one wave each, in one launch per etree level (an option,
+off by default).  On the same inputs, the factor with
 the launch on must match the factor with it off (the level's update, POTRF
 and TRSM launches) to 1e-13 * max|L| (the update sums and the TRSM differ in
 order of operations only), both must match the reference's factor to the
@@ -70,6 +71,8 @@ def test_fused_batch_matches_reference(gpu, ref_lib):
     mats = [problems.grid2d_lognormal(int(s), rng, shift=1e-3) for s in rng.integers(6, 40, size=60)]
     cm = cholmod.Common(precision="fp64")
     b = batch.analyze_batch(mats, cm)
+    b.L._numeric(cm)
+    b.L.set_small_fused(True)
     assert batch.factorize_batch(b), cm.status
     B = np.random.default_rng(2).uniform(-1, 1, (b.A.n, 3))
     X = batch.solve_batch(b, B)
