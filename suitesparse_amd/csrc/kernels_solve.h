@@ -737,11 +737,10 @@ __global__ __launch_bounds__(256) void k_sweep_small_bwd(const SweepTask *__rest
 // inverse of the diagonal block at the end.  A per-supernode counter of
 // finished blocks passes blocks between the workgroups of a launch: block
 // values and counters are read and written with device-scope atomic loads and
-// stores (performed at the device coherence point), ordered by the memory
-// model: every producer thread issues an agent-scope release fence after its
-// block stores, the workgroup barrier follows, and one thread stores the
-// counter with release semantics; a consumer that sees the counter issues an
-// agent-scope acquire fence before its block loads.  The launch order puts
+// stores (performed at the device coherence point, no cache maintenance), and
+// a producer waits for its block's stores to complete (s_waitcnt) before the
+// workgroup barrier that precedes the counter store; a consumer issues its
+// block loads only after its counter load returned.  The launch order puts
 // every block after the blocks it depends on, so a workgroup waits only for
 // workgroups of lower index: the lowest unfinished workgroup never waits, and
 // workgroups start in index order, so every wait ends.  A wait is still
@@ -780,15 +779,15 @@ __device__ __forceinline__ int chain_wait(const int *cnt, int need, int *abort) 
         }
         c = __builtin_amdgcn_readfirstlane(ld_dev(cnt));
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // block loads after the counter load
+    asm volatile("" ::: "memory");   // block loads stay after the counter load
     return c;
 }
 
 // Publish block count v of a supernode once every thread's block stores are done.
 __device__ __forceinline__ void chain_publish(int *cnt, int v) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(cnt, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) st_dev(cnt, v);
 }
 
 __global__ void k_chain_reset(int *__restrict__ cnt, int m) {
@@ -1052,9 +1051,9 @@ __global__ __launch_bounds__(256) void k_chain_bwd(const SweepTask *__restrict__
         __syncthreads();
     }
     if (below) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (tid == 0) __hip_atomic_fetch_add(cnt + t.p0, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) __hip_atomic_fetch_add(cnt + t.p0, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
         chain_publish(cnt + t.p0, nblk - tb);
     }
