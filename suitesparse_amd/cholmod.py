@@ -233,11 +233,13 @@ class Factor:
         each walking its share of the tiles (default 3; 0: one workgroup per tile)."""
         check("mxc_set_option", lib().mxc_set_option(self._num, 8, int(per_cu)))
 
-    def set_small_fused(self, on: bool):
-        """FP64: factor each level's supernodes of at most 16 columns and 64 rows
-        (those with descendant updates) whole, one wave each, in one launch per
-        level (default off: measured slower than the level's separate launches)."""
-        check("mxc_set_option", lib().mxc_set_option(self._num, 9, 1 if on else 0))
+    def set_small_fused(self, max_per_level):
+        """FP64: factor the supernodes of at most 16 columns and 64 rows (those
+        with descendant updates) whole, a workgroup each, in one launch per level,
+        in every level that has at most max_per_level of them (True: every level,
+        False or 0: none)."""
+        v = (1 << 62) if max_per_level is True else int(max_per_level)
+        check("mxc_set_option", lib().mxc_set_option(self._num, 9, v))
 
     def set_graphs(self, on: bool):
         """Replay the factorization / solve sweeps from captured hipGraphs (default on)."""

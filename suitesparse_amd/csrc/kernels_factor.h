@@ -1236,23 +1236,23 @@ __global__ __launch_bounds__(256) void k_trsm_tiny(const UpdDesc *__restrict__ d
 // numeric factorization (t_cholmod_super_numeric_worker.c:537-1190: the
 // descendant updates by dsyrk/dgemm scattered through the relative map, then
 // dpotrf of the diagonal block and dtrsm of the rows below) is done by one
-// wave, the supernode held in LDS from its assembly to its last store.  Four
-// waves per workgroup walk the level's tasks on a grid sized to the device
-// (a persistent grid: task w + 4 b, then + 4 gridDim.x, ...).  This replaces
-// the level's descendant-update launch (atomic scatter into W), its POTRF
-// launch and its TRSM launch for these supernodes by one launch with no
-// atomics, and each descendant's contributions land in descendant order.
+// workgroup, the supernode held in LDS from its assembly to its last store.
+// The workgroups walk the level's tasks on a grid sized to the device (a
+// persistent grid: task b, then b + gridDim.x, ...).  This replaces the
+// level's descendant-update launch, its POTRF launch and its TRSM launch for
+// these supernodes by one launch.
 //
 // Per descendant pair (the UpdDesc of the update launch, MODE_SCATTER
-// semantics): C = Y X' by 16x16x4 FP64 MFMAs straight from W (row blocks of
-// 16 rows of Y against the N <= 16 rows of X), then S[rel(i)][rel(j)] -= C(i, j)
-// for i >= j, rel the block rows' positions among the supernode's rows.
-// Factorization: right-looking over the nscol columns with lane r owning row
-// r of the panel, potrf16_wave's pivot arithmetic (r_j = 1/sqrt(d_jj),
-// L_jj = d_jj r_j, L_rj = a_rj r_j, a non-positive pivot reported and replaced
-// by 1) applied to the rows below the diagonal block as well, which makes the
-// TRSM part of the same pass.  The inverse of the diagonal block is formed as
-// potrf16_wave forms it, column c by lane c.
+// semantics), wave w taking pairs w, w + 4, ...: C = Y X' by 16x16x4 FP64
+// MFMAs straight from W (row blocks of 16 rows of Y against the N <= 16 rows
+// of X), then S[rel(i)][rel(j)] -= C(i, j) for i >= j by LDS atomics, rel the
+// block rows' positions among the supernode's rows.  Factorization, by wave 0:
+// right-looking over the nscol columns with lane r owning row r of the panel,
+// potrf16_wave's pivot arithmetic (r_j = 1/sqrt(d_jj), L_jj = d_jj r_j,
+// L_rj = a_rj r_j, a non-positive pivot reported and replaced by 1) applied
+// to the rows below the diagonal block as well, which makes the TRSM part of
+// the same pass.  The inverse of the diagonal block is formed as potrf16_wave
+// forms it, column c by lane c.
 // ---------------------------------------------------------------------------
 constexpr int SMALL_COLS = 16;
 constexpr int SMALL_NROW = 64;
@@ -1261,36 +1261,28 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
                                                      const UpdDesc *__restrict__ descs, double *__restrict__ W,
                                                      const int *__restrict__ Ls, int *__restrict__ fail) {
     constexpr int SD = SMALL_COLS + 1;
-    __shared__ double S[4][SMALL_NROW * SD];
-    __shared__ int rows[4][SMALL_NROW];
+    __shared__ double S[SMALL_NROW * SD];
+    __shared__ int rows[SMALL_NROW];
     __shared__ int rel[4][SMALL_NROW];
-    __shared__ double rpiv[4][SMALL_COLS];     // r_j = 1/sqrt(d_jj) of each column
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    double *Sw = S[w];
-    int *rw = rows[w], *rl = rel[w];
-    const int stride = gridDim.x * 4;
-    SmallTask tn = {};
-    if (blockIdx.x * 4 + w < ntask) tn = tasks[blockIdx.x * 4 + w];
-    for (int ti = blockIdx.x * 4 + w; ti < ntask; ti += stride) {
-        const SmallTask t = tn;
-        if (ti + stride < ntask) tn = tasks[ti + stride];      // the next task's descriptor in flight
+    __shared__ double rpiv[SMALL_COLS];        // r_j = 1/sqrt(d_jj) of each column
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int jl = lane & 15, kq = lane >> 4;
+    int *rl = rel[w];
+    for (int ti = blockIdx.x; ti < ntask; ti += gridDim.x) {
+        const SmallTask t = tasks[ti];
         const int nsrow = t.nsrow, nscol = t.nscol;
-        double a[SMALL_COLS];
+        // the assembled block (thread e: row e % 64 of columns e / 64, e / 64 + 4, ...) and its rows
 #pragma unroll
-        for (int c = 0; c < SMALL_COLS; c++)
-            a[c] = (lane < nsrow && c < nscol) ? W[t.lx + lane + (long long)c * nsrow] : 0.0;
-        rw[lane] = lane < nsrow ? Ls[t.psi + lane] : INT_MAX;
-#pragma unroll
-        for (int c = 0; c < SMALL_COLS; c++) Sw[lane * SD + c] = a[c];
-        wave_sync_lds();
+        for (int q = 0; q < SMALL_NROW * SMALL_COLS / 256; q++) {
+            const int r = lane, c = w + 4 * q;
+            S[r * SD + c] = (r < nsrow && c < nscol) ? W[t.lx + r + (long long)c * nsrow] : 0.0;
+        }
+        if (w == 0) rows[lane] = lane < nsrow ? Ls[t.psi + lane] : INT_MAX;
+        __syncthreads();
 
-        const int jl = lane & 15, kq = lane >> 4;
-        UpdDesc un = {};
-        if (t.nu > 0) un = descs[t.u0];
 #pragma unroll 1
-        for (int p = t.u0; p < t.u0 + t.nu; p++) {
-            const UpdDesc u = un;
-            if (p + 1 < t.u0 + t.nu) un = descs[p + 1];       // the next pair's descriptor in flight
+        for (int p = t.u0 + w; p < t.u0 + t.nu; p += 4) {
+            const UpdDesc u = descs[p];
             if (lane < u.M) {
                 const int g = Ls[u.ls_off + lane];
                 int r;
@@ -1300,7 +1292,7 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
                     int lo = nscol, hi = nsrow - 1;
                     while (lo < hi) {
                         const int mid = (lo + hi) >> 1;
-                        if (rw[mid] < g) lo = mid + 1; else hi = mid;
+                        if (rows[mid] < g) lo = mid + 1; else hi = mid;
                     }
                     r = lo;
                 }
@@ -1334,7 +1326,7 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
                         if (n < nblk) acc[n] = MfmaOp<double>::mma(xa[c], yb[c][n], acc[n]);
                 }
             }
-            wave_sync_lds();                 // rel[] of every lane
+            wave_sync_lds();                 // rel[] of every lane of the wave
             // acc[n][v] = C(i, j), i = 16 n + lane % 16, j = row(lane, v)
 #pragma unroll
             for (int n = 0; n < 4; n++) {
@@ -1344,61 +1336,67 @@ __global__ __launch_bounds__(256) void k_small_level(const SmallTask *__restrict
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const int j = MfmaOp<double>::row(lane, v);
-                    if (j < u.N && j <= i) Sw[ri * SD + rl[j]] -= acc[n][v];
+                    if (j < u.N && j <= i)
+                        __hip_atomic_fetch_add(&S[ri * SD + rl[j]], -acc[n][v], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
-            wave_sync_lds();                 // the block's stores, before the next pair's map
+            wave_sync_lds();                 // the map's reads, before the next pair's map
         }
+        __syncthreads();
 
-        // the panel, right-looking, lane r owning row r
+        if (w == 0) {
+            // the panel, right-looking, lane r owning row r
+            double a[SMALL_COLS];
 #pragma unroll
-        for (int c = 0; c < SMALL_COLS; c++) a[c] = c < nscol ? Sw[lane * SD + c] : 0.0;
-        int badj = SMALL_COLS;
+            for (int c = 0; c < SMALL_COLS; c++) a[c] = c < nscol ? S[lane * SD + c] : 0.0;
+            int badj = SMALL_COLS;
 #pragma unroll
-        for (int j = 0; j < SMALL_COLS; j++) {
-            if (j >= nscol) continue;
-            double djj = lane_bcast(a[j], j);
-            const bool bad = !(djj > 0.0);
-            badj = (bad && badj == SMALL_COLS) ? j : badj;
-            djj = bad ? 1.0 : djj;
-            const double rj = rsqrt_nr(djj);
-            if (lane == j) rpiv[w][j] = rj;
-            const double l = (lane > j) ? a[j] * rj : ((lane == j) ? djj * rj : 0.0);
-            a[j] = l;
+            for (int j = 0; j < SMALL_COLS; j++) {
+                if (j >= nscol) continue;
+                double djj = lane_bcast(a[j], j);
+                const bool bad = !(djj > 0.0);
+                badj = (bad && badj == SMALL_COLS) ? j : badj;
+                djj = bad ? 1.0 : djj;
+                const double rj = rsqrt_nr(djj);
+                if (lane == j) rpiv[j] = rj;
+                const double l = (lane > j) ? a[j] * rj : ((lane == j) ? djj * rj : 0.0);
+                a[j] = l;
 #pragma unroll
-            for (int c = j + 1; c < SMALL_COLS; c++)
-                if (c < nscol) a[c] -= l * lane_bcast(l, c);
+                for (int c = j + 1; c < SMALL_COLS; c++)
+                    if (c < nscol) a[c] -= l * lane_bcast(l, c);
+            }
+            if (lane == 0 && badj < nscol) atomicMin(fail, t.k1 + badj);
+            if (lane < nsrow) {
+#pragma unroll
+                for (int c = 0; c < SMALL_COLS; c++)
+                    if (c < nscol && c <= lane) W[t.lx + lane + (long long)c * nsrow] = a[c];
+            }
+            // the inverse of the diagonal block: lane c forms column c
+            if (lane < nscol) {
+#pragma unroll
+                for (int c = 0; c < SMALL_COLS; c++) S[lane * SD + c] = a[c];
+            }
+            wave_sync_lds();
+            // x overwrites the partial sums in place: x_k = sm_k r_k once sm_k is final
+            double x[SMALL_COLS];
+#pragma unroll
+            for (int i = 0; i < SMALL_COLS; i++) x[i] = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < SMALL_COLS; k++) {
+                if (k >= nscol) break;
+                x[k] = x[k] * rpiv[k];
+#pragma unroll
+                for (int i = k + 1; i < SMALL_COLS; i++)
+                    if (i < nscol) x[i] -= S[i * SD + k] * x[k];
+            }
+            if (lane < nscol) {
+#pragma unroll
+                for (int i = 0; i < SMALL_COLS; i++)
+                    if (i < nscol) W[t.winv + i + (long long)lane * nscol] = (i >= lane) ? x[i] : 0.0;
+            }
         }
-        if (lane == 0 && badj < nscol) atomicMin(fail, t.k1 + badj);
-        if (lane < nsrow) {
-#pragma unroll
-            for (int c = 0; c < SMALL_COLS; c++)
-                if (c < nscol && c <= lane) W[t.lx + lane + (long long)c * nsrow] = a[c];
-        }
-        // the inverse of the diagonal block: lane c forms column c
-        if (lane < nscol) {
-#pragma unroll
-            for (int c = 0; c < SMALL_COLS; c++) Sw[lane * SD + c] = a[c];
-        }
-        wave_sync_lds();
-        // x overwrites the partial sums in place: x_k = sm_k r_k once sm_k is final
-        double x[SMALL_COLS];
-#pragma unroll
-        for (int i = 0; i < SMALL_COLS; i++) x[i] = (i == lane) ? 1.0 : 0.0;
-#pragma unroll
-        for (int k = 0; k < SMALL_COLS; k++) {
-            if (k >= nscol) break;
-            x[k] = x[k] * rpiv[w][k];
-#pragma unroll
-            for (int i = k + 1; i < SMALL_COLS; i++)
-                if (i < nscol) x[i] -= Sw[i * SD + k] * x[k];
-        }
-        if (lane < nscol) {
-#pragma unroll
-            for (int i = 0; i < SMALL_COLS; i++)
-                if (i < nscol) W[t.winv + i + (long long)lane * nscol] = (i >= lane) ? x[i] : 0.0;
-        }
-        wave_sync_lds();                     // this task's LDS reads, before the next task's stores
+        __syncthreads();                     // this task's LDS reads, before the next task's stores
     }
 }
 

@@ -242,7 +242,8 @@ struct mxc_factor {
     int64_t nready = 0;                  // diagonal tasks covered by d_ready
     bool merge_t2 = true;                // FP64: fold second-stream updates into later trailing launches
     int t2_per_cu = 3;                   // FP64 second-stream trailing launches: workgroups per CU (0: one per tile)
-    bool small_fused = false;            // FP64: small supernodes of a level in one launch (k_small_level)
+    int64_t small_max = 0;               // FP64: a level's small supernodes go to one k_small_level launch
+                                         // when it has at most this many of them (0: never)
     int64_t nsmall = 0;                  // supernodes the plan factors that way
     bool deterministic = false;          // descendant updates in rounds: one pair per target per launch
     size_t bytes = 0;
@@ -461,11 +462,15 @@ static int flatten_upload(mxc_factor *F) {
     // FP64: supernodes of at most SMALL_COLS columns and SMALL_NROW rows that
     // have descendant updates are factored whole by k_small_level, one launch
     // at the head of their level (their update pairs, diagonal tasks and TRSM
-    // blocks leave the level's other launches).  Those without updates (the
-    // leaves) keep the one-lane and one-wave POTRF / TRSM launches, which take
-    // a whole level of them with far more parallelism than a wave each.
+    // blocks leave the level's other launches), in a level with at most
+    // small_max of them: a short level is latency-bound, and the fused launch
+    // replaces three launches there and runs a supernode's pairs four at a
+    // time; a wide level keeps the separate launches, whose independent tiles
+    // keep more waves in flight.  Those without updates (the leaves) keep the
+    // one-lane and one-wave POTRF / TRSM launches.  Not in deterministic mode
+    // (the fused launch adds a supernode's pairs with LDS atomics).
     // task_of[s]: the level's task of supernode s, or -1; npair[s]: its pairs.
-    const bool fuse = F->small_fused && F->precision == MXC_FP64;
+    const bool fuse = F->small_max > 0 && F->precision == MXC_FP64 && !F->deterministic;
     std::vector<int> task_of(fuse ? F->nsuper : 0, -1), npair(fuse ? F->nsuper : 0, 0);
     auto small_s = [&](int64_t s) {
         return F->Super[s + 1] - F->Super[s] <= SMALL_COLS && F->Lpi[s + 1] - F->Lpi[s] <= SMALL_NROW;
@@ -576,7 +581,13 @@ static int flatten_upload(mxc_factor *F) {
                                            (int)F->Super[s], (int)F->Lpi[s], 0, 0});
                 fused_small.push_back(s);
             }
-            const long long nt = (long long)smalls.size() - t0;
+            long long nt = (long long)smalls.size() - t0;
+            if (nt > F->small_max) {            // a wide level: the separate launches
+                for (int64_t s : fused_small) task_of[s] = -1;
+                fused_small.clear();
+                smalls.resize(t0);
+                nt = 0;
+            }
             if (nt > 0) {
                 std::vector<std::vector<UpdDesc>> pairs(nt);
                 for (size_t q = 0; q < tgt.size(); q++) {
@@ -1786,7 +1797,7 @@ static int enqueue_factor(mxc_factor *F) {
             }
             case L_SMALL:
                 if constexpr (std::is_same<T, double>::value) {
-                    const long long g = std::min<long long>((L.count + 3) / 4, (long long)device_cus() * 8);
+                    const long long g = std::min<long long>(L.count, (long long)device_cus() * 8);
                     hipLaunchKernelGGL(k_small_level, dim3((unsigned)g), dim3(256), 0, sm, F->d_small + L.begin,
                                        (int)L.count, F->d_desc, W, F->d_Ls, F->d_fail);
                 }
@@ -2768,8 +2779,9 @@ extern "C" int mxc_set_option(mxc_factor *F, int option, int64_t value) {
             F->fold_trsm = value != 0;
             return flatten_upload(F);
         case MXC_OPT_SMALL_FUSED:
-            if (F->small_fused == (value != 0)) return MXC_OK;
-            F->small_fused = value != 0;
+            if (value < 0) return MXC_INVALID;
+            if (F->small_max == value) return MXC_OK;
+            F->small_max = value;
             return flatten_upload(F);
         default:
             return MXC_INVALID;
