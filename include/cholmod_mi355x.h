@@ -338,15 +338,15 @@ int mxc_solve_refine(mxc_factor *F, const double *B, int64_t nrhs, double *X, in
  * room for the main stream's diagonal factorizations, which otherwise wait
  * for the whole second-stream launch to drain. */
 #define MXC_OPT_T2_GRID 8
-/* MXC_OPT_SMALL_FUSED (default 0, FP64 factors): the supernodes of at most 16
- * columns and 64 rows that have descendant updates are factored whole by one
- * launch per etree level, one wave per supernode on a persistent grid: its
- * descendant updates, diagonal factorization, inverse and TRSM, held in LDS
- * throughout.  Off by default: a wave per supernode takes its update pairs
- * one after another at three waves per SIMD, which measured slower than the
- * level's update, POTRF and TRSM launches, whose independent tiles keep many
- * more waves in flight (4,096 2D systems: 34.0 vs 31.4 ms per FP64
- * factorization; a 300 x 300 grid: 2.9 vs 2.6 ms). */
+/* MXC_OPT_SMALL_FUSED (default 0, FP64 factors): in every etree level with at
+ * most this many supernodes of at most 16 columns and 64 rows that have
+ * descendant updates, those supernodes are factored whole by one launch, a
+ * workgroup per supernode on a persistent grid: its descendant updates (four
+ * pairs at a time), diagonal factorization, inverse and TRSM, held in LDS
+ * throughout (0: never; not in deterministic mode).  Off by default: every
+ * threshold measured slower than the level's update, POTRF and TRSM launches
+ * (a 300 x 300 grid: 2.70 ms with 64 against 2.55 ms; 4,096 2D systems with
+ * every level fused: 49.8 against 31.3 ms). */
 #define MXC_OPT_SMALL_FUSED 9
 int mxc_set_option(mxc_factor *F, int option, int64_t value);
 

@@ -1,8 +1,9 @@
 """GPU: the fused small-supernode launch (k_small_level, FP64 factors).
 
 Supernodes of at most 16 columns and 64 rows that have descendant updates
-are factored whole, one wave each, in one launch per etree level (an option,
-off by default).  On the same inputs, the factor with
+are factored whole, a workgroup each, in one launch per etree level, in the
+levels with at most a threshold of them (an option, off by default).  On the
+same inputs, the factor with
 the launch on must match the factor with it off (the level's update, POTRF
 and TRSM launches) to 1e-13 * max|L| (the update sums and the TRSM differ in
 order of operations only), both must match the reference's factor to the
@@ -64,6 +65,24 @@ def test_fused_matches_unfused_and_reference(name, make, ordering, gpu, ref_lib)
     # the option changes the computation: the fused rows below the diagonal
     # block come from elimination, the TRSM launch multiplies by the inverse
     assert d > 0
+
+
+@pytest.mark.parametrize("threshold", [8, 64])
+def test_fused_threshold_matches_unfused(threshold, gpu):
+    """Only the levels with at most `threshold` small supernodes are fused; the
+    others keep the separate launches in the same factorization."""
+    A = problems.grid2d_lognormal(80, np.random.default_rng(9))
+    xs = {}
+    for v in (0, threshold):
+        cm = cholmod.Common(nmethods=1, ordering=cholmod.CHOLMOD_AMD, precision="fp64")
+        L = cholmod.analyze(A, cm)
+        L._numeric(cm)
+        L.set_small_fused(v)
+        assert cholmod.factorize(A, L, cm) and cm.status == cholmod.CHOLMOD_OK
+        xs[v] = L.x
+    d = np.abs(xs[threshold] - xs[0]).max() / np.abs(xs[0]).max()
+    print(f"threshold {threshold}: vs unfused {d:.2e}", flush=True)
+    assert d <= 1e-13
 
 
 def test_fused_batch_matches_reference(gpu, ref_lib):
