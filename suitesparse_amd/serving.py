@@ -23,7 +23,12 @@ never by a finalizer on an arbitrary thread.
 
 The ordering is AMD for 2D grids and METIS for 3D grids (the product's own
 restatements, bit-exact with cholmod_l_analyze's AMD and METIS); the request's
-pattern key names its kind.
+pattern key names its kind.  A new 3D pattern is first served from an AMD plan
+(its analysis takes about a quarter of METIS's on one core), while a
+background thread orders the same pattern with METIS and builds that plan;
+the first request to take the plan after it is ready swaps it in (the METIS
+factor has fewer entries and flops), so only the first requests of a pattern
+wait for an analysis, and never for METIS's.
 """
 from __future__ import annotations
 
@@ -68,6 +73,7 @@ class _Plan:
     state: str = "building"        # "building" -> "ready"
     last_used: float = 0.0
     users: int = 0
+    pending: object = None         # (handle, nbytes) of the final-ordering plan, built in the background
 
 
 def plan_bytes_estimate(xsize: int, nsuper_cols_sq: int, nnz: int, n: int, nrhs: int,
@@ -98,8 +104,17 @@ class DeviceBackend:
         xguess = 40 * nnz + 64 * req.A.n
         return plan_bytes_estimate(xguess, 64 * req.A.n, nnz, req.A.n, req.B.shape[1], self.precision)
 
-    def build(self, req: Request):
-        ordering = cholmod.CHOLMOD_METIS if req.key.startswith("3d") else cholmod.CHOLMOD_AMD
+    def final_ordering(self, key: str) -> int:
+        return cholmod.CHOLMOD_METIS if key.startswith("3d") else cholmod.CHOLMOD_AMD
+
+    def first_ordering(self, key: str) -> int:
+        """The ordering of a pattern's first plan: AMD (a METIS plan follows
+        in the background when final_ordering differs)."""
+        return cholmod.CHOLMOD_AMD
+
+    def build(self, req: Request, ordering: int | None = None):
+        if ordering is None:
+            ordering = self.first_ordering(req.key)
         cm = cholmod.Common(nmethods=1, ordering=ordering, precision=self.precision)
         L = cholmod.analyze(req.A, cm)
         if L is None:
@@ -141,6 +156,8 @@ class Server:
         self.completed: list[Request] = []
         self.evictions = 0
         self.builds = 0
+        self.upgrades = 0               # plans replaced by their final-ordering plan
+        self._upgraders: list[threading.Thread] = []
         self.backend.init()
 
     # -------------------------------------------------------------- lifecycle
@@ -171,6 +188,8 @@ class Server:
             self._cv.notify_all()
         for th in self._threads:
             th.join()
+        for th in list(self._upgraders):
+            th.join()
         with self._cv:
             for p in list(self._plans.values()):
                 self._release(p)
@@ -191,6 +210,13 @@ class Server:
             if p is not None:
                 if p.users == 0 and p.state == "ready":
                     del self._queue[idx]
+                    if p.pending is not None:      # the final-ordering plan is ready: swap it in
+                        old, (p.handle, nbytes) = p.handle, p.pending
+                        p.pending = None
+                        self._reserved -= p.nbytes
+                        p.nbytes = nbytes
+                        self.backend.free(old)
+                        self.upgrades += 1
                     p.users += 1
                     req.reused_plan = True
                     return req, p
@@ -227,6 +253,10 @@ class Server:
         p.nbytes = 0
         if p.handle is not None:
             h, p.handle = p.handle, None
+            self.backend.free(h)
+        if p.pending is not None:
+            (h, nb), p.pending = p.pending, None
+            self._reserved -= nb
             self.backend.free(h)
 
     def _evict(self, nbytes: int):
@@ -282,6 +312,40 @@ class Server:
             plan.handle = handle
             plan.state = "ready"
             self.builds += 1
+            upgrade = (hasattr(self.backend, "final_ordering") and not self._stop and
+                       self.backend.final_ordering(req.key) != self.backend.first_ordering(req.key))
+            if upgrade:
+                # the pattern's matrices, held here: the request may drop them once it is done
+                pattern = Request(rid=req.rid, A=req.A, B=req.B, key=req.key)
+                th = threading.Thread(target=self._upgrade, args=(pattern, plan), name=f"upgrade-{req.key}",
+                                      daemon=True)
+                self._upgraders.append(th)
+                th.start()
+
+    def _upgrade(self, pattern: Request, plan: _Plan):
+        """Build the pattern's final-ordering plan beside the serving one and
+        leave it in plan.pending; its admission estimate is reserved first (no
+        upgrade when it does not fit)."""
+        req = pattern
+        with self._cv:
+            need = self.backend.estimate(pattern)
+            if self._stop or self._plans.get(plan.key) is not plan or self._reserved + need > self.budget:
+                return
+            self._reserved += need
+        try:
+            handle, nbytes = self.backend.build(pattern, self.backend.final_ordering(req.key))
+        except Exception:
+            with self._cv:
+                self._reserved -= need
+            return
+        with self._cv:
+            self._reserved += nbytes - need
+            if self._plans.get(plan.key) is plan and plan.state == "ready" and not self._stop:
+                plan.pending = (handle, nbytes)
+            else:                               # evicted or stopped meanwhile
+                self._reserved -= nbytes
+                self.backend.free(handle)
+            self._cv.notify_all()
 
 
 # ----------------------------------------------------------------------------
@@ -440,6 +504,7 @@ def run_stream(server: Server, reqs, arrivals, bank: PatternBank | None = None,
         "rhs_per_s": sum(nr) / span,
         "plan_reuse": sum(r.reused_plan for r in done) / max(1, len(done)),
         "evictions": server.evictions,
+        "plan_upgrades": server.upgrades,
         "max_relres": max((r.info or {}).get("relres", 0.0) for r in ok) if ok else None,
         "fp64_fallbacks": sum(bool((r.info or {}).get("fp64_fallback")) for r in ok),
         "late_submissions": late,

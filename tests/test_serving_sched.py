@@ -96,7 +96,7 @@ def _wait_done(srv, count, timeout=60.0):
 
 def _check_reserved(srv):
     with srv._cv:
-        assert srv._reserved == sum(p.nbytes for p in srv._plans.values())
+        assert srv._reserved == sum(p.nbytes + (p.pending[1] if p.pending else 0) for p in srv._plans.values())
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
@@ -170,3 +170,39 @@ def test_oversized_request_fails_without_blocking():
     assert all(r.status == -2 for r in srv.completed)
     srv.drain()
     assert srv.reserved_bytes() == 0
+
+
+class UpgradingStub(StubBackend):
+    """Patterns k0..k2 have a final ordering other than their first one: the
+    server builds their second plan in the background and swaps it in."""
+    UPGRADED = ("k0", "k1", "k2")
+
+    def first_ordering(self, key):
+        return 0
+
+    def final_ordering(self, key):
+        return 1 if key in self.UPGRADED else 0
+
+    def build(self, req, ordering=None):
+        return super().build(req)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_background_plan_upgrade(seed):
+    random.seed(seed)
+    be = UpgradingStub(build_s=0.02)
+    srv = serving.Server(slots=6, hbm_budget=1 << 40, backend=be)
+    reqs = _requests(400, 6, seed)
+    srv.start()
+    _submit_concurrently(srv, reqs)
+    _wait_done(srv, len(reqs))
+    srv.drain()                                   # joins the background builds too
+    assert not be.errors, be.errors[:5]
+    assert all(r.status == 0 for r in srv.completed)
+    keys = {r.key for r in reqs}
+    up = [k for k in keys if k in UpgradingStub.UPGRADED]
+    assert 1 <= srv.upgrades <= len(up)
+    for k in keys:                                # a second plan only for the upgraded patterns
+        assert be.builds[k] == (2 if k in up else 1) or (k in up and be.builds[k] == 1)
+    assert srv.reserved_bytes() == 0
+    assert all(v == 1 for v in be.freed.values()) and not be.live
