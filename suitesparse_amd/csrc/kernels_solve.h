@@ -600,24 +600,34 @@ __global__ __launch_bounds__(256) void k_sweep_quad_fwd(const SweepTask *__restr
         lr[c] = (q < below && c < nb) ? W[t.lx + nb + q + (long long)c * t.nsrow] : (T)0;
     }
     const long long g = q < below ? (long long)Ls[t.psi + nb + q] : 0;
-    for (int z = 0; z < nrhs; z++) {
-        T *xz = X + (long long)z * n;
-        ys[grp][q] = q < nb ? xz[t.k1 + q] : (T)0;
-        wave_sync_lds();
-        T y = 0;
+    for (int z0 = 0; z0 < nrhs; z0 += ZS) {
+        // the pass's right-hand-side entries, all loads in flight before the
+        // first LDS fence (each fence would otherwise wait for the next load)
+        T xv[ZS];
 #pragma unroll
-        for (int c = 0; c < QUAD_COLS; c++) y += iv[c] * ys[grp][c];
-        wave_sync_lds();
-        if (q < nb) xz[t.k1 + q] = y;
-        ys[grp][q] = y;
-        wave_sync_lds();
-        if (q < below) {
-            T a = 0;
+        for (int zz = 0; zz < ZS; zz++)
+            xv[zz] = (q < nb && z0 + zz < nrhs) ? X[(long long)(z0 + zz) * n + t.k1 + q] : (T)0;
 #pragma unroll
-            for (int c = 0; c < QUAD_COLS; c++) a += lr[c] * ys[grp][c];
-            atomic_add(xz + g, -a);
+        for (int zz = 0; zz < ZS; zz++) {
+            if (z0 + zz >= nrhs) break;
+            T *xz = X + (long long)(z0 + zz) * n;
+            ys[grp][q] = xv[zz];
+            wave_sync_lds();
+            T y = 0;
+#pragma unroll
+            for (int c = 0; c < QUAD_COLS; c++) y += iv[c] * ys[grp][c];
+            wave_sync_lds();
+            if (q < nb) xz[t.k1 + q] = y;
+            ys[grp][q] = y;
+            wave_sync_lds();
+            if (q < below) {
+                T a = 0;
+#pragma unroll
+                for (int c = 0; c < QUAD_COLS; c++) a += lr[c] * ys[grp][c];
+                atomic_add(xz + g, -a);
+            }
+            wave_sync_lds();
         }
-        wave_sync_lds();
     }
 }
 
@@ -640,22 +650,35 @@ __global__ __launch_bounds__(256) void k_sweep_quad_bwd(const SweepTask *__restr
 #pragma unroll
     for (int r = 0; r < QUAD_ROWS; r++) lc[r] = (q < nb && r < below) ? W[t.lx + nb + r + (long long)q * t.nsrow] : (T)0;
     const long long g = q < below ? (long long)Ls[t.psi + nb + q] : 0;
-    for (int z = 0; z < nrhs; z++) {
-        T *xz = X + (long long)z * n;
-        vs[grp][q] = q < below ? xz[g] : (T)0;       // ancestor rows: final before the launch
-        wave_sync_lds();
-        T p = 0;
+    for (int z0 = 0; z0 < nrhs; z0 += ZS) {
+        // the pass's entries (ancestor rows: final before the launch), all
+        // loads in flight before the first LDS fence
+        T xa[ZS], xo[ZS];
 #pragma unroll
-        for (int r = 0; r < QUAD_ROWS; r++) p += lc[r] * vs[grp][r];
-        const T v = q < nb ? xz[t.k1 + q] - p : (T)0;
-        wave_sync_lds();
-        vs[grp][q] = v;
-        wave_sync_lds();
-        T y = 0;
+        for (int zz = 0; zz < ZS; zz++) {
+            const bool zin = z0 + zz < nrhs;
+            xa[zz] = (q < below && zin) ? X[(long long)(z0 + zz) * n + g] : (T)0;
+            xo[zz] = (q < nb && zin) ? X[(long long)(z0 + zz) * n + t.k1 + q] : (T)0;
+        }
 #pragma unroll
-        for (int i = 0; i < QUAD_COLS; i++) y += iv[i] * vs[grp][i];
-        if (q < nb) xz[t.k1 + q] = y;
-        wave_sync_lds();
+        for (int zz = 0; zz < ZS; zz++) {
+            if (z0 + zz >= nrhs) break;
+            T *xz = X + (long long)(z0 + zz) * n;
+            vs[grp][q] = xa[zz];
+            wave_sync_lds();
+            T p = 0;
+#pragma unroll
+            for (int r = 0; r < QUAD_ROWS; r++) p += lc[r] * vs[grp][r];
+            const T v = q < nb ? xo[zz] - p : (T)0;
+            wave_sync_lds();
+            vs[grp][q] = v;
+            wave_sync_lds();
+            T y = 0;
+#pragma unroll
+            for (int i = 0; i < QUAD_COLS; i++) y += iv[i] * vs[grp][i];
+            if (q < nb) xz[t.k1 + q] = y;
+            wave_sync_lds();
+        }
     }
 }
 
