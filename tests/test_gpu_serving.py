@@ -66,3 +66,22 @@ def test_admission_under_tight_budget(gpu):
     for r in res.values():
         R = r.B - r.A.matvec(r.X)
         assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(r.B, axis=0)) <= 1e-10
+
+
+def test_background_metis_plan_replaces_amd(gpu):
+    """A 3D pattern is first served from an AMD plan; its METIS plan, built in
+    the background, replaces it for the later requests of the pattern, and
+    every request still converges."""
+    from suitesparse_amd import problems
+    A0 = problems.grid3d_lognormal(12, np.random.default_rng(3))
+    rng = np.random.default_rng(8)
+    reqs = [serving.Request(rid=k, A=problems.revalue(A0, np.random.default_rng(50 + k)),
+                            B=rng.uniform(-1, 1, (A0.n, 2)), key="3d_12") for k in range(10)]
+    srv = serving.Server(slots=2, hbm_budget=16 << 30, precision="fp32", tol=1e-10)
+    stats = serving.run_stream(srv, reqs, np.arange(len(reqs)) * 0.1)
+    print(stats, flush=True)
+    assert stats["ok"] == len(reqs)
+    assert srv.upgrades == 1 and srv.builds == 1
+    for r in srv.completed:
+        R = r.B - r.A.matvec(r.X)
+        assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(r.B, axis=0)) <= 1e-10
